@@ -1,0 +1,379 @@
+/*
+ * ref_harness.c -- drives the reference's own hot-path C files, compiled
+ * from /root/reference/src by oracle/Makefile into oracle/_ref/.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Used to pin the oracle restatement
+ * (ifos_oracle.c) and to write the golden fixtures under tests/golden/.
+ *
+ * The reference kernels (update_v_PML, update_s_elastic_PML,
+ * update_s_visc_PML, psource, surface_elastic_PML, surface_PML, seismo_ssg,
+ * holbergcoeff, PML_pro, matcopy*, av_*, prepare_update_s, wavelet,
+ * zero_fdveps*) are called unmodified, with the globals they read defined by
+ * the reference's own globvar.h.  The per-step call order restates
+ * src/IFOS2D.c:1651-1803 (forward) and :2224-2419 (adjoint); the snapshot
+ * store, zero-lag correlation and chain rule are inline code in the
+ * reference's main() and are restated here the same way as in the oracle.
+ * Residual computation (calc_res) needs FFTW at link time in the reference,
+ * which is absent, so the adjoint sources are passed in by the caller.
+ *
+ * Single MPI rank (MPICH from the image): INDEX[] all point to rank 0 and
+ * POS = (0,0), as initproc() sets them for NPROCX = NPROCY = 1.
+ */
+#include "fd.h"
+#include "globvar.h"
+
+static int harness_mpi_ready = 0;
+static char *harness_bsend_buf = NULL;
+
+static void harness_setup(const int *ip, const float *fp)
+{
+    if (!harness_mpi_ready) {
+        int inited = 0;
+        MPI_Initialized(&inited);
+        if (!inited) MPI_Init(NULL, NULL);
+        int sz = 1 << 24;
+        harness_bsend_buf = (char *)malloc((size_t)sz);
+        MPI_Buffer_attach(harness_bsend_buf, sz);
+        harness_mpi_ready = 1;
+    }
+    /* ip / fp mirror orc_params field order (oracle/ifos_oracle.h) */
+    NX = ip[0]; NY = ip[1]; FDORDER = ip[2]; MAXRELERROR = ip[3]; FW = ip[4];
+    FREE_SURF = ip[5]; BOUNDARY = ip[6]; L = ip[7]; VELOCITY = ip[8];
+    ADJOINT_TYPE = ip[9]; PARAMETERIZATION = ip[10]; NT = ip[12];
+    SOURCE_SHAPE = ip[13]; SOURCE_TYPE = ip[14];
+    DH = fp[0]; DT = fp[1]; VPPML = fp[2]; FPML = fp[3]; npower = fp[4];
+    k_max_PML = fp[5]; F_REF = fp[6]; TAU = fp[7];
+    NXG = NX; NYG = NY; IENDX = NX; IENDY = NY;
+    NPROCX = NPROCY = 1; NPROC = 1; MYID = 0;
+    POS[1] = POS[2] = 0;
+    INDEX[1] = INDEX[2] = INDEX[3] = INDEX[4] = 0;
+    WAVETYPE = 1; ACOUSTIC = 0; VERBOSE = 0; SEISMO = 1; NDT = 1;
+    IDX = IDY = 1;
+    FP = stderr;
+}
+
+float ref_hc[7];
+
+void ref_holbergcoeff(const int *ip, const float *fp, float *hc_out)
+{
+    harness_setup(ip, fp);
+    float *hc = holbergcoeff();
+    for (int k = 0; k < 7; k++) hc_out[k] = hc[k];
+    free_vector(hc, 0, 6);
+}
+
+typedef struct {
+    float *d_x, *K_x, *al_x, *a_x, *b_x, *d_xh, *K_xh, *al_xh, *a_xh, *b_xh;
+    float *d_y, *K_y, *al_y, *a_y, *b_y, *d_yh, *K_yh, *al_yh, *a_yh, *b_yh;
+} pml_t;
+
+static void pml_make(pml_t *P)
+{
+    int n = 2 * FW;
+    float **v[20] = {&P->d_x, &P->K_x, &P->al_x, &P->a_x, &P->b_x, &P->d_xh, &P->K_xh, &P->al_xh, &P->a_xh, &P->b_xh,
+                     &P->d_y, &P->K_y, &P->al_y, &P->a_y, &P->b_y, &P->d_yh, &P->K_yh, &P->al_yh, &P->a_yh, &P->b_yh};
+    for (int k = 0; k < 20; k++) *v[k] = vector(1, n > 0 ? n : 1);
+    if (FW > 0)
+        PML_pro(P->d_x, P->K_x, P->al_x, P->a_x, P->b_x, P->d_xh, P->K_xh, P->al_xh, P->a_xh, P->b_xh,
+                P->d_y, P->K_y, P->al_y, P->a_y, P->b_y, P->d_yh, P->K_yh, P->al_yh, P->a_yh, P->b_yh);
+}
+
+static void pml_free(pml_t *P)
+{
+    int n = 2 * FW;
+    float *v[20] = {P->d_x, P->K_x, P->al_x, P->a_x, P->b_x, P->d_xh, P->K_xh, P->al_xh, P->a_xh, P->b_xh,
+                    P->d_y, P->K_y, P->al_y, P->a_y, P->b_y, P->d_yh, P->K_yh, P->al_yh, P->a_yh, P->b_yh};
+    for (int k = 0; k < 20; k++) free_vector(v[k], 1, n > 0 ? n : 1);
+}
+
+void ref_pml_pro(const int *ip, const float *fp, float *out12)
+{
+    harness_setup(ip, fp);
+    pml_t P;
+    pml_make(&P);
+    int n = 2 * FW;
+    float *src[12] = {P.K_x, P.a_x, P.b_x, P.K_xh, P.a_xh, P.b_xh, P.K_y, P.a_y, P.b_y, P.K_yh, P.a_yh, P.b_yh};
+    for (int k = 0; k < 12; k++)
+        for (int i = 1; i <= n; i++) out12[k * n + i - 1] = src[k][i];
+    pml_free(&P);
+}
+
+/* wavelet(): srcpos rows as in the reference srcpos_loc[1..8][1..nsrc] */
+void ref_wavelet(const int *ip, const float *fp, int nsrc, const float *srcpos, float *out)
+{
+    harness_setup(ip, fp);
+    float **sp = matrix(1, 8, 1, nsrc);
+    for (int l = 1; l <= nsrc; l++)
+        for (int k = 1; k <= 8; k++) sp[k][l] = srcpos[(l - 1) * 8 + k - 1];
+    float **sig = wavelet(sp, nsrc, 1, 0, 0);
+    for (int l = 1; l <= nsrc; l++)
+        for (int nt = 1; nt <= NT; nt++) out[(size_t)(l - 1) * NT + nt - 1] = sig[l][nt];
+    free_matrix(sp, 1, 8, 1, nsrc);
+}
+
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int nd, nsnap;
+    float *hc;
+    pml_t P;
+    float **prho, **ppi, **pu, **ptaus, **ptaup, *peta, *etaip, *etajm;
+    float **prip, **prjp, **puipjp, **ptausipjp, **fipjp, **f, **g;
+    float *bip, *bjm, *cip, *cjm;
+    float ***dip, ***d, ***e, ***pr, ***pp, ***pq;
+    float **pvx, **pvy, **psxx, **psyy, **psxy, **ux, **uy, **uxy, **uyx, **pvxp1, **pvyp1;
+    float **pvz, **psxz, **psyz, **uyz, **uxz, **pvxm1, **pvym1, **uttx, **utty;
+    float ***pt, ***po;
+    float **psi_sxx_x, **psi_sxy_x, **psi_vxx, **psi_vyx, **psi_vxxs, **psi_sxz_x, **psi_vzx;
+    float **psi_syy_y, **psi_sxy_y, **psi_vyy, **psi_vxy, **psi_syz_y, **psi_vzy;
+} run_t;
+
+static void run_init(run_t *R, const float *vp, const float *vs, const float *rho, const float *taus,
+                     const float *taup, const float *FLv)
+{
+    int nd = FDORDER / 2 + 1;
+    R->nd = nd;
+    R->hc = holbergcoeff();
+    pml_make(&R->P);
+#define GRID matrix(-nd + 1, NY + nd, -nd + 1, NX + nd)
+#define TENS(n) f3tensor(-nd + 1, NY + nd, -nd + 1, NX + nd, 1, n)
+    R->prho = GRID; R->ppi = GRID; R->pu = GRID;
+    R->prip = GRID; R->prjp = GRID; R->puipjp = GRID;
+    for (int j = 1; j <= NY; j++)
+        for (int i = 1; i <= NX; i++) {
+            size_t k = (size_t)(j - 1) * NX + (i - 1);
+            R->prho[j][i] = rho[k]; R->ppi[j][i] = vp[k]; R->pu[j][i] = vs[k];
+        }
+    int nl = L > 0 ? L : 1;
+    if (L) {
+        FL = vector(1, L);
+        for (int l = 1; l <= L; l++) FL[l] = FLv[l];
+        R->ptaus = GRID; R->ptaup = GRID; R->ptausipjp = GRID;
+        R->fipjp = GRID; R->f = GRID; R->g = GRID;
+        R->peta = vector(1, L); R->etaip = vector(1, L); R->etajm = vector(1, L);
+        R->bip = vector(1, L); R->bjm = vector(1, L); R->cip = vector(1, L); R->cjm = vector(1, L);
+        R->dip = TENS(L); R->d = TENS(L); R->e = TENS(L);
+        for (int j = 1; j <= NY; j++)
+            for (int i = 1; i <= NX; i++) {
+                size_t k = (size_t)(j - 1) * NX + (i - 1);
+                R->ptaus[j][i] = taus[k]; R->ptaup[j][i] = taup[k];
+            }
+        /* readmod: eta = DT / tau_sigma (src/readmod.c:47-51) */
+        for (int l = 1; l <= L; l++) {
+            float pts = 1.0 / (2.0 * PI * FL[l]);
+            R->peta[l] = DT / pts;
+        }
+        matcopy(R->prho, R->ppi, R->pu, R->ptaus, R->ptaup);
+    } else {
+        matcopy_elastic(R->prho, R->ppi, R->pu);
+    }
+    av_mue(R->pu, R->puipjp, R->prho);
+    av_rho(R->prho, R->prip, R->prjp);
+    if (L) {
+        av_tau(R->ptaus, R->ptausipjp);
+        prepare_update_s(R->etajm, R->etaip, R->peta, R->fipjp, R->pu, R->puipjp, R->ppi, R->prho, R->ptaus,
+                         R->ptaup, R->ptausipjp, R->f, R->g, R->bip, R->bjm, R->cip, R->cjm, R->dip, R->d, R->e);
+    }
+    R->pvx = GRID; R->pvy = GRID; R->psxx = GRID; R->psyy = GRID; R->psxy = GRID;
+    R->ux = GRID; R->uy = GRID; R->uxy = GRID; R->uyx = GRID; R->pvxp1 = GRID; R->pvyp1 = GRID;
+    R->pvz = GRID; R->psxz = GRID; R->psyz = GRID; R->uyz = GRID; R->uxz = GRID;
+    R->pvxm1 = GRID; R->pvym1 = GRID; R->uttx = GRID; R->utty = GRID;
+    R->pr = TENS(nl); R->pp = TENS(nl); R->pq = TENS(nl); R->pt = TENS(nl); R->po = TENS(nl);
+    int fw2 = FW > 0 ? 2 * FW : 1;
+    R->psi_sxx_x = matrix(1, NY, 1, fw2); R->psi_sxy_x = matrix(1, NY, 1, fw2);
+    R->psi_vxx = matrix(1, NY, 1, fw2); R->psi_vyx = matrix(1, NY, 1, fw2);
+    R->psi_vxxs = matrix(1, NY, 1, fw2); R->psi_sxz_x = matrix(1, NY, 1, fw2); R->psi_vzx = matrix(1, NY, 1, fw2);
+    R->psi_syy_y = matrix(1, fw2, 1, NX); R->psi_sxy_y = matrix(1, fw2, 1, NX);
+    R->psi_vyy = matrix(1, fw2, 1, NX); R->psi_vxy = matrix(1, fw2, 1, NX);
+    R->psi_syz_y = matrix(1, fw2, 1, NX); R->psi_vzy = matrix(1, fw2, 1, NX);
+#undef GRID
+#undef TENS
+}
+
+static void run_zero(run_t *R)
+{
+    int nd = R->nd;
+    if (L)
+        zero_fdveps_visc(-nd + 1, NY + nd, -nd + 1, NX + nd, R->pvx, R->pvy, R->pvz, R->psxx, R->psyy, R->psxy,
+                         R->psxz, R->psyz, R->ux, R->uy, R->uxy, R->pvxp1, R->pvyp1, R->psi_sxx_x, R->psi_sxy_x,
+                         R->psi_sxz_x, R->psi_vxx, R->psi_vyx, R->psi_vzx, R->psi_syy_y, R->psi_sxy_y,
+                         R->psi_syz_y, R->psi_vyy, R->psi_vxy, R->psi_vzy, R->psi_vxxs, R->pr, R->pp, R->pq,
+                         R->pt, R->po);
+    else
+        zero_fdveps(-nd + 1, NY + nd, -nd + 1, NX + nd, R->pvx, R->pvy, R->pvz, R->psxx, R->psyy, R->psxy,
+                    R->psxz, R->psyz, R->ux, R->uy, R->uxy, R->pvxp1, R->pvyp1, R->psi_sxx_x, R->psi_sxy_x,
+                    R->psi_sxz_x, R->psi_vxx, R->psi_vyx, R->psi_vzx, R->psi_syy_y, R->psi_sxy_y, R->psi_syz_y,
+                    R->psi_vyy, R->psi_vxy, R->psi_vzy, R->psi_vxxs);
+}
+
+/* one time step, driver order of src/IFOS2D.c:1682-1800 */
+static void run_step(run_t *R, int nt, int sw, float **srcpos_loc, float **sig, float **sig1, int nsrc)
+{
+    pml_t *P = &R->P;
+    update_v_PML(1, NX, 1, NY, nt, R->pvx, R->pvxp1, R->pvxm1, R->pvy, R->pvyp1, R->pvym1, R->uttx, R->utty,
+                 R->psxx, R->psyy, R->psxy, R->prip, R->prjp, srcpos_loc, sig, sig1, nsrc, NULL, R->hc, 0, sw,
+                 P->K_x, P->a_x, P->b_x, P->K_xh, P->a_xh, P->b_xh, P->K_y, P->a_y, P->b_y, P->K_yh, P->a_yh,
+                 P->b_yh, R->psi_sxx_x, R->psi_syy_y, R->psi_sxy_y, R->psi_sxy_x);
+    if (L)
+        update_s_visc_PML(1, NX, 1, NY, R->pvx, R->pvy, R->ux, R->uy, R->uxy, R->uyx, R->psxx, R->psyy, R->psxy,
+                          R->ppi, R->pu, R->puipjp, R->prho, R->hc, 0, R->pr, R->pp, R->pq, R->fipjp, R->f, R->g,
+                          R->bip, R->bjm, R->cip, R->cjm, R->d, R->e, R->dip, P->K_x, P->a_x, P->b_x, P->K_xh,
+                          P->a_xh, P->b_xh, P->K_y, P->a_y, P->b_y, P->K_yh, P->a_yh, P->b_yh, R->psi_vxx,
+                          R->psi_vyy, R->psi_vxy, R->psi_vyx);
+    else
+        update_s_elastic_PML(1, NX, 1, NY, R->pvx, R->pvy, R->ux, R->uy, R->uxy, R->uyx, R->psxx, R->psyy,
+                             R->psxy, R->ppi, R->pu, R->puipjp, NULL, R->prho, R->hc, 0, P->K_x, P->a_x, P->b_x,
+                             P->K_xh, P->a_xh, P->b_xh, P->K_y, P->a_y, P->b_y, P->K_yh, P->a_yh, P->b_yh,
+                             R->psi_vxx, R->psi_vyy, R->psi_vxy, R->psi_vyx);
+    if (sw == 0 && SOURCE_TYPE == 1) psource(nt, R->psxx, R->psyy, NULL, srcpos_loc, sig, nsrc, 0);
+    if (FREE_SURF) {
+        if (L)
+            surface_PML(1, R->pvx, R->pvy, R->psxx, R->psyy, R->psxy, R->psyz, R->pp, R->pq, R->ppi, R->pu,
+                        R->prho, R->ptaup, R->ptaus, R->etajm, R->peta, R->hc, P->K_x, P->a_x, P->b_x, R->psi_vxxs,
+                        R->ux, R->uy, R->uxy, R->uyz, R->psxz, R->uxz);
+        else
+            surface_elastic_PML(1, R->pvx, R->pvy, R->psxx, R->psyy, R->psxy, R->psyz, R->ppi, R->pu, R->prho,
+                                R->hc, P->K_x, P->a_x, P->b_x, R->psi_vxxs, R->ux, R->uy, R->uxy, R->uyz, R->psxz,
+                                R->uxz);
+    }
+}
+
+static void copy_interior(float **a, float *out)
+{
+    for (int j = 1; j <= NY; j++)
+        for (int i = 1; i <= NX; i++) out[(size_t)(j - 1) * NX + (i - 1)] = a[j][i];
+}
+
+/* forward run: seismograms [ntr][NT], optional final wavefields, optional
+ * snapshots (5 fields x nsnap planes of [NY][NX]) */
+static void run_forward(run_t *R, int nsrc, const float *srcpos, int ntr, const int *recpos, float *seis_vx,
+                        float *seis_vy, float **snap, int *help, int dtinv)
+{
+    float **sp = matrix(1, 8, 1, nsrc);
+    for (int l = 1; l <= nsrc; l++)
+        for (int k = 1; k <= 8; k++) sp[k][l] = srcpos[(l - 1) * 8 + k - 1];
+    float **signals = wavelet(sp, nsrc, 1, 0, 0);
+    int **rp = imatrix(1, 3, 1, ntr > 0 ? ntr : 1);
+    for (int t = 1; t <= ntr; t++) { rp[1][t] = recpos[2 * (t - 1)]; rp[2][t] = recpos[2 * (t - 1) + 1]; rp[3][t] = t; }
+    float **svx = matrix(1, ntr > 0 ? ntr : 1, 1, NT), **svy = matrix(1, ntr > 0 ? ntr : 1, 1, NT);
+    run_zero(R);
+    int hin = 1, hin1 = 1;
+    for (int nt = 1; nt <= NT; nt++) {
+        run_step(R, nt, 0, sp, signals, signals, nsrc);
+        seismo_ssg(nt, ntr, rp, svx, svy, NULL, NULL, NULL, NULL, R->pvx, R->pvy, NULL, R->psxx, R->psyy, NULL,
+                   R->ppi, R->pu, R->hc);
+        if (nt == hin1) {
+            if (snap) {
+                size_t off = (size_t)(hin - 1) * NX * NY;
+                copy_interior(R->pvxp1, snap[0] + off);
+                copy_interior(R->pvyp1, snap[1] + off);
+                copy_interior(VELOCITY ? R->ux : R->psxx, snap[2] + off);
+                copy_interior(VELOCITY ? R->uy : R->psyy, snap[3] + off);
+                copy_interior(VELOCITY ? R->uxy : R->psxy, snap[4] + off);
+            }
+            hin++;
+            hin1 += dtinv;
+            help[nt] = 1;
+        }
+    }
+    for (int t = 1; t <= ntr; t++)
+        for (int nt = 1; nt <= NT; nt++) {
+            seis_vx[(size_t)(t - 1) * NT + nt - 1] = svx[t][nt];
+            seis_vy[(size_t)(t - 1) * NT + nt - 1] = svy[t][nt];
+        }
+    free_matrix(sp, 1, 8, 1, nsrc);
+}
+
+int ref_forward(const int *ip, const float *fp, const float *FLv, const float *vp, const float *vs, const float *rho,
+                const float *taus, const float *taup, int nsrc, const float *srcpos, int ntr, const int *recpos,
+                float *seis_vx, float *seis_vy, float *final5)
+{
+    harness_setup(ip, fp);
+    run_t R;
+    run_init(&R, vp, vs, rho, taus, taup, FLv);
+    int *help = (int *)calloc((size_t)NT + 2, sizeof(int));
+    run_forward(&R, nsrc, srcpos, ntr, recpos, seis_vx, seis_vy, NULL, help, ip[11] > 0 ? ip[11] : 1);
+    if (final5) {
+        size_t pl = (size_t)NX * NY;
+        copy_interior(R.pvx, final5);
+        copy_interior(R.pvy, final5 + pl);
+        copy_interior(R.psxx, final5 + 2 * pl);
+        copy_interior(R.psyy, final5 + 3 * pl);
+        copy_interior(R.psxy, final5 + 4 * pl);
+    }
+    free(help);
+    return 0;
+}
+
+/* forward with snapshots, then the adjoint run driven by the given
+ * time-reversed residuals diff_vx/diff_vy [ntr][NT], correlation and chain
+ * rule.  Outputs per-shot gradients [NY][NX]. */
+int ref_gradient(const int *ip, const float *fp, const float *FLv, const float *vp, const float *vs,
+                 const float *rho, const float *taus, const float *taup, int nsrc, const float *srcpos, int ntr,
+                 const int *recpos, const float *diff_vx, const float *diff_vy, float *seis_vx, float *seis_vy,
+                 float *grad_vp, float *grad_vs, float *grad_rho)
+{
+    harness_setup(ip, fp);
+    int DTINV = ip[11];
+    run_t R;
+    run_init(&R, vp, vs, rho, taus, taup, FLv);
+    int nsnap = NT / DTINV + (NT % DTINV ? 1 : 0);
+    size_t pl = (size_t)NX * NY;
+    float *snap[5];
+    for (int k = 0; k < 5; k++) snap[k] = (float *)calloc((size_t)nsnap * pl, sizeof(float));
+    int *help = (int *)calloc((size_t)NT + 2, sizeof(int));
+    run_forward(&R, nsrc, srcpos, ntr, recpos, seis_vx, seis_vy, snap, help, DTINV);
+
+    float **srcpos_loc_back = matrix(1, 6, 1, ntr);
+    for (int t = 1; t <= ntr; t++) {
+        srcpos_loc_back[1][t] = recpos[2 * (t - 1)];
+        srcpos_loc_back[2][t] = recpos[2 * (t - 1) + 1];
+    }
+    float **dvx = matrix(1, ntr, 1, NT), **dvy = matrix(1, ntr, 1, NT);
+    for (int t = 1; t <= ntr; t++)
+        for (int nt = 1; nt <= NT; nt++) {
+            dvx[t][nt] = diff_vx[(size_t)(t - 1) * NT + nt - 1];
+            dvy[t][nt] = diff_vy[(size_t)(t - 1) * NT + nt - 1];
+        }
+    float *wc = (float *)calloc(pl, sizeof(float)), *wcu = (float *)calloc(pl, sizeof(float));
+    float *wcr = (float *)calloc(pl, sizeof(float));
+    run_zero(&R);
+    int hin = 1;
+    for (int nt = 1; nt <= NT; nt++) {
+        run_step(&R, nt, 1, srcpos_loc_back, dvx, dvy, ntr);
+        if (help[NT - nt + 1] == 1) {
+            size_t off = (size_t)(nsnap - hin) * pl;
+            for (int j = 1; j <= NY; j++)
+                for (int i = 1; i <= NX; i++) {
+                    size_t c = (size_t)(j - 1) * NX + (i - 1);
+                    float Frx = snap[0][off + c], Fry = snap[1][off + c];
+                    float Fx = snap[2][off + c], Fy = snap[3][off + c], Fu = snap[4][off + c];
+                    float sxx = R.psxx[j][i], syy = R.psyy[j][i], sxy = R.psxy[j][i];
+                    float muss, lamss;
+                    wcr[c] += (R.pvxp1[j][i] * Frx) + (R.pvyp1[j][i] * Fry);
+                    wc[c] += (Fx + Fy) * (sxx + syy);
+                    muss = R.prho[j][i] * R.pu[j][i] * R.pu[j][i];
+                    lamss = R.prho[j][i] * R.ppi[j][i] * R.ppi[j][i] - 2.0 * muss;
+                    if (R.pu[j][i] > 0.0)
+                        wcu[c] += ((1.0 / (muss * muss)) * (Fu * sxy)) +
+                                  (.25 * (Fx + Fy * (sxx + syy)) / ((lamss + muss) * (lamss + muss))) +
+                                  (.25 * (Fx - Fy * (sxx - syy)) / (muss * muss));
+                }
+            hin++;
+        }
+    }
+    for (int j = 1; j <= NY; j++)
+        for (int i = 1; i <= NX; i++) {
+            size_t c = (size_t)(j - 1) * NX + (i - 1);
+            float rh = R.prho[j][i], u = R.pu[j][i], pv = R.ppi[j][i];
+            float wl = -DT * wc[c], muss = rh * u * u, lamss = rh * pv * pv - 2.0 * muss;
+            wl = (1.0 / (4.0 * (lamss + muss) * (lamss + muss))) * wl;
+            grad_vp[c] = 2.0 * pv * rh * wl;
+            float wmu = -DT * wcu[c];
+            grad_vs[c] = -4.0 * rh * u * wl + 2.0 * rh * u * wmu;
+            float wrs = -DT * wcr[c];
+            grad_rho[c] = ((pv * pv) - (2.0 * u * u)) * wl + u * u * wmu + wrs;
+        }
+    free(wc); free(wcu); free(wcr); free(help);
+    for (int k = 0; k < 5; k++) free(snap[k]);
+    return 0;
+}

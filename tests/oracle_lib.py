@@ -1,0 +1,197 @@
+"""ctypes access to the checkers under oracle/ (test infrastructure).
+
+liboracle.so / liboracle64.so are the CPU restatement (oracle/ifos_oracle.c);
+_ref/libifosref.so drives the reference's own compiled hot-path files
+(oracle/ref_harness.c).  Only tests, smoke() and bench.py's cpu_baseline leg
+use this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+_INT_FIELDS = ["NX", "NY", "FDORDER", "MAXRELERROR", "FW", "FREE_SURF", "BOUNDARY", "L",
+               "VELOCITY", "ADJOINT_TYPE", "PARAMETERIZATION", "DTINV", "NT",
+               "SOURCE_SHAPE", "SOURCE_TYPE", "TRKILL"]
+_FLT_FIELDS = ["DH", "DT", "VPPML", "FPML", "npower", "k_max_PML", "F_REF", "TAU"]
+
+
+class OrcParams(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [(n, C.c_float) for n in _FLT_FIELDS]
+                + [("FL", C.c_float * 8)])
+
+
+def make_params(**kw):
+    p = OrcParams()
+    defaults = dict(MAXRELERROR=1, FW=10, FREE_SURF=1, BOUNDARY=0, L=0, VELOCITY=0,
+                    ADJOINT_TYPE=1, PARAMETERIZATION=1, DTINV=1, SOURCE_SHAPE=1, SOURCE_TYPE=1,
+                    TRKILL=0, VPPML=3500.0, FPML=30.0, npower=2.0, k_max_PML=1.0, F_REF=0.0, TAU=0.0)
+    defaults.update(kw)
+    fl = defaults.pop("FL", None)
+    for k, v in defaults.items():
+        setattr(p, k, v)
+    if fl is not None:
+        for i, v in enumerate(fl):
+            p.FL[i + 1] = v
+    return p
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, ty=C.c_float):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ty))
+
+
+class _Lib:
+    def __init__(self, path):
+        self.lib = C.CDLL(path)
+
+
+def oracle(fp64=False):
+    path = os.path.join(ORACLE_DIR, "liboracle64.so" if fp64 else "liboracle.so")
+    if not os.path.exists(path):
+        raise RuntimeError("oracle not built: run make -C oracle")
+    return C.CDLL(path)
+
+
+def ref_available():
+    return os.path.exists(os.path.join(ORACLE_DIR, "_ref", "libifosref.so"))
+
+
+_REF = None
+
+
+def ref():
+    global _REF
+    if _REF is None:
+        _REF = C.CDLL(os.path.join(ORACLE_DIR, "_ref", "libifosref.so"))
+    return _REF
+
+
+def _ip_fp(p):
+    base = C.addressof(p)
+    ip = C.cast(base, C.POINTER(C.c_int))
+    fp = C.cast(base + 4 * len(_INT_FIELDS), C.POINTER(C.c_float))
+    fl = C.cast(base + 4 * (len(_INT_FIELDS) + len(_FLT_FIELDS)), C.POINTER(C.c_float))
+    return ip, fp, fl
+
+
+# ---------------------------------------------------------------------------
+def holberg(FDORDER, MAXRELERROR, which="oracle"):
+    out = np.zeros(7, np.float32)
+    if which == "oracle":
+        oracle().orc_holbergcoeff(C.c_int(FDORDER), C.c_int(MAXRELERROR), _ptr(out))
+    else:
+        p = make_params(NX=10, NY=10, FDORDER=FDORDER, MAXRELERROR=MAXRELERROR, NT=1, DH=1.0, DT=1e-4)
+        ip, fp, _ = _ip_fp(p)
+        ref().ref_holbergcoeff(ip, fp, _ptr(out))
+    return out
+
+
+def pml_pro(p, which="oracle"):
+    out = np.zeros((12, 2 * p.FW), np.float32)
+    if which == "oracle":
+        oracle().orc_pml_pro(C.byref(p), _ptr(out))
+    else:
+        ip, fp, _ = _ip_fp(p)
+        ref().ref_pml_pro(ip, fp, _ptr(out))
+    return out
+
+
+def wavelet(p, srcpos, which="oracle"):
+    srcpos = _f32(srcpos).reshape(-1, 8)
+    n = srcpos.shape[0]
+    out = np.zeros((n, p.NT), np.float32)
+    if which == "oracle":
+        oracle().orc_wavelet(C.byref(p), C.c_int(n), _ptr(srcpos), _ptr(out))
+    else:
+        ip, fp, _ = _ip_fp(p)
+        ref().ref_wavelet(ip, fp, C.c_int(n), _ptr(srcpos), _ptr(out))
+    return out
+
+
+def forward(p, model, srcpos, recpos, which="oracle", fp64=False):
+    """Returns (seis_vx, seis_vy, final5) with final5 = vx,vy,sxx,syy,sxy interiors."""
+    vp, vs, rho = (_f32(model[k]) for k in ("vp", "vs", "rho"))
+    taus = _f32(model["taus"]) if p.L else None
+    taup = _f32(model["taup"]) if p.L else None
+    srcpos = _f32(srcpos).reshape(-1, 8)
+    recpos = np.ascontiguousarray(recpos, dtype=np.int32).reshape(-1, 2)
+    ntr = recpos.shape[0]
+    dt = np.float64 if fp64 else np.float32
+    ty = C.c_double if fp64 else C.c_float
+    svx = np.zeros((ntr, p.NT), dt)
+    svy = np.zeros((ntr, p.NT), dt)
+    fin = np.zeros((5, p.NY, p.NX), dt)
+    if which == "oracle":
+        rc = oracle(fp64).orc_forward(C.byref(p), _ptr(vp), _ptr(vs), _ptr(rho), _ptr(taus), _ptr(taup),
+                                      C.c_int(srcpos.shape[0]), _ptr(srcpos), C.c_int(ntr),
+                                      _ptr(recpos, C.c_int), _ptr(svx, ty), _ptr(svy, ty), _ptr(fin, ty))
+    else:
+        ip, fp, fl = _ip_fp(p)
+        rc = ref().ref_forward(ip, fp, fl, _ptr(vp), _ptr(vs), _ptr(rho), _ptr(taus), _ptr(taup),
+                               C.c_int(srcpos.shape[0]), _ptr(srcpos), C.c_int(ntr),
+                               _ptr(recpos, C.c_int), _ptr(svx), _ptr(svy), _ptr(fin))
+    assert rc == 0, rc
+    return svx, svy, fin
+
+
+def calc_res(p, syn, obs, kill=None):
+    syn = _f32(syn)
+    obs = _f32(obs)
+    ntr = syn.shape[0]
+    diff = np.zeros_like(syn)
+    k = None if kill is None else np.ascontiguousarray(kill, dtype=np.int32)
+    oracle().orc_calc_res.restype = C.c_double
+    l2 = oracle().orc_calc_res(C.byref(p), C.c_int(ntr), _ptr(syn), _ptr(obs), _ptr(k, C.c_int), _ptr(diff))
+    return diff, l2
+
+
+def gradient(p, model, srcpos, recpos, obs_vx, obs_vy, kill=None, fp64=False):
+    """Oracle forward+adjoint+gradient.  Returns dict."""
+    vp, vs, rho = (_f32(model[k]) for k in ("vp", "vs", "rho"))
+    taus = _f32(model["taus"]) if p.L else None
+    taup = _f32(model["taup"]) if p.L else None
+    srcpos = _f32(srcpos).reshape(-1, 8)
+    recpos = np.ascontiguousarray(recpos, dtype=np.int32).reshape(-1, 2)
+    ntr = recpos.shape[0]
+    dt = np.float64 if fp64 else np.float32
+    ty = C.c_double if fp64 else C.c_float
+    svx = np.zeros((ntr, p.NT), dt)
+    svy = np.zeros((ntr, p.NT), dt)
+    g = np.zeros((3, p.NY, p.NX), dt)
+    mis = np.zeros(5, np.float64)
+    k = None if kill is None else np.ascontiguousarray(kill, dtype=np.int32)
+    rc = oracle(fp64).orc_gradient(C.byref(p), _ptr(vp), _ptr(vs), _ptr(rho), _ptr(taus), _ptr(taup),
+                                   C.c_int(srcpos.shape[0]), _ptr(srcpos), C.c_int(ntr), _ptr(recpos, C.c_int),
+                                   _ptr(_f32(obs_vx)), _ptr(_f32(obs_vy)), _ptr(k, C.c_int),
+                                   _ptr(svx, ty), _ptr(svy, ty), _ptr(g[0], ty), _ptr(g[1], ty), _ptr(g[2], ty),
+                                   _ptr(mis, C.c_double))
+    assert rc == 0, rc
+    return dict(seis_vx=svx, seis_vy=svy, grad_vp=g[0], grad_vs=g[1], grad_rho=g[2], misfit=mis)
+
+
+def ref_gradient(p, model, srcpos, recpos, diff_vx, diff_vy):
+    vp, vs, rho = (_f32(model[k]) for k in ("vp", "vs", "rho"))
+    taus = _f32(model["taus"]) if p.L else None
+    taup = _f32(model["taup"]) if p.L else None
+    srcpos = _f32(srcpos).reshape(-1, 8)
+    recpos = np.ascontiguousarray(recpos, dtype=np.int32).reshape(-1, 2)
+    ntr = recpos.shape[0]
+    svx = np.zeros((ntr, p.NT), np.float32)
+    svy = np.zeros((ntr, p.NT), np.float32)
+    g = np.zeros((3, p.NY, p.NX), np.float32)
+    ip, fp, fl = _ip_fp(p)
+    rc = ref().ref_gradient(ip, fp, fl, _ptr(vp), _ptr(vs), _ptr(rho), _ptr(taus), _ptr(taup),
+                            C.c_int(srcpos.shape[0]), _ptr(srcpos), C.c_int(ntr), _ptr(recpos, C.c_int),
+                            _ptr(_f32(diff_vx)), _ptr(_f32(diff_vy)), _ptr(svx), _ptr(svy),
+                            _ptr(g[0]), _ptr(g[1]), _ptr(g[2]))
+    assert rc == 0, rc
+    return dict(seis_vx=svx, seis_vy=svy, grad_vp=g[0], grad_vs=g[1], grad_rho=g[2])
