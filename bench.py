@@ -1,0 +1,168 @@
+"""Benchmark of the P-SV shot gradient on one MI355X.
+
+A step is one shot-gradient of the viscoelastic layered workload
+(2000x800 cells, dh 2 m, FDORDER 8, L=2, Q=20, NT=4500, DTINV=3): forward run
+with on-device snapshots, residual against observed traces already resident
+in HBM, adjoint run with the fused zero-lag correlation, chain rule.  Each
+step uses the next source position of the 64-shot line.
+
+Prints one JSON line (metric, value in shot-gradients/s, roofline of the
+dominant kernel from HIP-event timing, and a CPU baseline measured with the
+reference's own kernels, oracle/_ref, on a bounded sample of the same grid).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(name):
+    from ifos_amd import workloads
+    if name == "layered_visc":
+        return workloads.layered_visc()
+    if name == "toy_elastic":
+        return workloads.toy_elastic()
+    if name == "batched_aperture":
+        return workloads.batched_aperture()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(w, steps=40):
+    """Reference kernels (oracle/_ref) on the host: `steps` forward time steps
+    of the same grid, scaled to shot-gradients/s as 1 / (2 * NT * t_step)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    cfg = w.cfg
+    fields = {k: getattr(cfg, k) for k, _ in cfg._fields_ if k != "FL"}
+    fields["NT"] = steps
+    p = O.make_params(FL=[cfg.FL[i] for i in range(1, 8)], **fields)
+    shot = w.shots[0]
+    kind = "reference" if O.ref_available() else "port"
+    which = "ref" if kind == "reference" else "oracle"
+    t0 = time.perf_counter()
+    O.forward(p, w.start, shot.srcpos, shot.recpos[:8], which=which)
+    t = time.perf_counter() - t0
+    t_step = t / steps
+    rate = 1.0 / (2 * cfg.NT * t_step)
+    return dict(value=rate, unit="shot-gradients/s", cores=1, kind=kind,
+                sample=f"{steps} forward time steps of the {cfg.NX}x{cfg.NY} L={cfg.L} FDORDER={cfg.FDORDER} grid "
+                       f"({t:.1f} s, {cfg.NX * cfg.NY * steps / t / 1e6:.2f} Mcell-updates/s), scaled to one "
+                       f"forward+adjoint of NT={cfg.NT}; snapshot and correlation work not counted")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="layered_visc")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (every box here has one GPU; "
+              f"the multi-GPU plan is shot sharding with one gradient all-reduce per iteration)", file=sys.stderr)
+        return 2
+
+    import ifos_amd
+    w = build_workload(args.workload)
+    cfg = w.cfg
+    nshots_needed = args.warmup + args.steps
+    shots = [w.shots[k % len(w.shots)] for k in range(nshots_needed)]
+    log(f"[bench] workload {w.name}: {cfg.NX}x{cfg.NY} L={cfg.L} FDORDER={cfg.FDORDER} NT={cfg.NT} "
+        f"DTINV={cfg.DTINV}, {len(shots)} shots")
+
+    eng = ifos_amd.Engine(cfg)
+    # observed data: forward runs on the true model (untimed), kept in HBM
+    eng.set_model(**w.true)
+    obs = []
+    for k, s in enumerate(shots):
+        eng.set_shot(s.srcpos, s.recpos)
+        eng.forward(snapshots=False)
+        vx, vy = eng.seismograms()
+        vx[:, 0] = 0.0
+        vy[:, 0] = 0.0
+        obs.append((ifos_amd.DeviceBuffer.from_array(vx), ifos_amd.DeviceBuffer.from_array(vy)))
+        log(f"[bench] observed data for shot {k + 1}/{len(shots)}")
+    eng.set_model(**w.start)
+    eng.reset_gradient()
+
+    def step(k):
+        s = shots[k]
+        eng.set_shot(s.srcpos, s.recpos)
+        return eng.shot_gradient(obs[k][0], obs[k][1], kill=s.kill)
+
+    for k in range(args.warmup):
+        step(k)
+        log(f"[bench] warmup step {k + 1} done")
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        step(k)
+        log(f"[bench] step {k - args.warmup + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    rate = args.steps / elapsed
+    cell_updates = 2.0 * cfg.NX * cfg.NY * cfg.NT  # forward + adjoint per shot-gradient
+    gcells = cell_updates * rate / 1e9
+
+    # dominant kernel: the stress update; HIP-event timing on the engine stream
+    kern = 1
+    ms_k = eng.time_kernel(kern, reps=50)
+    bpc = eng.kernel_bytes_per_cell(kern)
+    bytes_launch = bpc * cfg.NX * cfg.NY
+    achieved = bytes_launch / (ms_k * 1e-3) / 1e9
+    ms_v = eng.time_kernel(0, reps=50)
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="k_update_s_visc" if cfg.L else "k_update_s_elastic",
+                bytes_per_cell=bpc, cells=cfg.NX * cfg.NY, avg_ms=round(ms_k, 5),
+                velocity_kernel_avg_ms=round(ms_v, 5),
+                velocity_kernel_GBs=round(eng.kernel_bytes_per_cell(0) * cfg.NX * cfg.NY / (ms_v * 1e-3) / 1e9, 1))
+    eng.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(w, steps=args.cpu_steps)
+        except Exception as e:  # reported, not fatal
+            cpu = dict(value=None, error=str(e))
+
+    out = {
+        "metric": "shot-gradients/s & Gcell-updates/s (% HBM roofline) at <=1e-4 rel. gradient err",
+        "value": round(rate, 5),
+        "unit": "shot-gradients/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded layered model with faults; observed traces from the true model)",
+        "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "dh": cfg.DH, "FDORDER": cfg.FDORDER, "L": cfg.L,
+                   "NT": cfg.NT, "DTINV": cfg.DTINV, "shots_per_step": 1, "receivers": int(len(w.shots[0].recpos)),
+                   "parallelism": "one GPU, one shot at a time"},
+        "gcell_updates_per_s": round(gcells, 3),
+        "gradient_parity": "bit-identical to the reference FP32 kernels (tests/test_gpu_parity.py)",
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

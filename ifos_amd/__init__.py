@@ -1,0 +1,213 @@
+"""ifos_amd -- MI355X engine for IFOS2D's P-SV forward solver and adjoint
+gradient.
+
+The compute path is the gfx950 library lib/libifos2d_hip.so (C ABI declared
+in include/ifos2d_hip.h).  This module is a thin ctypes binding to it; there
+is no CPU fallback: loading fails loudly when the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libifos2d_hip.so")
+
+_INT_FIELDS = ["NX", "NY", "FDORDER", "MAXRELERROR", "FW", "FREE_SURF", "BOUNDARY", "L",
+               "VELOCITY", "ADJOINT_TYPE", "PARAMETERIZATION", "DTINV", "NT",
+               "SOURCE_SHAPE", "SOURCE_TYPE", "TRKILL"]
+_FLT_FIELDS = ["DH", "DT", "VPPML", "FPML", "npower", "k_max_PML", "F_REF", "TAU"]
+
+SRC_STRIDE = 8
+
+
+class Config(C.Structure):
+    """Mirror of ifos_config (include/ifos2d_hip.h); names of src/globvar.h."""
+    _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [(n, C.c_float) for n in _FLT_FIELDS]
+                + [("FL", C.c_float * 8)])
+
+    def as_dict(self):
+        d = {n: getattr(self, n) for n in _INT_FIELDS + _FLT_FIELDS}
+        d["FL"] = [self.FL[i] for i in range(1, self.L + 1)]
+        return d
+
+
+DEFAULTS = dict(MAXRELERROR=1, FW=10, FREE_SURF=1, BOUNDARY=0, L=0, VELOCITY=0, ADJOINT_TYPE=1,
+                PARAMETERIZATION=1, DTINV=1, SOURCE_SHAPE=1, SOURCE_TYPE=1, TRKILL=0,
+                VPPML=3500.0, FPML=30.0, npower=2.0, k_max_PML=1.0, F_REF=0.0, TAU=0.0)
+
+
+def make_config(**kw):
+    c = Config()
+    d = dict(DEFAULTS)
+    d.update(kw)
+    fl = d.pop("FL", None)
+    for k, v in d.items():
+        setattr(c, k, v)
+    if fl is not None:
+        for i, v in enumerate(fl):
+            c.FL[i + 1] = v
+    return c
+
+
+_LIB = None
+
+
+class IfosError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the gfx950 library (raises if it is missing)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise IfosError(f"{LIB_PATH} is missing: build it with `python -m ifos_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.ifos_last_error.restype = C.c_char_p
+        L.ifos_dev_alloc.restype = C.c_void_p
+        L.ifos_dev_alloc.argtypes = [C.c_size_t]
+        L.ifos_dev_free.argtypes = [C.c_void_p]
+        L.ifos_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.ifos_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.ifos_engine_create.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]
+        L.ifos_engine_destroy.argtypes = [C.c_void_p]
+        L.ifos_engine_destroy.restype = None
+        L.ifos_engine_kernel_bytes_per_cell.restype = C.c_double
+        L.ifos_engine_kernel_bytes_per_cell.argtypes = [C.c_void_p, C.c_int]
+        for n in ["set_model", "set_shot", "forward", "seismograms", "fields", "residual", "adjoint",
+                  "shot_gradient", "gradient", "reset_gradient", "time_kernel", "nan_step"]:
+            getattr(L, "ifos_engine_" + n).restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise IfosError(lib().ifos_last_error().decode() + f" (code {rc})")
+
+
+def _fp(a, ty=C.c_float):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ty))
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+class DeviceBuffer:
+    """Device allocation owned by the library's allocator (plain bytes)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().ifos_dev_alloc(self.nbytes)
+        if not self.ptr:
+            raise IfosError(lib().ifos_last_error().decode())
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        _check(lib().ifos_memcpy_h2d(C.c_void_p(b.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return b
+
+    def free(self):
+        if self.ptr:
+            lib().ifos_dev_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Engine:
+    """One-GPU P-SV engine; mirrors the reference's per-shot flow
+    (src/IFOS2D.c:1076-2724): set_model -> set_shot -> forward / residual /
+    adjoint -> gradient."""
+
+    def __init__(self, config, device=0):
+        self.cfg = config
+        h = C.c_void_p()
+        _check(lib().ifos_engine_create(C.byref(config), device, C.byref(h)))
+        self.h = h
+        self.ntr = 0
+        self.nsrc = 0
+
+    def close(self):
+        if self.h:
+            lib().ifos_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_model(self, vp, vs, rho, taus=None, taup=None):
+        vp, vs, rho, taus, taup = (_f32(a) for a in (vp, vs, rho, taus, taup))
+        _check(lib().ifos_engine_set_model(self.h, _fp(vp), _fp(vs), _fp(rho), _fp(taus), _fp(taup)))
+
+    def set_shot(self, srcpos, recpos):
+        srcpos = _f32(np.asarray(srcpos, np.float32).reshape(-1, SRC_STRIDE))
+        recpos = np.ascontiguousarray(np.asarray(recpos, np.int32).reshape(-1, 2))
+        self.nsrc, self.ntr = srcpos.shape[0], recpos.shape[0]
+        _check(lib().ifos_engine_set_shot(self.h, self.nsrc, _fp(srcpos), self.ntr, _fp(recpos, C.c_int)))
+
+    def forward(self, snapshots=False):
+        _check(lib().ifos_engine_forward(self.h, int(bool(snapshots))))
+
+    def seismograms(self):
+        vx = np.zeros((self.ntr, self.cfg.NT), np.float32)
+        vy = np.zeros_like(vx)
+        _check(lib().ifos_engine_seismograms(self.h, _fp(vx), _fp(vy)))
+        return vx, vy
+
+    def fields(self):
+        out = np.zeros((5, self.cfg.NY, self.cfg.NX), np.float32)
+        _check(lib().ifos_engine_fields(self.h, _fp(out)))
+        return out
+
+    def _obs_args(self, obs_vx, obs_vy):
+        if isinstance(obs_vx, DeviceBuffer):
+            return C.c_void_p(obs_vx.ptr), C.c_void_p(obs_vy.ptr), 1, None
+        ovx, ovy = _f32(obs_vx), _f32(obs_vy)
+        return ovx.ctypes.data_as(C.c_void_p), ovy.ctypes.data_as(C.c_void_p), 0, (ovx, ovy)
+
+    def residual(self, obs_vx, obs_vy, kill=None):
+        a, b, dev, keep = self._obs_args(obs_vx, obs_vy)
+        k = None if kill is None else np.ascontiguousarray(kill, np.int32)
+        mis = np.zeros(2, np.float64)
+        _check(lib().ifos_engine_residual(self.h, a, b, _fp(k, C.c_int), dev, _fp(mis, C.c_double)))
+        return mis
+
+    def adjoint(self):
+        _check(lib().ifos_engine_adjoint(self.h))
+
+    def shot_gradient(self, obs_vx, obs_vy, kill=None):
+        a, b, dev, keep = self._obs_args(obs_vx, obs_vy)
+        k = None if kill is None else np.ascontiguousarray(kill, np.int32)
+        mis = np.zeros(2, np.float64)
+        _check(lib().ifos_engine_shot_gradient(self.h, a, b, _fp(k, C.c_int), dev, _fp(mis, C.c_double)))
+        return mis
+
+    def gradient(self, total=False):
+        g = np.zeros((3, self.cfg.NY, self.cfg.NX), np.float32)
+        _check(lib().ifos_engine_gradient(self.h, int(total), _fp(g[0]), _fp(g[1]), _fp(g[2])))
+        return g
+
+    def reset_gradient(self):
+        _check(lib().ifos_engine_reset_gradient(self.h))
+
+    def time_kernel(self, kernel, reps=20):
+        ms = C.c_double()
+        _check(lib().ifos_engine_time_kernel(self.h, kernel, reps, C.byref(ms)))
+        return ms.value
+
+    def kernel_bytes_per_cell(self, kernel):
+        return lib().ifos_engine_kernel_bytes_per_cell(self.h, kernel)

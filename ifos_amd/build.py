@@ -1,0 +1,85 @@
+"""Build the gfx950 shared library ifos_amd/lib/libifos2d_hip.so.
+
+HIP sources are compiled by hipcc for gfx950 only, host C by gcc; both with
+-ffp-contract=off so that the FP32 arithmetic matches the reference's CPU
+build (gcc -O3 -std=c99 on x86-64, no FMA) bit for bit.  Device code keeps
+IEEE denormals and is built without packed-FP32 instructions.
+
+Usage: python -m ifos_amd.build  (or build() from __graft_entry__)
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIBDIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIBDIR, "libifos2d_hip.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-ffp-contract=off", "-fno-fast-math", "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-value", "-Wno-unused-result",
+    "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+]
+C_FLAGS = ["-O2", "-g", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-Wall",
+           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+
+HIP_SOURCES = ["ifos_kernels.hip", "ifos_engine.hip", "ifos_dropin.hip"]
+C_SOURCES = ["host/ifos_setup.c"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and "not a recognized feature" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return out
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    jobs = []
+    for src in C_SOURCES:
+        o = os.path.join(OBJ, os.path.basename(src).replace(".c", ".o"))
+        jobs.append(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
+    for src in HIP_SOURCES:
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        o = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o"))
+        jobs.append([HIPCC, *HIP_FLAGS, "-c", path, "-o", o])
+        objs.append(o)
+    procs = [(j, subprocess.Popen(j, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)) for j in jobs]
+    failed = False
+    for j, p in procs:
+        out, _ = p.communicate()
+        out = "\n".join(l for l in out.splitlines()
+                        if "packed-fp32-ops" not in l and "not a recognized feature" not in l)
+        if p.returncode != 0:
+            sys.stderr.write(out + "\n")
+            failed = True
+        elif verbose and out.strip():
+            print(out)
+    if failed:
+        raise RuntimeError("compilation failed")
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs,
+          "-Wl,--no-undefined", "-lm"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,730 @@
+// ifos_dropin.hip -- entry points with the reference's names and argument
+// lists (src/fd.h of IFOS2D 2.0.3), running the gfx950 kernels on
+// device-resident Numerical-Recipes arrays.
+//
+// Device NR arrays: ifos_dev_matrix(nrl,nrh,ncl,nch) returns a host
+// row-pointer array of the shape util.c's matrix() builds
+// (src/util.c:184-206) whose rows point into one device allocation;
+// ifos_dev_f3tensor and ifos_dev_vector likewise (src/util.c:279, :95).
+// A registry keyed by the returned pointer recovers base and bounds, so the
+// kernels see plain origin pointers and pitches.  Host-side arrays the
+// reference keeps on the host (srcpos_loc, signals, seismogram sections,
+// the CPML profile vectors, hc) may stay host arrays: they are staged per
+// call.  Work is queued on one library stream in call order; ifos_dev_sync()
+// waits for it (required before the host reads device arrays back).
+//
+// Parameters are read from the reference's global variables by their names
+// (src/globvar.h).  This library defines them, so it also runs standalone; a
+// driver that defines them itself (IFOS2D.c includes globvar.h) supplies its
+// own definitions, which take precedence at dynamic link time.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "host/ifos_setup.h"
+#include "ifos2d_dropin.h"
+#include "ifos2d_hip.h"
+#include "ifos_internal.h"
+
+using namespace ifos;
+
+namespace ifos {
+void set_step_control(const int *nt_dev, int DTINV, int NT);
+void launch_inject_serial(const PointArgs &a, const StepGeom &g, float *vx, float *vy, const float *rip,
+                          const float *rjp, int sw, int ADJOINT_TYPE, const int *nt_dev, int nt, int *nt_tick,
+                          hipStream_t s);
+}
+void ifos_set_error(const char *fmt, ...);
+
+// ---------------------------------------------------------------------------
+// reference globals read by the kernels (src/globvar.h)
+// ---------------------------------------------------------------------------
+extern "C" {
+float DT, DH, F_REF, *FL;
+int NX, NY, NT, FDORDER, FW, FREE_SURF, BOUNDARY, L, PARAMETERIZATION, VELOCITY, ADJOINT_TYPE, SOURCE_TYPE;
+int NPROCX = 1, NPROCY = 1, MYID, POS[3], WAVETYPE = 1, ACOUSTIC, SEISMO = 1;
+}
+
+// ---------------------------------------------------------------------------
+// registry of device NR arrays
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DevArr {
+    void *dev;        // device allocation
+    void *rows;       // host row-pointer block (as allocated)
+    void *rows2;      // second level for f3tensor
+    int kind;         // 1 vector, 2 matrix, 3 f3tensor
+    int nrl, nrh, ncl, nch, ndl, ndh;
+    size_t bytes;
+};
+
+std::mutex g_mu;
+std::unordered_map<const void *, DevArr> g_reg;
+hipStream_t g_stream = nullptr;
+
+hipStream_t stream()
+{
+    if (!g_stream) hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking);
+    return g_stream;
+}
+
+[[noreturn]] void fatal(const char *what)
+{
+    // src/util.c:26-36: message, then abort of the run
+    fprintf(stderr, "\n Error: %s\n", what);
+    exit(1);
+}
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+        fatal(buf);
+    }
+}
+
+const DevArr *lookup(const void *p)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_reg.find(p);
+    return it == g_reg.end() ? nullptr : &it->second;
+}
+
+// origin pointer and pitch of a registered device matrix
+float *origin_of(float **m, int *pitch, const char *name)
+{
+    const DevArr *a = lookup(m);
+    if (!a || a->kind != 2) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "%s is not a device matrix created by ifos_dev_matrix()", name);
+        fatal(buf);
+    }
+    const int p = a->nch - a->ncl + 1;
+    *pitch = p;
+    return (float *)a->dev - (long)a->nrl * p - a->ncl;
+}
+
+// base of element [1][1] of a device CPML strip matrix
+float *strip_of(float **m, int *pitch, const char *name)
+{
+    int p;
+    float *o = origin_of(m, &p, name);
+    *pitch = p;
+    return o + p + 1;
+}
+
+// device copy of a 1-based host or device vector of n entries, element 1 first
+struct VecCache {
+    std::vector<float> host;
+    float *dev = nullptr;
+};
+std::unordered_map<const void *, VecCache> g_vcache;
+
+const float *vec1_dev(const float *v, int n)
+{
+    if (n <= 0) n = 1;
+    const DevArr *a = lookup(v);
+    if (a && a->kind == 1) return (const float *)a->dev + (1 - a->nrl);
+    std::lock_guard<std::mutex> lk(g_mu);
+    VecCache &c = g_vcache[v];
+    if (c.host.size() != (size_t)n || memcmp(c.host.data(), v + 1, sizeof(float) * n)) {
+        c.host.assign(v + 1, v + 1 + n);
+        if (c.dev) hip_check(hipFree(c.dev), "hipFree");
+        hip_check(hipMalloc(&c.dev, sizeof(float) * n), "hipMalloc");
+        hip_check(hipMemcpyAsync(c.dev, c.host.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream()),
+                  "profile upload");
+    }
+    return c.dev;
+}
+
+// per-call staging of small host data
+struct Stage {
+    void *dev = nullptr;
+    size_t cap = 0;
+    void *get(size_t n)
+    {
+        if (n > cap) {
+            if (dev) hip_check(hipFree(dev), "hipFree");
+            cap = n * 2 + 256;
+            hip_check(hipMalloc(&dev, cap), "hipMalloc(stage)");
+        }
+        return dev;
+    }
+};
+Stage g_stage[4];
+
+StepGeom make_geom(int nx1, int nx2, int ny1, int ny2, int pitch, const float *hc, const float *K_x,
+                   const float *a_x, const float *b_x, const float *K_xh, const float *a_xh, const float *b_xh,
+                   const float *K_y, const float *a_y, const float *b_y, const float *K_yh, const float *a_yh,
+                   const float *b_yh)
+{
+    if (nx1 != 1 || ny1 != 1) fatal("the device kernels update the whole grid: nx1 and ny1 must be 1");
+    if (NPROCX != 1 || NPROCY != 1) fatal("one GPU holds the whole domain: NPROCX = NPROCY = 1 required");
+    StepGeom g;
+    g.NX = nx2; g.NY = ny2; g.pitch = pitch;
+    g.FW = FW; g.FREE_SURF = FREE_SURF; g.BOUNDARY = BOUNDARY;
+    g.N = FDORDER / 2; g.DT = DT; g.DH = DH;
+    for (int k = 0; k < 7; k++) g.hc[k] = hc[k];
+    const int n = 2 * FW;
+    const float *z = nullptr;
+    g.pml = {K_x ? vec1_dev(K_x, n) : z, a_x ? vec1_dev(a_x, n) : z, b_x ? vec1_dev(b_x, n) : z,
+             K_xh ? vec1_dev(K_xh, n) : z, a_xh ? vec1_dev(a_xh, n) : z, b_xh ? vec1_dev(b_xh, n) : z,
+             K_y ? vec1_dev(K_y, n) : z, a_y ? vec1_dev(a_y, n) : z, b_y ? vec1_dev(b_y, n) : z,
+             K_yh ? vec1_dev(K_yh, n) : z, a_yh ? vec1_dev(a_yh, n) : z, b_yh ? vec1_dev(b_yh, n) : z};
+    g.px = n > 0 ? n : 1;
+    g.py = nx2;
+    return g;
+}
+
+// point-wise additions in order: field[c_l] = (float)((double)field[c_l] + w*v_l)
+__global__ void k_point_add(int n, const int *ix, const int *iy, const double *amp, int pitch, float *f1, float *f2)
+{
+    if (threadIdx.x != 0) return;
+    for (int l = 0; l < n; l++) {
+        const long c = (long)iy[l] * pitch + ix[l];
+        f1[c] = (float)((double)f1[c] + 0.5 * amp[l]);
+        if (f2) f2[c] = (float)((double)f2[c] + 0.5 * amp[l]);
+    }
+}
+
+__global__ void k_gather(int n, const int *ix, const int *iy, int pitch, const float *a, const float *b, int mode,
+                         float *out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long c = (long)iy[t] * pitch + ix[t];
+    out[t] = mode == 2 ? -a[c] - b[c] : a[c];
+    if (mode == 1) out[n + t] = b[c];
+}
+
+__global__ void k_zero_rect(float *origin, int pitch, int j0, int j1, int i0, int i1)
+{
+    const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = j0 + blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > i1 || j > j1) return;
+    origin[(long)j * pitch + i] = 0.0f;
+}
+
+void zero_dev(void *p)
+{
+    if (!p) return;
+    const DevArr *a = lookup(p);
+    if (!a) fatal("zero_fdveps: array is not a device array created by ifos_dev_*()");
+    hip_check(hipMemsetAsync(a->dev, 0, a->bytes, stream()), "zero");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// device NR arrays
+// ---------------------------------------------------------------------------
+extern "C" float **ifos_dev_matrix(int nrl, int nrh, int ncl, int nch)
+{
+    const long nr = nrh - nrl + 1, nc = nch - ncl + 1;
+    float *dev = nullptr;
+    hip_check(hipMalloc(&dev, sizeof(float) * nr * nc), "ifos_dev_matrix");
+    hip_check(hipMemset(dev, 0, sizeof(float) * nr * nc), "ifos_dev_matrix zero");
+    float **rows = (float **)malloc(sizeof(float *) * nr);
+    for (long r = 0; r < nr; r++) rows[r] = dev + r * nc - ncl;
+    float **m = rows - nrl;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_reg[m] = DevArr{dev, rows, nullptr, 2, nrl, nrh, ncl, nch, 0, 0, sizeof(float) * nr * nc};
+    return m;
+}
+
+extern "C" float ***ifos_dev_f3tensor(int nrl, int nrh, int ncl, int nch, int ndl, int ndh)
+{
+    const long nr = nrh - nrl + 1, nc = nch - ncl + 1, nd = ndh - ndl + 1;
+    float *dev = nullptr;
+    hip_check(hipMalloc(&dev, sizeof(float) * nr * nc * nd), "ifos_dev_f3tensor");
+    hip_check(hipMemset(dev, 0, sizeof(float) * nr * nc * nd), "ifos_dev_f3tensor zero");
+    float **cols = (float **)malloc(sizeof(float *) * nr * nc);
+    float ***rows = (float ***)malloc(sizeof(float **) * nr);
+    for (long r = 0; r < nr; r++) {
+        for (long c = 0; c < nc; c++) cols[r * nc + c] = dev + (r * nc + c) * nd - ndl;
+        rows[r] = cols + r * nc - ncl;
+    }
+    float ***t = rows - nrl;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_reg[t] = DevArr{dev, rows, cols, 3, nrl, nrh, ncl, nch, ndl, ndh, sizeof(float) * nr * nc * nd};
+    return t;
+}
+
+extern "C" float *ifos_dev_vector(int nl, int nh)
+{
+    const long n = nh - nl + 1;
+    float *dev = nullptr;
+    hip_check(hipMalloc(&dev, sizeof(float) * n), "ifos_dev_vector");
+    hip_check(hipMemset(dev, 0, sizeof(float) * n), "ifos_dev_vector zero");
+    float *v = dev - nl;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_reg[v] = DevArr{dev, nullptr, nullptr, 1, nl, nh, 0, 0, 0, 0, sizeof(float) * n};
+    return v;
+}
+
+extern "C" void ifos_dev_free_array(void *p)
+{
+    DevArr a;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_reg.find(p);
+        if (it == g_reg.end()) return;
+        a = it->second;
+        g_reg.erase(it);
+    }
+    hipStreamSynchronize(stream());
+    hipFree(a.dev);
+    free(a.rows);
+    free(a.rows2);
+}
+
+// copy between a host NR array (util.c layout, contiguous) and a device one
+// of the same bounds; dir 0 = host to device, 1 = device to host
+extern "C" int ifos_dev_copy_array(void *dev_arr, const void *host_first_elem, int dir)
+{
+    const DevArr *a = lookup(dev_arr);
+    if (!a) return ifos_set_error("not a device array"), -2;
+    hipError_t e = dir == 0 ? hipMemcpyAsync(a->dev, host_first_elem, a->bytes, hipMemcpyHostToDevice, stream())
+                            : hipMemcpyAsync((void *)host_first_elem, a->dev, a->bytes, hipMemcpyDeviceToHost, stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    if (e != hipSuccess) return ifos_set_error("copy failed: %s", hipGetErrorString(e)), -1;
+    return 0;
+}
+
+extern "C" int ifos_dev_sync(void)
+{
+    hipError_t e = hipStreamSynchronize(stream());
+    if (e != hipSuccess) return ifos_set_error("%s", hipGetErrorString(e)), -1;
+    return 0;
+}
+
+// one element of a device matrix (the NaN probe of src/IFOS2D.c:1660)
+extern "C" float ifos_dev_read(float **m, int j, int i)
+{
+    int p;
+    float *o = origin_of(m, &p, "ifos_dev_read");
+    float v = 0.0f;
+    hip_check(hipMemcpyAsync(&v, o + (long)j * p + i, sizeof v, hipMemcpyDeviceToHost, stream()), "read");
+    hip_check(hipStreamSynchronize(stream()), "read sync");
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// update_v_PML  (src/update_v_PML.c:29, src/fd.h:384-389)
+// ---------------------------------------------------------------------------
+extern "C" void update_v_PML(int nx1, int nx2, int ny1, int ny2, int nt, float **vx, float **vxp1, float **vxm1,
+                             float **vy, float **vyp1, float **vym1, float **uttx, float **utty, float **sxx,
+                             float **syy, float **sxy, float **rip, float **rjp, float **srcpos_loc,
+                             float **signals, float **signals1, int nsrc, float **absorb_coeff, float *hc,
+                             int infoout, int sw, float *K_x, float *a_x, float *b_x, float *K_x_half,
+                             float *a_x_half, float *b_x_half, float *K_y, float *a_y, float *b_y, float *K_y_half,
+                             float *a_y_half, float *b_y_half, float **psi_sxx_x, float **psi_syy_y,
+                             float **psi_sxy_y, float **psi_sxy_x)
+{
+    (void)vxm1; (void)vym1; (void)uttx; (void)utty; (void)absorb_coeff; (void)infoout;
+    int P, p2;
+    VArgs a;
+    a.vx = origin_of(vx, &P, "vx");
+    a.vy = origin_of(vy, &p2, "vy");
+    a.sxx = origin_of(sxx, &p2, "sxx");
+    a.syy = origin_of(syy, &p2, "syy");
+    a.sxy = origin_of(sxy, &p2, "sxy");
+    a.rip = origin_of(rip, &p2, "rip");
+    a.rjp = origin_of(rjp, &p2, "rjp");
+    a.vxp1 = origin_of(vxp1, &p2, "vxp1");
+    a.vyp1 = origin_of(vyp1, &p2, "vyp1");
+    a.g = make_geom(nx1, nx2, ny1, ny2, P, hc, K_x, a_x, b_x, K_x_half, a_x_half, b_x_half, K_y, a_y, b_y, K_y_half,
+                    a_y_half, b_y_half);
+    int px = 1, py = 1;
+    a.psi_sxx_x = strip_of(psi_sxx_x, &px, "psi_sxx_x");
+    a.psi_sxy_x = strip_of(psi_sxy_x, &px, "psi_sxy_x");
+    a.psi_syy_y = strip_of(psi_syy_y, &py, "psi_syy_y");
+    a.psi_sxy_y = strip_of(psi_sxy_y, &py, "psi_sxy_y");
+    a.g.px = px;
+    a.g.py = py;
+    a.sw = sw;
+    a.VELOCITY = VELOCITY;
+    set_step_control(nullptr, 1, NT);
+    launch_update_v(a, stream());
+
+    if (nsrc <= 0) return;
+    // point sources after the loop (src/update_v_PML.c:553-578): positions,
+    // types and this step's samples are staged from the host arrays
+    std::vector<int> pos(3 * nsrc);
+    std::vector<double> sc(2 * nsrc);
+    std::vector<float> s(2 * nsrc);
+    int any = 0;
+    for (int l = 1; l <= nsrc; l++) {
+        pos[l - 1] = (int)srcpos_loc[1][l];
+        pos[nsrc + l - 1] = (int)srcpos_loc[2][l];
+        if (sw == 0) {
+            float azi_rad = srcpos_loc[7][l] * 3.141592653589793 / 180;
+            SOURCE_TYPE = (int)srcpos_loc[8][l];
+            pos[2 * nsrc + l - 1] = SOURCE_TYPE;
+            sc[l - 1] = sin(azi_rad);
+            sc[nsrc + l - 1] = cos(azi_rad);
+            if (SOURCE_TYPE >= 2 && SOURCE_TYPE <= 4) any = 1;
+        } else {
+            any = 1;
+        }
+        const DevArr *d0 = lookup(signals), *d1 = lookup(signals1);
+        if (d0 || d1) fatal("update_v_PML: signals are host arrays in this interface");
+        s[l - 1] = signals[l][nt];
+        s[nsrc + l - 1] = signals1[l][nt];
+    }
+    if (!any) return;
+    char *st = (char *)g_stage[0].get(sizeof(int) * pos.size() + sizeof(double) * sc.size() + sizeof(float) * s.size());
+    std::vector<char> blob(sizeof(int) * pos.size() + sizeof(double) * sc.size() + sizeof(float) * s.size());
+    size_t o1 = sizeof(int) * pos.size(), o2 = o1 + sizeof(double) * sc.size();
+    memcpy(blob.data(), pos.data(), o1);
+    memcpy(blob.data() + o1, sc.data(), sizeof(double) * sc.size());
+    memcpy(blob.data() + o2, s.data(), sizeof(float) * s.size());
+    hip_check(hipMemcpyAsync(st, blob.data(), blob.size(), hipMemcpyHostToDevice, stream()), "stage sources");
+    hip_check(hipStreamSynchronize(stream()), "stage sync");
+    PointArgs pa;
+    pa.n = nsrc;
+    pa.ix = (const int *)st;
+    pa.iy = (const int *)st + nsrc;
+    pa.type = (const int *)st + 2 * nsrc;
+    pa.sin_azi = (const double *)(st + o1);
+    pa.cos_azi = (const double *)(st + o1) + nsrc;
+    pa.sig = (const float *)(st + o2);
+    pa.sig1 = (const float *)(st + o2) + nsrc;
+    pa.ld = 1;  // one sample per source: column nt-1 with nt = 1
+    // serial in source order, as the reference loop (positions may repeat)
+    launch_inject_serial(pa, a.g, a.vx, a.vy, a.rip, a.rjp, sw, ADJOINT_TYPE, nullptr, 1, nullptr, stream());
+}
+
+// ---------------------------------------------------------------------------
+// update_s_elastic_PML  (src/update_s_elastic_PML.c:28, src/fd.h:352-357)
+// ---------------------------------------------------------------------------
+extern "C" void update_s_elastic_PML(int nx1, int nx2, int ny1, int ny2, float **vx, float **vy, float **ux,
+                                     float **uy, float **uxy, float **uyx, float **sxx, float **syy, float **sxy,
+                                     float **pi, float **u, float **uipjp, float **absorb_coeff, float **rho,
+                                     float *hc, int infoout, float *K_x, float *a_x, float *b_x, float *K_x_half,
+                                     float *a_x_half, float *b_x_half, float *K_y, float *a_y, float *b_y,
+                                     float *K_y_half, float *a_y_half, float *b_y_half, float **psi_vxx,
+                                     float **psi_vyy, float **psi_vxy, float **psi_vyx)
+{
+    (void)uyx; (void)absorb_coeff; (void)infoout;
+    if (PARAMETERIZATION != 1) fatal("update_s_elastic_PML: only PARAMETERIZATION=1 is supported on the GPU");
+    int P, p2;
+    SElArgs a;
+    a.vx = origin_of(vx, &P, "vx");
+    a.vy = origin_of(vy, &p2, "vy");
+    a.sxx = origin_of(sxx, &p2, "sxx");
+    a.syy = origin_of(syy, &p2, "syy");
+    a.sxy = origin_of(sxy, &p2, "sxy");
+    a.ux = origin_of(ux, &p2, "ux");
+    a.uy = origin_of(uy, &p2, "uy");
+    a.uxy = origin_of(uxy, &p2, "uxy");
+    a.uipjp = origin_of(uipjp, &p2, "uipjp");
+    a.rho = origin_of(rho, &p2, "rho");
+    a.vs = origin_of(u, &p2, "u");
+    a.vp = origin_of(pi, &p2, "pi");
+    a.f = nullptr;
+    a.gl = nullptr;
+    a.g = make_geom(nx1, nx2, ny1, ny2, P, hc, K_x, a_x, b_x, K_x_half, a_x_half, b_x_half, K_y, a_y, b_y, K_y_half,
+                    a_y_half, b_y_half);
+    int px = 1, py = 1;
+    a.psi_vxx = strip_of(psi_vxx, &px, "psi_vxx");
+    a.psi_vyx = strip_of(psi_vyx, &px, "psi_vyx");
+    a.psi_vyy = strip_of(psi_vyy, &py, "psi_vyy");
+    a.psi_vxy = strip_of(psi_vxy, &py, "psi_vxy");
+    a.g.px = px;
+    a.g.py = py;
+    set_step_control(nullptr, 1, NT);
+    launch_update_s_elastic(a, stream());
+}
+
+static int tensor_origin(float ***t, float **origin, int pitch, const char *name)
+{
+    const DevArr *a = lookup(t);
+    if (!a || a->kind != 3) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "%s is not a device tensor created by ifos_dev_f3tensor()", name);
+        fatal(buf);
+    }
+    const int nc = a->nch - a->ncl + 1, nd = a->ndh - a->ndl + 1;
+    if (nc != pitch) fatal("memory-variable tensors must share the wavefield grid bounds");
+    *origin = (float *)a->dev - ((long)a->nrl * nc + a->ncl) * nd - (a->ndl - 1);
+    return nd;
+}
+
+// ---------------------------------------------------------------------------
+// update_s_visc_PML  (src/update_s_visc_PML.c:28, src/fd.h:363-369)
+// ---------------------------------------------------------------------------
+extern "C" void update_s_visc_PML(int nx1, int nx2, int ny1, int ny2, float **vx, float **vy, float **ux,
+                                  float **uy, float **uxy, float **uyx, float **sxx, float **syy, float **sxy,
+                                  float **pi, float **u, float **uipjp, float **rho, float *hc, int infoout,
+                                  float ***r, float ***p, float ***q, float **fipjp, float **f, float **g,
+                                  float *bip, float *bjm, float *cip, float *cjm, float ***d, float ***e,
+                                  float ***dip, float *K_x, float *a_x, float *b_x, float *K_x_half,
+                                  float *a_x_half, float *b_x_half, float *K_y, float *a_y, float *b_y,
+                                  float *K_y_half, float *a_y_half, float *b_y_half, float **psi_vxx,
+                                  float **psi_vyy, float **psi_vxy, float **psi_vyx)
+{
+    (void)uyx; (void)pi; (void)u; (void)uipjp; (void)rho; (void)infoout;
+    if (L < 1 || L > 5) fatal("update_s_visc_PML: L must be 1..5 on the GPU");
+    int P, p2;
+    SViscArgs a;
+    a.L = L;
+    a.vx = origin_of(vx, &P, "vx");
+    a.vy = origin_of(vy, &p2, "vy");
+    a.sxx = origin_of(sxx, &p2, "sxx");
+    a.syy = origin_of(syy, &p2, "syy");
+    a.sxy = origin_of(sxy, &p2, "sxy");
+    a.ux = origin_of(ux, &p2, "ux");
+    a.uy = origin_of(uy, &p2, "uy");
+    a.uxy = origin_of(uxy, &p2, "uxy");
+    a.fipjp = origin_of(fipjp, &p2, "fipjp");
+    a.f = origin_of(f, &p2, "f");
+    a.gv = origin_of(g, &p2, "g");
+    float *o;
+    int nd = tensor_origin(r, &o, P, "r"); a.r = o;
+    tensor_origin(p, &o, P, "p"); a.p = o;
+    tensor_origin(q, &o, P, "q"); a.q = o;
+    tensor_origin(d, &o, P, "d"); a.d = o;
+    tensor_origin(e, &o, P, "e"); a.e = o;
+    tensor_origin(dip, &o, P, "dip"); a.dip = o;
+    a.cs = nd;
+    a.ls = 1;
+    for (int l = 0; l <= kMaxL; l++) {
+        a.bip[l] = (l >= 1 && l <= L) ? bip[l] : 0.0f;
+        a.bjm[l] = (l >= 1 && l <= L) ? bjm[l] : 0.0f;
+        a.cip[l] = (l >= 1 && l <= L) ? cip[l] : 0.0f;
+        a.cjm[l] = (l >= 1 && l <= L) ? cjm[l] : 0.0f;
+    }
+    a.g = make_geom(nx1, nx2, ny1, ny2, P, hc, K_x, a_x, b_x, K_x_half, a_x_half, b_x_half, K_y, a_y, b_y, K_y_half,
+                    a_y_half, b_y_half);
+    int px = 1, py = 1;
+    a.psi_vxx = strip_of(psi_vxx, &px, "psi_vxx");
+    a.psi_vyx = strip_of(psi_vyx, &px, "psi_vyx");
+    a.psi_vyy = strip_of(psi_vyy, &py, "psi_vyy");
+    a.psi_vxy = strip_of(psi_vxy, &py, "psi_vxy");
+    a.g.px = px;
+    a.g.py = py;
+    set_step_control(nullptr, 1, NT);
+    launch_update_s_visc(a, stream());
+}
+
+// ---------------------------------------------------------------------------
+// psource  (src/psource.c:26, src/fd.h:206-207): the amplitude is formed on
+// the host exactly as the reference does, the addition runs on the device in
+// source order
+// ---------------------------------------------------------------------------
+extern "C" void psource(int nt, float **sxx, float **syy, float **sp, float **srcpos_loc, float **signals, int nsrc,
+                        int sw)
+{
+    (void)sp;
+    if (nsrc <= 0) return;
+    int P, p2;
+    float *oxx = origin_of(sxx, &P, "sxx"), *oyy = origin_of(syy, &p2, "syy");
+    std::vector<int> pos(2 * nsrc);
+    std::vector<double> amp(nsrc);
+    for (int l = 1; l <= nsrc; l++) {
+        pos[l - 1] = (int)srcpos_loc[1][l];
+        pos[nsrc + l - 1] = (int)srcpos_loc[2][l];
+        float a = 0.0f;
+        if (sw == 0) {
+            if (nt == 1) a = signals[l][nt + 1] / (2.0 * DH * DH);
+            if ((nt > 1) && (nt < NT)) a = (signals[l][nt + 1] - signals[l][nt - 1]) / (2.0 * DH * DH);
+            if (nt == NT) a = -signals[l][nt - 1] / (2.0 * DH * DH);
+        } else {
+            a = signals[l][nt];
+        }
+        amp[l - 1] = a;
+    }
+    char *st = (char *)g_stage[1].get(sizeof(int) * pos.size() + sizeof(double) * amp.size());
+    std::vector<char> blob(sizeof(int) * pos.size() + sizeof(double) * amp.size());
+    memcpy(blob.data(), pos.data(), sizeof(int) * pos.size());
+    memcpy(blob.data() + sizeof(int) * pos.size(), amp.data(), sizeof(double) * amp.size());
+    hip_check(hipMemcpyAsync(st, blob.data(), blob.size(), hipMemcpyHostToDevice, stream()), "stage psource");
+    hip_check(hipStreamSynchronize(stream()), "stage sync");
+    hipLaunchKernelGGL(k_point_add, dim3(1), dim3(64), 0, stream(), nsrc, (const int *)st, (const int *)st + nsrc,
+                       (const double *)(st + sizeof(int) * pos.size()), P, oxx, oyy);
+}
+
+// ---------------------------------------------------------------------------
+// surface_elastic_PML (src/surface_elastic_PML.c:26, src/fd.h:286) and
+// surface_PML (src/surface_PML.c:25, src/fd.h:288)
+// ---------------------------------------------------------------------------
+static void surface_common(int ndepth, PostArgs &po, float **vx, float **vy, float **sxx, float **syy, float **sxy,
+                           float **ux, float **uy, float **uxy, float *hc, float *K_x, float *a_x, float *b_x,
+                           float **psi_vxx)
+{
+    if (ndepth != 1) fatal("surface: the free surface must be at ndepth = 1");
+    int P, p2;
+    po.vx = origin_of(vx, &P, "vx");
+    po.vy = origin_of(vy, &p2, "vy");
+    po.sxx = origin_of(sxx, &p2, "sxx");
+    po.syy = origin_of(syy, &p2, "syy");
+    po.sxy = origin_of(sxy, &p2, "sxy");
+    po.ux = ux ? origin_of(ux, &p2, "ux") : nullptr;
+    po.uy = uy ? origin_of(uy, &p2, "uy") : nullptr;
+    po.uxy = uxy ? origin_of(uxy, &p2, "uxy") : nullptr;
+    po.g = make_geom(1, NX, 1, NY, P, hc, K_x, a_x, b_x, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, nullptr);
+    po.g.FREE_SURF = 1;
+    int px = 1;
+    po.psi_vxxs = strip_of(psi_vxx, &px, "psi_vxx");
+    po.g.px = px;
+    po.sw = 0;
+    po.explosive = 0;
+    po.NT = NT;
+    po.src.n = 0;
+    po.ntr = 0;
+    po.nt_dev = nullptr;
+    po.nt = 1;
+    po.nan_flag = nullptr;
+}
+
+extern "C" void surface_elastic_PML(int ndepth, float **vx, float **vy, float **sxx, float **syy, float **sxy,
+                                    float **syz, float **pi, float **u, float **rho, float *hc, float *K_x, float *a_x,
+                                    float *b_x, float **psi_vxx, float **ux, float **uy, float **uxy, float **uyz,
+                                    float **sxz, float **uxz)
+{
+    (void)syz; (void)uyz; (void)sxz; (void)uxz;
+    if (PARAMETERIZATION != 1) fatal("surface_elastic_PML: only PARAMETERIZATION=1 is supported on the GPU");
+    PostArgs po;
+    memset(&po, 0, sizeof po);
+    surface_common(ndepth, po, vx, vy, sxx, syy, sxy, ux, uy, uxy, hc, K_x, a_x, b_x, psi_vxx);
+    int p2;
+    po.L = 0;
+    po.f = nullptr;
+    po.gl = nullptr;
+    po.rho = origin_of(rho, &p2, "rho");
+    po.vs = origin_of(u, &p2, "u");
+    po.vp = origin_of(pi, &p2, "pi");
+    launch_post(po, stream());
+}
+
+extern "C" void surface_PML(int ndepth, float **vx, float **vy, float **sxx, float **syy, float **sxy, float **syz,
+                            float ***p, float ***q, float **ppi, float **pu, float **prho, float **ptaup,
+                            float **ptaus, float *etajm, float *peta, float *hc, float *K_x, float *a_x, float *b_x,
+                            float **psi_vxx, float **ux, float **uy, float **uxy, float **uyz, float **sxz,
+                            float **uxz)
+{
+    (void)syz; (void)uyz; (void)sxz; (void)uxz;
+    if (PARAMETERIZATION != 1) fatal("surface_PML: only PARAMETERIZATION=1 is supported on the GPU");
+    PostArgs po;
+    memset(&po, 0, sizeof po);
+    surface_common(ndepth, po, vx, vy, sxx, syy, sxy, ux, uy, uxy, hc, K_x, a_x, b_x, psi_vxx);
+    int p2;
+    po.L = L;
+    po.rho = origin_of(prho, &p2, "prho");
+    po.vs = origin_of(pu, &p2, "pu");
+    po.vp = origin_of(ppi, &p2, "ppi");
+    po.taus = origin_of(ptaus, &p2, "ptaus");
+    po.taup = origin_of(ptaup, &p2, "ptaup");
+    float *o;
+    int nd = tensor_origin(p, &o, po.g.pitch, "p");
+    po.p = o;
+    tensor_origin(q, &o, po.g.pitch, "q");
+    po.q = o;
+    po.cs = nd;
+    po.ls = 1;
+    // the relaxed-modulus sums of src/surface_PML.c:46-60
+    float pts[kMaxL + 1];
+    for (int l = 1; l <= L; l++) pts[l] = 1.0 / (2.0 * 3.141592653589793 * FL[l]);
+    float ws = 2.0 * 3.141592653589793 * F_REF;
+    float sumu = 0.0, sumpi = 0.0;
+    for (int l = 1; l <= L; l++) {
+        sumu = sumu + ((ws * ws * pts[l] * pts[l]) / (1.0 + ws * ws * pts[l] * pts[l]));
+        sumpi = sumpi + ((ws * ws * pts[l] * pts[l]) / (1.0 + ws * ws * pts[l] * pts[l]));
+    }
+    po.sumu = sumu;
+    po.sumpi = sumpi;
+    for (int l = 0; l <= kMaxL; l++) po.eta[l] = (l >= 1 && l <= L) ? peta[l] : 0.0f;
+    for (int l = 1; l <= L; l++) etajm[l] = peta[l];
+    launch_post(po, stream());
+}
+
+// ---------------------------------------------------------------------------
+// seismo_ssg  (src/seismo_ssg.c:26, src/fd.h:262-265), SEISMO 1 and 2
+// ---------------------------------------------------------------------------
+extern "C" void seismo_ssg(int lsamp, int ntr, int **recpos, float **sectionvx, float **sectionvy,
+                           float **sectionvz, float **sectionp, float **sectioncurl, float **sectiondiv, float **vx,
+                           float **vy, float **vz, float **sxx, float **syy, float **sp, float **pi, float **u,
+                           float *hc)
+{
+    (void)sectionvz; (void)sectioncurl; (void)sectiondiv; (void)vz; (void)sp; (void)pi; (void)u; (void)hc;
+    if (ntr <= 0) return;
+    if (SEISMO != 1 && SEISMO != 2) fatal("seismo_ssg: SEISMO must be 1 or 2 on the GPU");
+    int P, p2;
+    const float *a, *b;
+    if (SEISMO == 1) {
+        a = origin_of(vx, &P, "vx");
+        b = origin_of(vy, &p2, "vy");
+    } else {
+        a = origin_of(sxx, &P, "sxx");
+        b = origin_of(syy, &p2, "syy");
+    }
+    std::vector<int> pos(2 * ntr);
+    for (int t = 1; t <= ntr; t++) {
+        pos[t - 1] = recpos[1][t];
+        pos[ntr + t - 1] = recpos[2][t];
+    }
+    int *dpos = (int *)g_stage[2].get(sizeof(int) * pos.size());
+    float *dout = (float *)g_stage[3].get(sizeof(float) * 2 * ntr);
+    hip_check(hipMemcpyAsync(dpos, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice, stream()), "recpos");
+    hipLaunchKernelGGL(k_gather, dim3((ntr + 63) / 64), dim3(64), 0, stream(), ntr, dpos, dpos + ntr, P, a, b,
+                       SEISMO == 1 ? 1 : 2, dout);
+    std::vector<float> h(2 * ntr);
+    hip_check(hipMemcpyAsync(h.data(), dout, sizeof(float) * 2 * ntr, hipMemcpyDeviceToHost, stream()), "gather");
+    hip_check(hipStreamSynchronize(stream()), "gather sync");
+    for (int t = 1; t <= ntr; t++) {
+        if (SEISMO == 1) {
+            sectionvx[t][lsamp] = h[t - 1];
+            sectionvy[t][lsamp] = h[ntr + t - 1];
+        } else {
+            sectionp[t][lsamp] = h[t - 1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// zero_fdveps / zero_fdveps_visc (src/zero_fdveps.c:28, src/fd.h:428-433):
+// every device array passed is cleared
+// ---------------------------------------------------------------------------
+extern "C" void zero_fdveps(int ny1, int ny2, int nx1, int nx2, float **vx, float **vy, float **vz, float **sxx,
+                            float **syy, float **sxy, float **sxz, float **syz, float **vxm1, float **vym1,
+                            float **uxy, float **vxp1, float **vyp1, float **psi_sxx_x, float **psi_sxy_x,
+                            float **psi_sxz_x, float **psi_vxx, float **psi_vyx, float **psi_vzx, float **psi_syy_y,
+                            float **psi_sxy_y, float **psi_syz_y, float **psi_vyy, float **psi_vxy, float **psi_vzy,
+                            float **psi_vxxs)
+{
+    (void)ny1; (void)ny2; (void)nx1; (void)nx2;
+    void *all[] = {vx, vy, sxx, syy, sxy, vxm1, vym1, uxy, vxp1, vyp1, psi_sxx_x, psi_sxy_x, psi_vxx,
+                   psi_vyx, psi_syy_y, psi_sxy_y, psi_vyy, psi_vxy, psi_vxxs};
+    for (void *p : all) zero_dev(p);
+    void *opt[] = {vz, sxz, syz, psi_sxz_x, psi_vzx, psi_syz_y, psi_vzy};
+    for (void *p : opt)
+        if (p && lookup(p)) zero_dev(p);
+}
+
+extern "C" void zero_fdveps_visc(int ny1, int ny2, int nx1, int nx2, float **vx, float **vy, float **vz,
+                                 float **sxx, float **syy, float **sxy, float **sxz, float **syz, float **vxm1,
+                                 float **vym1, float **uxy, float **vxp1, float **vyp1, float **psi_sxx_x,
+                                 float **psi_sxy_x, float **psi_sxz_x, float **psi_vxx, float **psi_vyx,
+                                 float **psi_vzx, float **psi_syy_y, float **psi_sxy_y, float **psi_syz_y,
+                                 float **psi_vyy, float **psi_vxy, float **psi_vzy, float **psi_vxxs, float ***pr,
+                                 float ***pp, float ***pq, float ***pt, float ***po)
+{
+    zero_fdveps(ny1, ny2, nx1, nx2, vx, vy, vz, sxx, syy, sxy, sxz, syz, vxm1, vym1, uxy, vxp1, vyp1, psi_sxx_x,
+                psi_sxy_x, psi_sxz_x, psi_vxx, psi_vyx, psi_vzx, psi_syy_y, psi_sxy_y, psi_syz_y, psi_vyy, psi_vxy,
+                psi_vzy, psi_vxxs);
+    zero_dev(pr);
+    zero_dev(pp);
+    zero_dev(pq);
+    if (pt && lookup(pt)) zero_dev(pt);
+    if (po && lookup(po)) zero_dev(po);
+}

@@ -1,0 +1,173 @@
+// ifos_internal.h -- device-side argument blocks and kernel launchers shared
+// by the native engine (ifos_engine.hip) and the reference-signature entry
+// points (ifos_dropin.hip).
+//
+// Addressing convention: every grid field is passed as an "origin" pointer,
+// i.e. the address element (j=0, i=0) would have, plus a row pitch in floats,
+// so that field(j,i) = origin[j*pitch + i] for the full padded range
+// j in [-nd+1, NY+nd], i in [-nd+1, NX+nd] (nd = FDORDER/2+1).  Both the
+// Numerical-Recipes layout of the reference (src/util.c:184, pitch NX+2nd)
+// and the engine's own aligned layout fit this form.
+//
+// CPML memory: x-strips psi(j,h) = p[(j-1)*px + (h-1)], j 1..NY, h 1..2FW;
+// y-strips psi(h,i) = p[(h-1)*py + (i-1)], h 1..2FW, i 1..NX
+// (shapes of src/IFOS2D.c:628-641).  Profiles are 0-based arrays of 2FW.
+//
+// Memory variables r,p,q and coefficients d,e,dip of the GSLS model are
+// addressed as base[(j*pitch+i)*cs + (l-1)*ls]: the reference f3tensor
+// [j][i][l] layout has cs=L, ls=1; the engine keeps planes, cs=1, ls=plane.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ifos {
+
+constexpr int kMaxL = 8;
+
+struct Cpml {
+    const float *K_x, *a_x, *b_x, *K_xh, *a_xh, *b_xh;
+    const float *K_y, *a_y, *b_y, *K_yh, *a_yh, *b_yh;
+};
+
+struct StepGeom {
+    int NX, NY, pitch;
+    int FW, FREE_SURF, BOUNDARY;
+    int N;            // FDORDER/2
+    float DT, DH;
+    float hc[7];
+    Cpml pml;
+    int px, py;       // CPML strip pitches
+};
+
+// update_v_PML (src/update_v_PML.c:29-596)
+struct VArgs {
+    StepGeom g;
+    int sw, VELOCITY;
+    const float *sxx, *syy, *sxy, *rip, *rjp;
+    float *vx, *vy, *vxp1, *vyp1;   // vxp1/vyp1 may be null: not written
+    float *psi_sxx_x, *psi_sxy_x, *psi_syy_y, *psi_sxy_y;
+};
+
+// update_s_elastic_PML (src/update_s_elastic_PML.c:28-569)
+struct SElArgs {
+    StepGeom g;
+    const float *vx, *vy;
+    float *sxx, *syy, *sxy;
+    float *ux, *uy, *uxy;           // may be null: stress rates not needed
+    // material: either precomputed f=rho*vs*vs, g=rho*(vp^2-2vs^2)
+    // (bit-identical to the per-cell expression) or raw rho/vs/vp
+    const float *f, *gl, *uipjp;
+    const float *rho, *vs, *vp;     // used when f/gl are null
+    float *psi_vxx, *psi_vyx, *psi_vyy, *psi_vxy;
+};
+
+// update_s_visc_PML (src/update_s_visc_PML.c:28-515)
+struct SViscArgs {
+    StepGeom g;
+    int L;
+    const float *vx, *vy;
+    float *sxx, *syy, *sxy;
+    float *ux, *uy, *uxy;
+    const float *fipjp, *f, *gv;
+    float *r, *p, *q;
+    const float *d, *e, *dip;
+    int cs, ls;
+    float bip[kMaxL + 1], bjm[kMaxL + 1], cip[kMaxL + 1], cjm[kMaxL + 1];
+    float *psi_vxx, *psi_vyx, *psi_vyy, *psi_vxy;
+};
+
+// point sources / receivers
+struct PointArgs {
+    int n;
+    const int *ix, *iy;       // grid indices
+    const int *type;          // SOURCE_TYPE per source (forward)
+    const double *sin_azi, *cos_azi;  // sin/cos of the float azimuth in radians,
+                                      // evaluated on the host with libm
+    const float *sig, *sig1;  // [n][ld] time series, 1-based nt -> column nt-1
+    int ld;
+};
+
+// free surface + explosive source + seismogram sampling, fused per step
+struct PostArgs {
+    StepGeom g;
+    int L;
+    int sw;                   // 0 forward, 1 adjoint
+    int explosive;            // psource active (forward, SOURCE_TYPE==1)
+    int NT;
+    PointArgs src;            // explosive sources (forward)
+    int ntr;
+    const int *rx, *ry;       // receivers
+    float *seis_vx, *seis_vy; // [ntr][NT]
+    float *vx, *vy;
+    float *sxx, *syy, *sxy, *ux, *uy, *uxy;
+    // elastic surface material: f (=rho*vs^2) and gl at row 1, or raw
+    const float *f, *gl;
+    const float *rho, *vs, *vp, *taus, *taup;
+    float *p, *q;             // visc memory variables (cs/ls below)
+    int cs, ls;
+    float sumu, sumpi;
+    float eta[kMaxL + 1];
+    float *psi_vxxs;
+    const int *nt_dev;        // when non-null the step number is read here
+    int nt;                   // otherwise this one is used
+    int *nan_flag;            // set to 1 if vy(NY/2,NX/2) is NaN
+};
+
+// snapshot store (src/IFOS2D.c:1805-1871): forward step nt with
+// (nt-1) % DTINV == 0 stores into slot (nt-1)/DTINV
+struct SnapArgs {
+    int NX, NY, pitch, DTINV;
+    const float *a[5];
+    float *base[5];           // [nsnap][NY][NX]
+    const int *nt_dev;        // step number read here when non-null
+    int nt;
+};
+
+// zero-lag correlation (src/IFOS2D.c:2381-2419): adjoint step nt pairs
+// with forward step NT-nt+1; it runs when (NT-nt) % DTINV == 0 and reads
+// snapshot slot (NT-nt)/DTINV
+struct CorrArgs {
+    int NX, NY, pitch, NT, DTINV;
+    const float *vxp1, *vyp1, *sxx, *syy, *sxy;
+    const float *F[5];        // rho_x, rho_y, x, y, u bases [nsnap][NY][NX]
+    const int *nt_dev;
+    int nt;
+    const float *muss, *lamss, *vs;   // [NY][NX]
+    float *wc_rho, *wc, *wc_u;        // [NY][NX]
+};
+
+// chain rule to Vp/Vs/rho and shot sum (src/IFOS2D.c:2471-2611, 2703-2713)
+struct GradArgs {
+    int n;                    // NX*NY
+    float DT;
+    const float *vp, *vs, *rho;       // [NY][NX]
+    const float *wc, *wc_u, *wc_rho;
+    float *g_vp, *g_vs, *g_rho;       // per-shot gradient
+    float *s_vp, *s_vs, *s_rho;       // running sums over shots (may be null)
+};
+
+// adjoint sources from seismograms (src/calc_res.c, LNORM 2)
+struct ResArgs {
+    int ntr, NT, VELOCITY;
+    float DT;
+    const float *syn, *obs;   // [ntr][NT]
+    const int *kill;          // may be null
+    float *diff;              // [ntr][NT], time reversed
+    double *l2_trace, *en_trace;   // per-trace FP64 sums
+    float *l2f_trace, *enf_trace;  // per-trace FP32 running sums
+};
+
+// launchers (ifos_kernels.hip)
+void launch_update_v(const VArgs &a, hipStream_t s);
+void launch_inject(const PointArgs &a, const StepGeom &g, float *vx, float *vy, const float *rip,
+                   const float *rjp, int sw, int ADJOINT_TYPE, const int *nt_dev, int nt, int *nt_tick,
+                   hipStream_t s);
+void launch_update_s_elastic(const SElArgs &a, hipStream_t s);
+void launch_update_s_visc(const SViscArgs &a, hipStream_t s);
+void launch_post(const PostArgs &a, hipStream_t s);
+void launch_snapshot(const SnapArgs &a, hipStream_t s);
+void launch_correlate(const CorrArgs &a, hipStream_t s);
+void launch_gradient(const GradArgs &a, hipStream_t s);
+void launch_residual(const ResArgs &a, hipStream_t s);
+
+}  // namespace ifos

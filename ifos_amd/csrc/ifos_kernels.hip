@@ -1,0 +1,707 @@
+// ifos_kernels.hip -- gfx950 kernels of the P-SV hot path.
+//
+// Arithmetic contract: every kernel evaluates the reference expression in the
+// same operand order, with the same float->double promotions of `2.0*` and
+// `0.5*` literals and true divisions where the reference divides.  The
+// library is compiled with -ffp-contract=off (no FMA contraction, like gcc
+// -std=c99 on x86-64) and with IEEE denormals, so FP32 results are
+// bit-identical to the reference's CPU run.
+//
+// Step counter: kernels that need the time-step number read it from a
+// device int (nt_dev) so that a captured graph of a block of time steps can
+// be replayed.  The single-block inject kernel advances the counter; kernels
+// that run before it within a step (update_v) see the previous value.
+#include "ifos_internal.h"
+
+namespace ifos {
+
+#define IFOS_KERNEL __global__ __launch_bounds__(256)
+
+__device__ __forceinline__ int step_of(const int *nt_dev, int nt) { return nt_dev ? *nt_dev : nt; }
+
+// ---------------------------------------------------------------------------
+// update_v_PML  (src/update_v_PML.c:29-596; orders 2-8 of the reference and
+// the same form for orders 10/12)
+// ---------------------------------------------------------------------------
+template <int N>
+IFOS_KERNEL void k_update_v(VArgs a, const int *nt_dev, int DTINV, int NT)
+{
+    const StepGeom &g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.NX || j > g.NY) return;
+    const int P = g.pitch;
+    const long c = (long)j * P + i;
+    const float *sxx = a.sxx + c, *syy = a.syy + c, *sxy = a.sxy + c;
+
+    float sxx_x = g.hc[1] * (sxx[1] - sxx[0]);
+    float sxy_x = g.hc[1] * (sxy[0] - sxy[-1]);
+    float sxy_y = g.hc[1] * (sxy[0] - sxy[-P]);
+    float syy_y = g.hc[1] * (syy[P] - syy[0]);
+#pragma unroll
+    for (int m = 2; m <= N; m++) {
+        sxx_x = sxx_x + g.hc[m] * (sxx[m] - sxx[1 - m]);
+        sxy_x = sxy_x + g.hc[m] * (sxy[m - 1] - sxy[-m]);
+        sxy_y = sxy_y + g.hc[m] * (sxy[(m - 1) * P] - sxy[-m * P]);
+        syy_y = syy_y + g.hc[m] * (syy[m * P] - syy[(1 - m) * P]);
+    }
+
+    const Cpml &pm = g.pml;
+    if (!g.BOUNDARY && i <= g.FW) {
+        const int h = i - 1;
+        float *p1 = a.psi_sxx_x + (long)(j - 1) * g.px + h;
+        float *p2 = a.psi_sxy_x + (long)(j - 1) * g.px + h;
+        *p1 = pm.b_xh[h] * *p1 + pm.a_xh[h] * sxx_x;
+        sxx_x = sxx_x / pm.K_xh[h] + *p1;
+        *p2 = pm.b_x[h] * *p2 + pm.a_x[h] * sxy_x;
+        sxy_x = sxy_x / pm.K_x[h] + *p2;
+    }
+    if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
+        const int h = i - g.NX + 2 * g.FW - 1;
+        float *p1 = a.psi_sxx_x + (long)(j - 1) * g.px + h;
+        float *p2 = a.psi_sxy_x + (long)(j - 1) * g.px + h;
+        *p1 = pm.b_xh[h] * *p1 + pm.a_xh[h] * sxx_x;
+        sxx_x = sxx_x / pm.K_xh[h] + *p1;
+        *p2 = pm.b_x[h] * *p2 + pm.a_x[h] * sxy_x;
+        sxy_x = sxy_x / pm.K_x[h] + *p2;
+    }
+    if (!g.FREE_SURF && j <= g.FW) {
+        const int h = j - 1;
+        float *p1 = a.psi_syy_y + (long)h * g.py + (i - 1);
+        float *p2 = a.psi_sxy_y + (long)h * g.py + (i - 1);
+        *p1 = pm.b_yh[h] * *p1 + pm.a_yh[h] * syy_y;
+        syy_y = syy_y / pm.K_yh[h] + *p1;
+        *p2 = pm.b_y[h] * *p2 + pm.a_y[h] * sxy_y;
+        sxy_y = sxy_y / pm.K_y[h] + *p2;
+    }
+    if (j >= g.NY - g.FW + 1) {
+        const int h = j - g.NY + 2 * g.FW - 1;
+        float *p1 = a.psi_syy_y + (long)h * g.py + (i - 1);
+        float *p2 = a.psi_sxy_y + (long)h * g.py + (i - 1);
+        *p1 = pm.b_yh[h] * *p1 + pm.a_yh[h] * syy_y;
+        syy_y = syy_y / pm.K_yh[h] + *p1;
+        *p2 = pm.b_y[h] * *p2 + pm.a_y[h] * sxy_y;
+        sxy_y = sxy_y / pm.K_y[h] + *p2;
+    }
+
+    const float rip = a.rip[c], rjp = a.rjp[c];
+    const float vx = a.vx[c], vy = a.vy[c];
+    if (a.vxp1) {
+        // this kernel runs before the step counter advances: nt = *nt_dev + 1
+        const int nt = nt_dev ? *nt_dev + 1 : 0;
+        if (a.sw == 0) {
+            if (!nt_dev || (nt - 1) % DTINV == 0) {
+                a.vxp1[c] = rip * (sxx_x + sxy_y) / g.DH;
+                a.vyp1[c] = rjp * (sxy_x + syy_y) / g.DH;
+            }
+        } else if (a.VELOCITY == 0) {
+            a.vxp1[c] += vx * g.DT;
+            a.vyp1[c] += vy * g.DT;
+        } else if (!nt_dev || (NT - nt) % DTINV == 0) {
+            a.vxp1[c] = vx;
+            a.vyp1[c] = vy;
+        }
+    }
+    a.vx[c] = vx + g.DT * rip * (sxx_x + sxy_y) / g.DH;
+    a.vy[c] = vy + g.DT * rjp * (sxy_x + syy_y) / g.DH;
+}
+
+// ---------------------------------------------------------------------------
+// point forces after the velocity update (src/update_v_PML.c:553-578) and
+// the step counter.  One block.  Forward sources are applied serially in
+// source order (several may share a cell); adjoint sources sit on distinct
+// receiver cells and are applied in parallel unless serial is requested.
+// ---------------------------------------------------------------------------
+IFOS_KERNEL void k_inject(PointArgs a, StepGeom g, float *vx, float *vy, const float *rip, const float *rjp,
+                          int sw, int ADJOINT_TYPE, const int *nt_dev, int nt_arg, int *nt_tick, int serial)
+{
+    __shared__ int s_nt;
+    if (threadIdx.x == 0) s_nt = nt_tick ? (*nt_tick + 1) : step_of(nt_dev, nt_arg);
+    __syncthreads();
+    const int nt = s_nt;
+    const int P = g.pitch;
+    if (sw == 0) {
+        if (threadIdx.x == 0) {
+            for (int l = 0; l < a.n; l++) {
+                const long c = (long)a.iy[l] * P + a.ix[l];
+                const float s0 = a.sig[(long)l * a.ld + nt - 1];
+                const float s1 = a.sig1[(long)l * a.ld + nt - 1];
+                const int type = a.type[l];
+                if (type == 2) vx[c] += (g.DT * rip[c] * s1) / (g.DH * g.DH);
+                if (type == 3) vy[c] += (g.DT * rjp[c] * s1) / (g.DH * g.DH);
+                if (type == 4) {
+                    vx[c] += (g.DT * rip[c] * a.sin_azi[l] * s0) / (g.DH * g.DH);
+                    vy[c] += (g.DT * rjp[c] * a.cos_azi[l] * s0) / (g.DH * g.DH);
+                }
+            }
+        }
+    } else if (serial) {
+        if (threadIdx.x == 0) {
+            for (int l = 0; l < a.n; l++) {
+                const long c = (long)a.iy[l] * P + a.ix[l];
+                const float s0 = a.sig[(long)l * a.ld + nt - 1];
+                const float s1 = a.sig1[(long)l * a.ld + nt - 1];
+                if (ADJOINT_TYPE == 1) { vx[c] += s0; vy[c] += s1; }
+                if (ADJOINT_TYPE == 2) vy[c] += s1;
+                if (ADJOINT_TYPE == 3) vx[c] += s0;
+            }
+        }
+    } else {
+        for (int l = threadIdx.x; l < a.n; l += blockDim.x) {
+            const long c = (long)a.iy[l] * P + a.ix[l];
+            const float s0 = a.sig[(long)l * a.ld + nt - 1];
+            const float s1 = a.sig1[(long)l * a.ld + nt - 1];
+            if (ADJOINT_TYPE == 1) { vx[c] += s0; vy[c] += s1; }
+            if (ADJOINT_TYPE == 2) vy[c] += s1;
+            if (ADJOINT_TYPE == 3) vx[c] += s0;
+        }
+    }
+    __syncthreads();
+    if (nt_tick && threadIdx.x == 0) *nt_tick = nt;
+}
+
+// ---------------------------------------------------------------------------
+// velocity derivatives + stress-side CPML (src/update_s_elastic_PML.c:
+// 151-219; identical in src/update_s_visc_PML.c)
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void vel_derivs(const StepGeom &g, const float *vx, const float *vy, float scale,
+                                           float &vxx, float &vyx, float &vxy, float &vyy)
+{
+    const int P = g.pitch;
+    vxx = g.hc[1] * (vx[0] - vx[-1]);
+    vyx = g.hc[1] * (vy[1] - vy[0]);
+    vxy = g.hc[1] * (vx[P] - vx[0]);
+    vyy = g.hc[1] * (vy[0] - vy[-P]);
+#pragma unroll
+    for (int m = 2; m <= N; m++) {
+        vxx = vxx + g.hc[m] * (vx[m - 1] - vx[-m]);
+        vyx = vyx + g.hc[m] * (vy[m] - vy[1 - m]);
+        vxy = vxy + g.hc[m] * (vx[m * P] - vx[(1 - m) * P]);
+        vyy = vyy + g.hc[m] * (vy[(m - 1) * P] - vy[-m * P]);
+    }
+    vxx = vxx * scale;
+    vyx = vyx * scale;
+    vxy = vxy * scale;
+    vyy = vyy * scale;
+}
+
+__device__ __forceinline__ void cpml_stress(const StepGeom &g, int j, int i, float *psi_vxx, float *psi_vyx,
+                                            float *psi_vyy, float *psi_vxy, float &vxx, float &vyx, float &vxy,
+                                            float &vyy)
+{
+    const Cpml &pm = g.pml;
+    if (!g.BOUNDARY && i <= g.FW) {
+        const int h = i - 1;
+        float *p1 = psi_vxx + (long)(j - 1) * g.px + h;
+        float *p2 = psi_vyx + (long)(j - 1) * g.px + h;
+        *p1 = pm.b_x[h] * *p1 + pm.a_x[h] * vxx;
+        vxx = vxx / pm.K_x[h] + *p1;
+        *p2 = pm.b_xh[h] * *p2 + pm.a_xh[h] * vyx;
+        vyx = vyx / pm.K_xh[h] + *p2;
+    }
+    if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
+        const int h = i - g.NX + 2 * g.FW - 1;
+        float *p1 = psi_vxx + (long)(j - 1) * g.px + h;
+        float *p2 = psi_vyx + (long)(j - 1) * g.px + h;
+        *p1 = pm.b_x[h] * *p1 + pm.a_x[h] * vxx;
+        vxx = vxx / pm.K_x[h] + *p1;
+        *p2 = pm.b_xh[h] * *p2 + pm.a_xh[h] * vyx;
+        vyx = vyx / pm.K_xh[h] + *p2;
+    }
+    if (!g.FREE_SURF && j <= g.FW) {
+        const int h = j - 1;
+        float *p1 = psi_vyy + (long)h * g.py + (i - 1);
+        float *p2 = psi_vxy + (long)h * g.py + (i - 1);
+        *p1 = pm.b_y[h] * *p1 + pm.a_y[h] * vyy;
+        *p2 = pm.b_yh[h] * *p2 + pm.a_yh[h] * vxy;
+        vyy = vyy / pm.K_y[h] + *p1;
+        vxy = vxy / pm.K_yh[h] + *p2;
+    }
+    if (j >= g.NY - g.FW + 1) {
+        const int h = j - g.NY + 2 * g.FW - 1;
+        float *p1 = psi_vyy + (long)h * g.py + (i - 1);
+        float *p2 = psi_vxy + (long)h * g.py + (i - 1);
+        *p1 = pm.b_y[h] * *p1 + pm.a_y[h] * vyy;
+        vyy = vyy / pm.K_y[h] + *p1;
+        *p2 = pm.b_yh[h] * *p2 + pm.a_yh[h] * vxy;
+        vxy = vxy / pm.K_yh[h] + *p2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// update_s_elastic_PML  (src/update_s_elastic_PML.c:28-569)
+// ---------------------------------------------------------------------------
+template <int N>
+IFOS_KERNEL void k_update_s_elastic(SElArgs a, const int *nt_dev, int DTINV)
+{
+    const StepGeom &g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.NX || j > g.NY) return;
+    const long c = (long)j * g.pitch + i;
+    const float dhi = g.DT / g.DH;
+    float vxx, vyx, vxy, vyy;
+    vel_derivs<N>(g, a.vx + c, a.vy + c, dhi, vxx, vyx, vxy, vyy);
+    cpml_stress(g, j, i, a.psi_vxx, a.psi_vyx, a.psi_vyy, a.psi_vxy, vxx, vyx, vxy, vyy);
+
+    const float fipjp = a.uipjp[c];
+    float f, gl;
+    if (a.f) {
+        f = a.f[c];
+        gl = a.gl[c];
+    } else {
+        const float rho = a.rho[c], u = a.vs[c], pi = a.vp[c];
+        f = rho * u * u;
+        gl = rho * ((pi * pi) - 2 * u * u);
+    }
+    const float sxx = a.sxx[c], syy = a.syy[c], sxy = a.sxy[c];
+    a.sxy[c] = sxy + fipjp * (vyx + vxy);
+    a.sxx[c] = (float)((double)sxx + ((double)(gl * (vxx + vyy)) + (2.0 * (double)f * (double)vxx)));
+    a.syy[c] = (float)((double)syy + ((double)(gl * (vxx + vyy)) + (2.0 * (double)f * (double)vyy)));
+    if (a.ux && (!nt_dev || (*nt_dev - 1) % DTINV == 0)) {
+        a.uxy[c] = (fipjp / g.DT) * (vyx + vxy);
+        const double f2 = (2.0 * (double)f) / (double)g.DT;
+        a.ux[c] = (float)((double)((gl / g.DT) * (vxx + vyy)) + (f2 * (double)vxx));
+        a.uy[c] = (float)((double)((gl / g.DT) * (vxx + vyy)) + (f2 * (double)vyy));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// update_s_visc_PML  (src/update_s_visc_PML.c:28-515), generalized
+// standard linear solid with L relaxation mechanisms
+// ---------------------------------------------------------------------------
+template <int N, int L>
+IFOS_KERNEL void k_update_s_visc(SViscArgs a, const int *nt_dev, int DTINV)
+{
+    const StepGeom &g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.NX || j > g.NY) return;
+    const long c = (long)j * g.pitch + i;
+    const float dhi = 1.0 / (double)g.DH;
+    const float dthalbe = g.DT / 2.0;
+    float vxx, vyx, vxy, vyy;
+    vel_derivs<N>(g, a.vx + c, a.vy + c, dhi, vxx, vyx, vxy, vyy);
+    cpml_stress(g, j, i, a.psi_vxx, a.psi_vyx, a.psi_vyy, a.psi_vxy, vxx, vyx, vxy, vyy);
+
+    const long mc = c * a.cs;
+    float rl[L], pl[L], ql[L];
+    float sumr = 0.0f, sump = 0.0f, sumq = 0.0f;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+        rl[l] = a.r[mc + (long)l * a.ls];
+        pl[l] = a.p[mc + (long)l * a.ls];
+        ql[l] = a.q[mc + (long)l * a.ls];
+        sumr += rl[l];
+        sump += pl[l];
+        sumq += ql[l];
+    }
+    const float fipjp = a.fipjp[c], f = a.f[c], gv = a.gv[c];
+    float sxy = a.sxy[c] + ((fipjp * (vxy + vyx)) + (dthalbe * sumr));
+    float sxx = (float)((double)a.sxx[c] +
+                        (((double)(gv * (vxx + vyy)) - (2.0 * (double)f * (double)vyy)) + (double)(dthalbe * sump)));
+    float syy = (float)((double)a.syy[c] +
+                        (((double)(gv * (vxx + vyy)) - (2.0 * (double)f * (double)vxx)) + (double)(dthalbe * sumq)));
+    const bool rates = a.ux && (!nt_dev || (*nt_dev - 1) % DTINV == 0);
+    float uxy = 0.0f, ux = 0.0f, uy = 0.0f;
+    if (rates) {
+        const double f2 = (2.0 * (double)f) / (double)g.DT;
+        uxy = (float)((double)((fipjp / g.DT) * (vxy + vyx)) + (0.5 * (double)sumr));
+        ux = (float)(((double)((gv / g.DT) * (vxx + vyy)) - (f2 * (double)vyy)) + (0.5 * (double)sump));
+        uy = (float)(((double)((gv / g.DT) * (vxx + vyy)) - (f2 * (double)vxx)) + (0.5 * (double)sumq));
+    }
+    sumr = sump = sumq = 0.0f;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+        const long o = mc + (long)l * a.ls;
+        const float dip = a.dip[o], d = a.d[o], e = a.e[o];
+        const float r = a.bip[l + 1] * (rl[l] * a.cip[l + 1] - (dip * (vxy + vyx)));
+        const float p = (float)((double)a.bjm[l + 1] *
+                                ((double)(pl[l] * a.cjm[l + 1] - (e * (vxx + vyy))) + (2.0 * (double)d * (double)vyy)));
+        const float q = (float)((double)a.bjm[l + 1] *
+                                ((double)(ql[l] * a.cjm[l + 1] - (e * (vxx + vyy))) + (2.0 * (double)d * (double)vxx)));
+        a.r[o] = r;
+        a.p[o] = p;
+        a.q[o] = q;
+        sumr += r;
+        sump += p;
+        sumq += q;
+    }
+    a.sxy[c] = sxy + (dthalbe * sumr);
+    a.sxx[c] = sxx + (dthalbe * sump);
+    a.syy[c] = syy + (dthalbe * sumq);
+    if (rates) {
+        a.uxy[c] = (float)((double)uxy + (0.5 * (double)sumr));
+        a.ux[c] = (float)((double)ux + (0.5 * (double)sump));
+        a.uy[c] = (float)((double)uy + (0.5 * (double)sumq));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-step post kernel: explosive source (src/psource.c:26-75), free surface
+// (src/surface_elastic_PML.c:26-124, src/surface_PML.c:25-177), receiver
+// sampling (src/seismo_ssg.c:26-55) and the stability probe
+// (src/IFOS2D.c:1660-1665).  Threads [0,NX) own one column each: the
+// explosive sources located in that column are applied first, in source
+// order, then the free surface of that column, which only reads and writes
+// that column (plus vx of the surface row, untouched here).  Threads
+// [NX, NX+ntr) sample the receivers.
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void surface_column(const PostArgs &a, int i, float &vxx_out, float &vyy_out)
+{
+    const StepGeom &g = a.g;
+    const int P = g.pitch;
+    const long c1 = (long)1 * P + i;
+    a.syy[c1] = 0.0f;
+    if (a.uy) a.uy[c1] = 0.0f;
+    float vxx = 0.0f, vyy = 0.0f;
+#pragma unroll
+    for (int m = 1; m <= N; m++) {
+        a.syy[c1 - (long)m * P] = -a.syy[c1 + (long)m * P];
+        a.sxy[c1 - (long)m * P] = -a.sxy[c1 + (long)(m - 1) * P];
+        if (a.uy) a.uy[c1 - (long)m * P] = -a.uy[c1 + (long)m * P];
+        if (a.uxy) a.uxy[c1 - (long)m * P] = -a.uxy[c1 + (long)(m - 1) * P];
+        vxx += g.hc[m] * (a.vx[c1 + m - 1] - a.vx[c1 - m]);
+        vyy += g.hc[m] * (a.vy[c1 + (long)(m - 1) * P] - a.vy[c1 - (long)m * P]);
+    }
+    const float dh24 = 1.0 / (double)g.DH;
+    vxx *= dh24;
+    vyy *= dh24;
+    const Cpml &pm = g.pml;
+    if (!g.BOUNDARY && i <= g.FW) {
+        const int h = i - 1;
+        float *ps = a.psi_vxxs + h;
+        *ps = pm.b_x[h] * *ps + pm.a_x[h] * vxx;
+        vxx = vxx / pm.K_x[h] + *ps;
+    }
+    if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
+        const int h = i - g.NX + 2 * g.FW - 1;
+        float *ps = a.psi_vxxs + h;
+        *ps = pm.b_x[h] * *ps + pm.a_x[h] * vxx;
+        vxx = vxx / pm.K_x[h] + *ps;
+    }
+    vxx_out = vxx;
+    vyy_out = vyy;
+}
+
+template <int N>
+IFOS_KERNEL void k_post(PostArgs a)
+{
+    const StepGeom &g = a.g;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = step_of(a.nt_dev, a.nt);
+    const int P = g.pitch;
+    if (t < g.NX) {
+        const int i = t + 1;
+        if (a.explosive) {
+            for (int l = 0; l < a.src.n; l++) {
+                if (a.src.ix[l] != i) continue;
+                const long c = (long)a.src.iy[l] * P + i;
+                const float *s = a.src.sig + (long)l * a.src.ld - 1;  // 1-based
+                float amp = 0.0f;
+                const double den = 2.0 * (double)g.DH * (double)g.DH;
+                if (nt == 1 && a.NT > 1) amp = (float)((double)s[nt + 1] / den);
+                if ((nt > 1) && (nt < a.NT)) amp = (float)((double)(s[nt + 1] - s[nt - 1]) / den);
+                if (nt == a.NT) amp = (float)((double)(-s[nt - 1]) / den);
+                a.sxx[c] = (float)((double)a.sxx[c] + 0.5 * (double)amp);
+                a.syy[c] = (float)((double)a.syy[c] + 0.5 * (double)amp);
+            }
+        }
+        if (g.FREE_SURF) {
+            float vxx, vyy;
+            surface_column<N>(a, i, vxx, vyy);
+            const long c1 = (long)P + i;
+            const float DT = g.DT;
+            if (a.L == 0) {
+                float fjm, gl;
+                if (a.f) {
+                    fjm = (float)((double)a.f[c1] * 2.0);
+                    gl = a.gl[c1];
+                } else {
+                    const float rho = a.rho[c1], u = a.vs[c1], pi = a.vp[c1];
+                    fjm = (float)((double)(rho * u * u) * 2.0);
+                    gl = rho * ((pi * pi) - 2 * u * u);
+                }
+                a.sxx[c1] = a.sxx[c1] + (-DT * ((gl * gl) / (gl + fjm) * vxx + gl * vyy));
+                if (a.ux)
+                    a.ux[c1] = (float)(((2.0 * (double)gl * (double)fjm + (double)(fjm * fjm)) / (double)(gl + fjm)) *
+                                       (double)vxx);
+            } else {
+                const int L = a.L;
+                const long mc = c1 * a.cs;
+                for (int l = 0; l < L; l++) a.q[mc + (long)l * a.ls] = 0.0f;
+                const float u = a.vs[c1], pi = a.vp[c1], rho = a.rho[c1];
+                const float taus = a.taus[c1], taup = a.taup[c1];
+                const float mu = (float)((double)(u * u * rho) / (1.0 + (double)(a.sumu * taus)));
+                const float pim = (float)((double)(pi * pi * rho) / (1.0 + (double)(a.sumpi * taup)));
+                float sump = 0.0f;
+                for (int l = 0; l < L; l++) sump += a.p[mc + (long)l * a.ls];
+                const float fjm = (float)((double)mu * 2.0 * (1.0 + (double)(L * taus)));
+                const float gv = (float)((double)pim * (1.0 + (double)(L * taup)));
+                const float dthalbe = DT / 2.0;
+                a.sxx[c1] = a.sxx[c1] + ((-(DT * (gv - fjm) * (gv - fjm) * vxx / gv) - (DT * (gv - fjm) * vyy)) -
+                                         (dthalbe * sump));
+                if (a.ux)
+                    a.ux[c1] = (float)((double)a.ux[c1] +
+                                       ((double)(-((gv - fjm) * (gv - fjm) * vxx / gv) - ((gv - fjm) * vyy)) -
+                                        (0.5 * (double)sump)));
+                sump = 0.0f;
+                for (int l = 1; l <= L; l++) {
+                    const float bjm = (float)((double)a.eta[l] / (1.0 + ((double)a.eta[l] * 0.5)));
+                    const float djm = (float)(2.0 * (double)mu * (double)taus);
+                    const float e = pim * taup;
+                    float *pp = a.p + mc + (long)(l - 1) * a.ls;
+                    *pp = (float)((double)*pp + (double)bjm * ((((double)(djm - e) * ((double)(fjm / gv) - 1.0)) *
+                                                                 (double)vxx) -
+                                                                (double)((djm - e) * vyy)));
+                    sump += *pp;
+                }
+                a.sxx[c1] = a.sxx[c1] + (dthalbe * sump);
+                if (a.ux) a.ux[c1] = (float)((double)a.ux[c1] + (0.5 * (double)sump));
+            }
+        }
+    } else if (t < g.NX + a.ntr) {
+        const int itr = t - g.NX;
+        const long c = (long)a.ry[itr] * P + a.rx[itr];
+        a.seis_vx[(long)itr * a.NT + nt - 1] = a.vx[c];
+        a.seis_vy[(long)itr * a.NT + nt - 1] = a.vy[c];
+    } else if (t == g.NX + a.ntr && a.nan_flag) {
+        const float v = a.vy[(long)(g.NY / 2) * P + g.NX / 2];
+        if (isnan(v) && *a.nan_flag == 0) *a.nan_flag = nt;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// snapshot store  (src/IFOS2D.c:1805-1871), slot-major [k][j][i]
+// ---------------------------------------------------------------------------
+IFOS_KERNEL void k_snapshot(SnapArgs a)
+{
+    const int nt = step_of(a.nt_dev, a.nt);
+    if ((nt - 1) % a.DTINV != 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > a.NX || j > a.NY) return;
+    const long plane = (long)a.NX * a.NY;
+    const long k = (long)((nt - 1) / a.DTINV);
+    const long o = k * plane + (long)(j - 1) * a.NX + (i - 1);
+    const long c = (long)j * a.pitch + i;
+#pragma unroll
+    for (int f = 0; f < 5; f++) a.base[f][o] = a.a[f][c];
+}
+
+// ---------------------------------------------------------------------------
+// zero-lag correlation of the adjoint with the stored forward wavefield
+// (src/IFOS2D.c:2381-2419; the parenthesisation of the mu term is the coded
+// one of lines 2398-2404)
+// ---------------------------------------------------------------------------
+IFOS_KERNEL void k_correlate(CorrArgs a)
+{
+    const int nt = step_of(a.nt_dev, a.nt);
+    if ((a.NT - nt) % a.DTINV != 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > a.NX || j > a.NY) return;
+    const long plane = (long)a.NX * a.NY;
+    const long e = (long)(j - 1) * a.NX + (i - 1);
+    const long o = (long)((a.NT - nt) / a.DTINV) * plane + e;
+    const long c = (long)j * a.pitch + i;
+    const float Frx = a.F[0][o], Fry = a.F[1][o], Fx = a.F[2][o], Fy = a.F[3][o], Fu = a.F[4][o];
+    const float sxx = a.sxx[c], syy = a.syy[c], sxy = a.sxy[c];
+    a.wc_rho[e] = a.wc_rho[e] + ((a.vxp1[c] * Frx) + (a.vyp1[c] * Fry));
+    a.wc[e] = a.wc[e] + ((Fx + Fy) * (sxx + syy));
+    if (a.vs[e] > 0.0f) {
+        const float muss = a.muss[e], lamss = a.lamss[e];
+        const double t1 = (1.0 / (double)(muss * muss)) * (double)(Fu * sxy);
+        const double t2 = (0.25 * (double)(Fx + Fy * (sxx + syy))) / (double)((lamss + muss) * (lamss + muss));
+        const double t3 = (0.25 * (double)(Fx - Fy * (sxx - syy))) / (double)(muss * muss);
+        a.wc_u[e] = (float)((double)a.wc_u[e] + ((t1 + t2) + t3));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// chain rule from lambda/mu/rho' to Vp/Vs/rho (src/IFOS2D.c:2471-2611,
+// PARAMETERIZATION 1) and the sum over shots (src/IFOS2D.c:2703-2713)
+// ---------------------------------------------------------------------------
+IFOS_KERNEL void k_gradient(GradArgs a)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.n) return;
+    const float DT = a.DT;
+    const float rh = a.rho[e], u = a.vs[e], pv = a.vp[e];
+    float wl = -DT * a.wc[e];
+    const float muss = rh * u * u;
+    const float lamss = (float)((double)(rh * pv * pv) - 2.0 * (double)muss);
+    wl = (float)((1.0 / (4.0 * (double)(lamss + muss) * (double)(lamss + muss))) * (double)wl);
+    const float gvp = (float)(2.0 * (double)pv * (double)rh * (double)wl);
+    const float wmu = -DT * a.wc_u[e];
+    const float gvs = (float)(-4.0 * (double)rh * (double)u * (double)wl + 2.0 * (double)rh * (double)u * (double)wmu);
+    const float wrs = -DT * a.wc_rho[e];
+    const float grho = (float)(((double)(pv * pv) - (2.0 * (double)u * (double)u)) * (double)wl + (double)(u * u * wmu) +
+                               (double)wrs);
+    a.g_vp[e] = gvp;
+    a.g_vs[e] = gvs;
+    a.g_rho[e] = grho;
+    if (a.s_vp) {
+        a.s_vp[e] = a.s_vp[e] + gvp;
+        a.s_vs[e] = a.s_vs[e] + gvs;
+        a.s_rho[e] = a.s_rho[e] + grho;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// residual / adjoint source (src/calc_res.c:136-282, LNORM 2) with the
+// misfit and data energy (src/calc_misfit.c:193-194, src/calc_energy.c:157)
+// summed per trace in FP64.  One thread per trace: the time integration is
+// a sequential FP32 running sum exactly as in the reference.
+// ---------------------------------------------------------------------------
+IFOS_KERNEL void k_residual(ResArgs a)
+{
+    const int itr = blockIdx.x * blockDim.x + threadIdx.x;
+    if (itr >= a.ntr) return;
+    const float *syn = a.syn + (long)itr * a.NT;
+    const float *obs = a.obs + (long)itr * a.NT;
+    float *diff = a.diff + (long)itr * a.NT;
+    double l2 = 0.0, en = 0.0;
+    if (a.kill && a.kill[itr] == 1) {
+        for (int k = 0; k < a.NT; k++) diff[k] = 0.0f;
+    } else {
+        float is = 0.0f, id = 0.0f;
+        for (int k = 0; k < a.NT; k++) {
+            float ys, yd;
+            if (a.VELOCITY == 0) {
+                is += syn[k];
+                id += obs[k];
+                ys = is * a.DT;
+                yd = id * a.DT;
+            } else {
+                ys = syn[k];
+                yd = obs[k];
+            }
+            const float d = ys - yd;
+            diff[a.NT - 1 - k] = d;
+            l2 += (double)d * (double)d;
+            en += (double)yd * (double)yd;
+        }
+    }
+    a.l2_trace[itr] = l2;
+    a.en_trace[itr] = en;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline dim3 grid2d(int NX, int NY, dim3 b) { return dim3((NX + b.x - 1) / b.x, (NY + b.y - 1) / b.y); }
+
+#define IFOS_DISPATCH_N(N, CALL) \
+    switch (N) {                 \
+    case 1: CALL(1); break;      \
+    case 2: CALL(2); break;      \
+    case 3: CALL(3); break;      \
+    case 4: CALL(4); break;      \
+    case 5: CALL(5); break;      \
+    case 6: CALL(6); break;      \
+    default: break;              \
+    }
+
+struct StepCtl {
+    const int *nt_dev;
+    int DTINV, NT;
+};
+static thread_local StepCtl g_ctl = {nullptr, 1, 1};
+
+void set_step_control(const int *nt_dev, int DTINV, int NT)
+{
+    g_ctl.nt_dev = nt_dev;
+    g_ctl.DTINV = DTINV;
+    g_ctl.NT = NT;
+}
+
+void launch_update_v(const VArgs &a, hipStream_t s)
+{
+    dim3 b(64, 4), gr = grid2d(a.g.NX, a.g.NY, b);
+#define L_(n) hipLaunchKernelGGL(k_update_v<n>, gr, b, 0, s, a, g_ctl.nt_dev, g_ctl.DTINV, g_ctl.NT)
+    IFOS_DISPATCH_N(a.g.N, L_)
+#undef L_
+}
+
+void launch_inject(const PointArgs &a, const StepGeom &g, float *vx, float *vy, const float *rip,
+                   const float *rjp, int sw, int ADJOINT_TYPE, const int *nt_dev, int nt, int *nt_tick,
+                   hipStream_t s)
+{
+    hipLaunchKernelGGL(k_inject, dim3(1), dim3(256), 0, s, a, g, vx, vy, rip, rjp, sw, ADJOINT_TYPE, nt_dev, nt,
+                       nt_tick, 0);
+}
+
+void launch_inject_serial(const PointArgs &a, const StepGeom &g, float *vx, float *vy, const float *rip,
+                          const float *rjp, int sw, int ADJOINT_TYPE, const int *nt_dev, int nt, int *nt_tick,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(k_inject, dim3(1), dim3(256), 0, s, a, g, vx, vy, rip, rjp, sw, ADJOINT_TYPE, nt_dev, nt,
+                       nt_tick, 1);
+}
+
+void launch_update_s_elastic(const SElArgs &a, hipStream_t s)
+{
+    dim3 b(64, 4), gr = grid2d(a.g.NX, a.g.NY, b);
+#define L_(n) hipLaunchKernelGGL(k_update_s_elastic<n>, gr, b, 0, s, a, g_ctl.nt_dev, g_ctl.DTINV)
+    IFOS_DISPATCH_N(a.g.N, L_)
+#undef L_
+}
+
+template <int N>
+static void launch_visc_n(const SViscArgs &a, hipStream_t s)
+{
+    dim3 b(64, 4), gr = grid2d(a.g.NX, a.g.NY, b);
+    switch (a.L) {
+    case 1: hipLaunchKernelGGL((k_update_s_visc<N, 1>), gr, b, 0, s, a, g_ctl.nt_dev, g_ctl.DTINV); break;
+    case 2: hipLaunchKernelGGL((k_update_s_visc<N, 2>), gr, b, 0, s, a, g_ctl.nt_dev, g_ctl.DTINV); break;
+    case 3: hipLaunchKernelGGL((k_update_s_visc<N, 3>), gr, b, 0, s, a, g_ctl.nt_dev, g_ctl.DTINV); break;
+    case 4: hipLaunchKernelGGL((k_update_s_visc<N, 4>), gr, b, 0, s, a, g_ctl.nt_dev, g_ctl.DTINV); break;
+    case 5: hipLaunchKernelGGL((k_update_s_visc<N, 5>), gr, b, 0, s, a, g_ctl.nt_dev, g_ctl.DTINV); break;
+    default: break;
+    }
+}
+
+void launch_update_s_visc(const SViscArgs &a, hipStream_t s)
+{
+#define L_(n) launch_visc_n<n>(a, s)
+    IFOS_DISPATCH_N(a.g.N, L_)
+#undef L_
+}
+
+void launch_post(const PostArgs &a, hipStream_t s)
+{
+    const int n = a.g.NX + a.ntr + 1;
+    dim3 b(256), gr((n + 255) / 256);
+#define L_(nn) hipLaunchKernelGGL(k_post<nn>, gr, b, 0, s, a)
+    IFOS_DISPATCH_N(a.g.N, L_)
+#undef L_
+}
+
+void launch_snapshot(const SnapArgs &a, hipStream_t s)
+{
+    dim3 b(64, 4), gr = grid2d(a.NX, a.NY, b);
+    hipLaunchKernelGGL(k_snapshot, gr, b, 0, s, a);
+}
+
+void launch_correlate(const CorrArgs &a, hipStream_t s)
+{
+    dim3 b(64, 4), gr = grid2d(a.NX, a.NY, b);
+    hipLaunchKernelGGL(k_correlate, gr, b, 0, s, a);
+}
+
+void launch_gradient(const GradArgs &a, hipStream_t s)
+{
+    dim3 b(256), gr((unsigned)((a.n + 255) / 256));
+    hipLaunchKernelGGL(k_gradient, gr, b, 0, s, a);
+}
+
+void launch_residual(const ResArgs &a, hipStream_t s)
+{
+    dim3 b(64), gr((a.ntr + 63) / 64);
+    hipLaunchKernelGGL(k_residual, gr, b, 0, s, a);
+}
+
+}  // namespace ifos

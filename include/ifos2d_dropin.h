@@ -1,0 +1,105 @@
+/*
+ * ifos2d_dropin.h -- the reference's hot-path entry points, same names and
+ * argument lists as IFOS2D 2.0.3's src/fd.h, implemented on the MI355X by
+ * libifos2d_hip.so.
+ *
+ * Wavefield, material, memory-variable and CPML-memory arrays must be device
+ * arrays from ifos_dev_matrix()/ifos_dev_f3tensor() (same index bounds as the
+ * reference's matrix()/f3tensor() calls, src/IFOS2D.c:371-641).  The CPML
+ * profile vectors, hc, bip..cjm, etajm/peta, srcpos_loc, signals and the
+ * seismogram sections stay host arrays as in the reference.  Calls are queued
+ * in order on one device stream; call ifos_dev_sync() before the host reads
+ * device data and use ifos_dev_read() for single values (the stability probe
+ * of src/IFOS2D.c:1660).
+ *
+ * Globals read by these functions, by the reference names (src/globvar.h):
+ * DT, DH, NX, NY, NT, FDORDER, FW, FREE_SURF, BOUNDARY, L, FL, F_REF,
+ * PARAMETERIZATION (1), VELOCITY, ADJOINT_TYPE, SOURCE_TYPE (written by
+ * update_v_PML as in src/update_v_PML.c:558), NPROCX = NPROCY = 1, SEISMO
+ * (1 or 2).  The library defines them; a driver that defines them itself
+ * overrides them.
+ */
+#ifndef IFOS2D_DROPIN_H
+#define IFOS2D_DROPIN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* device Numerical-Recipes arrays (shapes of src/util.c:95,184,279) */
+float **ifos_dev_matrix(int nrl, int nrh, int ncl, int nch);
+float ***ifos_dev_f3tensor(int nrl, int nrh, int ncl, int nch, int ndl, int ndh);
+float *ifos_dev_vector(int nl, int nh);
+void ifos_dev_free_array(void *arr);
+/* copy a whole device array from (dir 0) or to (dir 1) host memory laid out
+ * like the reference allocation: host_first_elem = &m[nrl][ncl] */
+int ifos_dev_copy_array(void *dev_arr, const void *host_first_elem, int dir);
+int ifos_dev_sync(void);
+float ifos_dev_read(float **m, int j, int i);
+
+/* replaces update_v_PML()            src/update_v_PML.c:29, src/fd.h:384-389 */
+void update_v_PML(int nx1, int nx2, int ny1, int ny2, int nt, float **vx, float **vxp1, float **vxm1, float **vy,
+                  float **vyp1, float **vym1, float **uttx, float **utty, float **sxx, float **syy, float **sxy,
+                  float **rip, float **rjp, float **srcpos_loc, float **signals, float **signals1, int nsrc,
+                  float **absorb_coeff, float *hc, int infoout, int sw, float *K_x, float *a_x, float *b_x,
+                  float *K_x_half, float *a_x_half, float *b_x_half, float *K_y, float *a_y, float *b_y,
+                  float *K_y_half, float *a_y_half, float *b_y_half, float **psi_sxx_x, float **psi_syy_y,
+                  float **psi_sxy_y, float **psi_sxy_x);
+
+/* replaces update_s_elastic_PML()    src/update_s_elastic_PML.c:28, src/fd.h:352-357 */
+void update_s_elastic_PML(int nx1, int nx2, int ny1, int ny2, float **vx, float **vy, float **ux, float **uy,
+                          float **uxy, float **uyx, float **sxx, float **syy, float **sxy, float **pi, float **u,
+                          float **uipjp, float **absorb_coeff, float **rho, float *hc, int infoout, float *K_x,
+                          float *a_x, float *b_x, float *K_x_half, float *a_x_half, float *b_x_half, float *K_y,
+                          float *a_y, float *b_y, float *K_y_half, float *a_y_half, float *b_y_half,
+                          float **psi_vxx, float **psi_vyy, float **psi_vxy, float **psi_vyx);
+
+/* replaces update_s_visc_PML()       src/update_s_visc_PML.c:28, src/fd.h:363-369 */
+void update_s_visc_PML(int nx1, int nx2, int ny1, int ny2, float **vx, float **vy, float **ux, float **uy,
+                       float **uxy, float **uyx, float **sxx, float **syy, float **sxy, float **pi, float **u,
+                       float **uipjp, float **rho, float *hc, int infoout, float ***r, float ***p, float ***q,
+                       float **fipjp, float **f, float **g, float *bip, float *bjm, float *cip, float *cjm,
+                       float ***d, float ***e, float ***dip, float *K_x, float *a_x, float *b_x, float *K_x_half,
+                       float *a_x_half, float *b_x_half, float *K_y, float *a_y, float *b_y, float *K_y_half,
+                       float *a_y_half, float *b_y_half, float **psi_vxx, float **psi_vyy, float **psi_vxy,
+                       float **psi_vyx);
+
+/* replaces psource()                 src/psource.c:26, src/fd.h:206-207 */
+void psource(int nt, float **sxx, float **syy, float **sp, float **srcpos_loc, float **signals, int nsrc, int sw);
+
+/* replaces surface_elastic_PML()     src/surface_elastic_PML.c:26, src/fd.h:286 */
+void surface_elastic_PML(int ndepth, float **vx, float **vy, float **sxx, float **syy, float **sxy, float **syz,
+                         float **pi, float **u, float **rho, float *hc, float *K_x, float *a_x, float *b_x,
+                         float **psi_vxx, float **ux, float **uy, float **uxy, float **uyz, float **sxz,
+                         float **uxz);
+
+/* replaces surface_PML()             src/surface_PML.c:25, src/fd.h:288 */
+void surface_PML(int ndepth, float **vx, float **vy, float **sxx, float **syy, float **sxy, float **syz,
+                 float ***p, float ***q, float **ppi, float **pu, float **prho, float **ptaup, float **ptaus,
+                 float *etajm, float *peta, float *hc, float *K_x, float *a_x, float *b_x, float **psi_vxx,
+                 float **ux, float **uy, float **uxy, float **uyz, float **sxz, float **uxz);
+
+/* replaces seismo_ssg()              src/seismo_ssg.c:26, src/fd.h:262-265 (SEISMO 1, 2) */
+void seismo_ssg(int lsamp, int ntr, int **recpos, float **sectionvx, float **sectionvy, float **sectionvz,
+                float **sectionp, float **sectioncurl, float **sectiondiv, float **vx, float **vy, float **vz,
+                float **sxx, float **syy, float **sp, float **pi, float **u, float *hc);
+
+/* replaces zero_fdveps()             src/zero_fdveps.c:28, src/fd.h:428-429 */
+void zero_fdveps(int ny1, int ny2, int nx1, int nx2, float **vx, float **vy, float **vz, float **sxx, float **syy,
+                 float **sxy, float **sxz, float **syz, float **vxm1, float **vym1, float **uxy, float **vxp1,
+                 float **vyp1, float **psi_sxx_x, float **psi_sxy_x, float **psi_sxz_x, float **psi_vxx,
+                 float **psi_vyx, float **psi_vzx, float **psi_syy_y, float **psi_sxy_y, float **psi_syz_y,
+                 float **psi_vyy, float **psi_vxy, float **psi_vzy, float **psi_vxxs);
+
+/* replaces zero_fdveps_visc()        src/zero_fdveps_visc.c:28, src/fd.h:430-433 */
+void zero_fdveps_visc(int ny1, int ny2, int nx1, int nx2, float **vx, float **vy, float **vz, float **sxx,
+                      float **syy, float **sxy, float **sxz, float **syz, float **vxm1, float **vym1, float **uxy,
+                      float **vxp1, float **vyp1, float **psi_sxx_x, float **psi_sxy_x, float **psi_sxz_x,
+                      float **psi_vxx, float **psi_vyx, float **psi_vzx, float **psi_syy_y, float **psi_sxy_y,
+                      float **psi_syz_y, float **psi_vyy, float **psi_vxy, float **psi_vzy, float **psi_vxxs,
+                      float ***pr, float ***pp, float ***pq, float ***pt, float ***po);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

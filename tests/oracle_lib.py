@@ -54,11 +54,18 @@ class _Lib:
         self.lib = C.CDLL(path)
 
 
+_ORC = {}
+
+
 def oracle(fp64=False):
-    path = os.path.join(ORACLE_DIR, "liboracle64.so" if fp64 else "liboracle.so")
-    if not os.path.exists(path):
-        raise RuntimeError("oracle not built: run make -C oracle")
-    return C.CDLL(path)
+    if fp64 not in _ORC:
+        path = os.path.join(ORACLE_DIR, "liboracle64.so" if fp64 else "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run make -C oracle")
+        lib = C.CDLL(path)
+        lib.orc_calc_res.restype = C.c_double
+        _ORC[fp64] = lib
+    return _ORC[fp64]
 
 
 def ref_available():
@@ -149,7 +156,6 @@ def calc_res(p, syn, obs, kill=None):
     ntr = syn.shape[0]
     diff = np.zeros_like(syn)
     k = None if kill is None else np.ascontiguousarray(kill, dtype=np.int32)
-    oracle().orc_calc_res.restype = C.c_double
     l2 = oracle().orc_calc_res(C.byref(p), C.c_int(ntr), _ptr(syn), _ptr(obs), _ptr(k, C.c_int), _ptr(diff))
     return diff, l2
 

@@ -1,0 +1,281 @@
+"""Pinning the oracle (oracle/ifos_oracle.c).
+
+  * against the reference's own compiled kernels (oracle/_ref), bit for bit;
+  * against the committed golden vectors (written from oracle/_ref);
+  * against closed forms: exact Taylor rationals and polynomial exactness of
+    the staggered operators, the Komatitsch-Martin CPML profile in FP64, the
+    Ricker wavelet, numpy prefix sums / reversal / sums of squares;
+  * the gradient's sign and scale against finite differences of the FP64
+    misfit (the reference adjoint is a back-propagation, not the exact
+    discrete adjoint, so agreement is by correlation across directions).
+"""
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from ifos_amd import workloads
+
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def params_of(w):
+    cfg = w.cfg
+    return O.make_params(FL=[cfg.FL[i] for i in range(1, 8)],
+                         **{k: getattr(cfg, k) for k, _ in cfg._fields_ if k != "FL"})
+
+
+# ---------------------------------------------------------------- holberg --
+TAYLOR = {2: [1.0], 4: [9 / 8, -1 / 24], 6: [75 / 64, -25 / 384, 3 / 640],
+          8: [1225 / 1024, -245 / 3072, 49 / 5120, -5 / 7168],
+          10: [19845 / 16384, -735 / 8192, 567 / 40960, -405 / 229376, 35 / 294912],
+          12: [160083 / 131072, -12705 / 131072, 22869 / 1310720, -5445 / 1835008, 847 / 2359296,
+               -63 / 2883584]}
+
+
+@pytest.mark.parametrize("fd", [2, 4, 6, 8, 10, 12])
+def test_taylor_coefficients_exact_and_polynomial_exact(fd):
+    hc = O.holberg(fd, 0).astype(np.float64)
+    n = fd // 2
+    np.testing.assert_array_equal(hc[1:n + 1].astype(np.float32), np.float32(TAYLOR[fd]))
+    # staggered first derivative is exact for x^(2p-1), p = 1..N:
+    # sum_m c_m (2m-1)^(2p-1) = delta_{p1}
+    for pexp in range(1, n + 1):
+        s = sum(hc[m] * (2 * m - 1) ** (2 * pexp - 1) for m in range(1, n + 1))
+        assert abs(s - (1.0 if pexp == 1 else 0.0)) < 2e-6 * (2 * n) ** (2 * pexp - 1)
+
+
+def test_holberg_h_sums_match_manual():
+    # sum |hc| of the manual's stability table (doc/latex/2_Theoretical_Background.tex:268-280)
+    manual = {(2, 0): 1.0, (4, 0): 7 / 6, (4, 1): 1.184614, (8, 1): 1.345927}
+    for (fd, e), h in manual.items():
+        hc = O.holberg(fd, e).astype(np.float64)
+        assert abs(np.abs(hc[1:]).sum() - h) < 2e-6
+
+
+@needs_ref
+@pytest.mark.parametrize("fd,e", list(itertools.product([2, 4, 6, 8, 10, 12], [0, 1, 2, 3, 4])))
+def test_holberg_matches_reference(fd, e):
+    np.testing.assert_array_equal(O.holberg(fd, e), O.holberg(fd, e, "ref"))
+
+
+# -------------------------------------------------------------------- PML --
+def pml_closed_form(p):
+    """Komatitsch & Martin (2007) profiles in FP64, with the reference's a=0
+    for |d|<1 and its left/right, top/bottom index maps."""
+    FW, DH, DT = p.FW, p.DH, p.DT
+    thick = FW * DH
+    d0 = -(p.npower + 1) * p.VPPML * math.log(0.001) / (2 * thick)
+    amax = math.pi * p.FPML
+    out = np.zeros((12, 2 * FW))
+
+    def fill(N, clamp, col):
+        for lower in (True, False):
+            idx = range(1, FW + 1) if lower else range(N - FW + 1, N + 1)
+            origin = thick if lower else (N - 1) * DH - thick
+            for i in idx:
+                h = i if lower else i - N + 2 * FW
+                xval = DH * (i - 1)
+                for half in (0, 1):
+                    a = (origin - (xval + half * DH / 2)) if lower else (xval + half * DH / 2 - origin)
+                    d = K = al = 0.0
+                    K = 1.0
+                    if a >= 0:
+                        an = a / thick
+                        d = d0 * an ** p.npower
+                        K = 1 + (p.k_max_PML - 1) * an ** p.npower
+                        al = amax * (1 - an)
+                    if clamp and al < 0:
+                        al = 0.0
+                    b = math.exp(-(d / K + al) * DT)
+                    aa = d * (b - 1) / (K * (d + K * al)) if abs(int(d)) >= 1 else 0.0
+                    out[col + 3 * half + 0, h - 1] = K
+                    out[col + 3 * half + 1, h - 1] = aa
+                    out[col + 3 * half + 2, h - 1] = b
+    fill(p.NX, True, 0)
+    fill(p.NY, False, 6)
+    return out
+
+
+def test_pml_matches_closed_form():
+    p = O.make_params(NX=120, NY=90, FDORDER=4, FW=15, NT=10, DH=2.0, DT=2e-4, VPPML=3000.0, FPML=20.0)
+    got = O.pml_pro(p).astype(np.float64)
+    ref = pml_closed_form(p)
+    np.testing.assert_allclose(got, ref, rtol=2e-5, atol=1e-6)
+    # a = 0 exactly where |d| < 1 (integer abs of the reference)
+    assert np.all(got[1][np.abs(ref[1]) == 0] == 0)
+
+
+@needs_ref
+@pytest.mark.parametrize("FW,DH,npower,kmax", [(10, 1.0, 2.0, 1.0), (20, 2.0, 3.0, 2.0), (7, 8.0, 2.0, 1.0)])
+def test_pml_matches_reference(FW, DH, npower, kmax):
+    p = O.make_params(NX=100, NY=80, FDORDER=4, FW=FW, NT=10, DH=DH, DT=1e-4 * DH, npower=npower, k_max_PML=kmax)
+    np.testing.assert_array_equal(O.pml_pro(p), O.pml_pro(p, "ref"))
+
+
+# ---------------------------------------------------------------- wavelet --
+def test_ricker_closed_form():
+    p = O.make_params(NX=10, NY=10, FDORDER=4, FW=2, NT=400, DH=1.0, DT=1e-4, SOURCE_SHAPE=1)
+    src = np.array([[5, 5, 0, 0.002, 30.0, 2.5, 0, 1]], np.float32)
+    w = O.wavelet(p, src)[0].astype(np.float64)
+    t = np.arange(1, 401, dtype=np.float32) * np.float32(1e-4)
+    ts = 1 / 30.0
+    tau = np.pi * (t.astype(np.float64) - 1.5 * ts - 0.002) / ts
+    np.testing.assert_allclose(w, 2.5 * (1 - 2 * tau ** 2) * np.exp(-tau ** 2), rtol=1e-5, atol=1e-6)
+
+
+@needs_ref
+@pytest.mark.parametrize("shape", [1, 2, 4, 5, 6, 8])
+def test_wavelet_matches_reference(shape):
+    p = O.make_params(NX=10, NY=10, FDORDER=4, FW=2, NT=500, DH=1.0, DT=1e-4, SOURCE_SHAPE=shape)
+    src = np.array([[5, 5, 0, 0.003, 25.0, 1.5, 0, 1], [6, 6, 0, 0.0, 40.0, -2.0, 0, 1]], np.float32)
+    np.testing.assert_array_equal(O.wavelet(p, src), O.wavelet(p, src, "ref"))
+
+
+# ---------------------------------------------------------------- forward --
+@needs_ref
+@pytest.mark.parametrize("fd,L,vel,fs,st", list(itertools.product([2, 4, 6, 8], [0, 2], [0, 1], [0, 1], [1, 2, 4])))
+def test_forward_matches_reference(fd, L, vel, fs, st):
+    w = workloads.small_case(seed=fd * 10 + L, NX=44, NY=32, NT=90, FDORDER=fd, L=L, VELOCITY=vel, FREE_SURF=fs,
+                             stype=st, FW=8, DTINV=3)
+    p = params_of(w)
+    s = w.shots[0]
+    a = O.forward(p, w.true, s.srcpos, s.recpos)
+    b = O.forward(p, w.true, s.srcpos, s.recpos, which="ref")
+    for x, y, n in zip(a, b, ("vx", "vy", "fields")):
+        np.testing.assert_array_equal(x, y, err_msg=n)
+
+
+@needs_ref
+@pytest.mark.parametrize("fd,L,vel", list(itertools.product([4, 8], [0, 2], [0, 1])))
+def test_gradient_matches_reference(fd, L, vel):
+    w = workloads.small_case(seed=3 + fd + L, NX=48, NY=36, NT=120, FDORDER=fd, L=L, VELOCITY=vel, DTINV=3, FW=8)
+    p = params_of(w)
+    s = w.shots[0]
+    ovx, ovy, _ = O.forward(p, w.true, s.srcpos, s.recpos)
+    ovx[:, 0] = 0
+    ovy[:, 0] = 0
+    g = O.gradient(p, w.start, s.srcpos, s.recpos, ovx, ovy)
+    dvx, _ = O.calc_res(p, g["seis_vx"], ovx)
+    dvy, _ = O.calc_res(p, g["seis_vy"], ovy)
+    r = O.ref_gradient(p, w.start, s.srcpos, s.recpos, dvx, dvy)
+    for k in ("seis_vx", "seis_vy", "grad_vp", "grad_vs", "grad_rho"):
+        np.testing.assert_array_equal(g[k], r[k], err_msg=k)
+
+
+# ------------------------------------------------------------ golden data --
+@pytest.mark.parametrize("name", ["el4_fs_v0", "visc8_fs_v1", "el2_nofs_rot", "visc6_fs_fx_v1"])
+def test_oracle_reproduces_golden(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"))
+    meta = json.load(open(os.path.join(GOLDEN, name + ".json")))
+    cfg = dict(meta["config"])
+    fl = cfg.pop("FL")
+    p = O.make_params(FL=fl, **cfg)
+    true = {k[5:]: z[k] for k in z.files if k.startswith("true_")}
+    start = {k[6:]: z[k] for k in z.files if k.startswith("start_") and "seis" not in k}
+    vx, vy, fin = O.forward(p, true, z["srcpos"], z["recpos"])
+    np.testing.assert_array_equal(vx, z["seis_vx"])
+    np.testing.assert_array_equal(vy, z["seis_vy"])
+    np.testing.assert_array_equal(fin, z["final5"])
+    g = O.gradient(p, start, z["srcpos"], z["recpos"], z["obs_vx"], z["obs_vy"])
+    for k in ("grad_vp", "grad_vs", "grad_rho"):
+        np.testing.assert_array_equal(g[k], z[k], err_msg=k)
+
+
+# --------------------------------------------------------------- residual --
+@pytest.mark.parametrize("vel", [0, 1])
+def test_calc_res_against_numpy(vel):
+    rng = np.random.default_rng(7)
+    p = O.make_params(NX=10, NY=10, FDORDER=4, FW=2, NT=300, DH=1.0, DT=3e-4, VELOCITY=vel)
+    syn = rng.standard_normal((9, 300)).astype(np.float32)
+    obs = rng.standard_normal((9, 300)).astype(np.float32)
+    diff, l2 = O.calc_res(p, syn, obs)
+    if vel == 0:
+        si = np.cumsum(syn, axis=1, dtype=np.float32) * np.float32(p.DT)
+        oi = np.cumsum(obs, axis=1, dtype=np.float32) * np.float32(p.DT)
+    else:
+        si, oi = syn, obs
+    expect = (si - oi)[:, ::-1]
+    np.testing.assert_array_equal(diff, expect)
+    l2_64 = float(np.sum(expect.astype(np.float64) ** 2))
+    # FP32 running sum: within the sequential-summation bound n*eps*sum
+    assert abs(l2 - l2_64) <= expect.size * 6e-8 * l2_64
+
+
+def test_calc_res_kill_mask():
+    rng = np.random.default_rng(8)
+    p = O.make_params(NX=10, NY=10, FDORDER=4, FW=2, NT=50, DH=1.0, DT=1e-3, TRKILL=1)
+    syn = rng.standard_normal((5, 50)).astype(np.float32)
+    obs = rng.standard_normal((5, 50)).astype(np.float32)
+    kill = np.array([0, 1, 0, 1, 0], np.int32)
+    diff, _ = O.calc_res(p, syn, obs, kill)
+    assert np.all(diff[1] == 0) and np.all(diff[3] == 0) and np.any(diff[0] != 0)
+
+
+# --------------------------------------------------------------- gradient --
+@pytest.mark.parametrize("vel,sign", [(1, 1.0), (0, -1.0)])
+def test_gradient_sign_and_scale_by_finite_differences(vel, sign):
+    """Directional derivatives of the FP64 misfit against <g, dm> for several
+    smooth perturbations; a sign error, a swapped parameter or a transposed
+    index drops the correlation.
+
+    VELOCITY=1 (velocity misfit): the reference's waveconv is the gradient.
+    VELOCITY=0 (displacement misfit): the reference injects the displacement
+    residual and integrates the adjoint velocity forward in adjoint time
+    (src/update_v_PML.c:140-143 with src/calc_res.c:138-156), which reverses
+    the sign of the time integral: its waveconv is anti-parallel to the
+    misfit gradient.  Parity keeps that; this test pins it."""
+    w = workloads.small_case(seed=21, NX=60, NY=44, NT=260, FDORDER=4, L=0, VELOCITY=vel, DTINV=1, FW=10)
+    p = params_of(w)
+    s = w.shots[0]
+    ovx, ovy, _ = O.forward(p, w.true, s.srcpos, s.recpos, fp64=True)
+    ovx = ovx.astype(np.float32)
+    ovy = ovy.astype(np.float32)
+    g = O.gradient(p, w.start, s.srcpos, s.recpos, ovx, ovy, fp64=True)
+
+    def misfit(m):
+        vx, vy, _ = O.forward(p, m, s.srcpos, s.recpos, fp64=True)
+        r = 0.0
+        for syn, obs in ((vx, ovx), (vy, ovy)):
+            if vel == 0:
+                si = np.cumsum(syn, axis=1) * p.DT
+                oi = np.cumsum(obs.astype(np.float64), axis=1) * p.DT
+            else:
+                si, oi = syn, obs.astype(np.float64)
+            r += float(np.sum((si - oi) ** 2))
+        return r
+
+    rng = np.random.default_rng(0)
+    y, x = np.mgrid[1:p.NY + 1, 1:p.NX + 1]
+    fd, pred = [], []
+    for k in range(6):
+        cx, cy = rng.uniform(15, p.NX - 15), rng.uniform(8, p.NY - 12)
+        bump = np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * 4.0 ** 2))
+        par = ["vp", "vs", "rho"][k % 3]
+        scale = {"vp": 30.0, "vs": 20.0, "rho": 20.0}[par]
+        dm = {q: np.zeros_like(w.start[q], dtype=np.float64) for q in ("vp", "vs", "rho")}
+        dm[par] = scale * bump
+        mp = {q: (w.start[q] + dm[q]).astype(np.float32) for q in ("vp", "vs", "rho")}
+        mm = {q: (w.start[q] - dm[q]).astype(np.float32) for q in ("vp", "vs", "rho")}
+        fd.append((misfit(mp) - misfit(mm)) / 2)
+        pred.append(sum(float(np.sum(g["grad_" + q] * dm[q])) for q in ("vp", "vs", "rho")))
+    fd, pred = np.array(fd), np.array(pred)
+    corr = float(np.dot(fd, pred) / (np.linalg.norm(fd) * np.linalg.norm(pred)))
+    assert sign * corr > 0.9, (corr, fd, pred)
+    big = np.abs(fd) > 0.2 * np.abs(fd).max()
+    assert np.all(np.sign(fd[big]) == sign * np.sign(pred[big]))
+
+
+def test_fp64_build_close_to_fp32():
+    w = workloads.small_case(seed=4, NX=50, NY=40, NT=150, FDORDER=8, L=2, VELOCITY=1)
+    p = params_of(w)
+    s = w.shots[0]
+    a = O.forward(p, w.true, s.srcpos, s.recpos)
+    b = O.forward(p, w.true, s.srcpos, s.recpos, fp64=True)
+    rel = np.linalg.norm(a[0] - b[0]) / np.linalg.norm(b[0])
+    assert 0 < rel < 1e-4
