@@ -70,6 +70,10 @@ def main():
     ap.add_argument("--workload", default="layered_visc")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only time the velocity and stress kernels on the live model (for counter runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="also run one device-to-device copy of this many MB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (every box here has one GPU; "
@@ -85,6 +89,22 @@ def main():
         f"DTINV={cfg.DTINV}, {len(shots)} shots")
 
     eng = ifos_amd.Engine(cfg)
+    if args.copy_probe:
+        nb = args.copy_probe * (1 << 20)
+        a, c = ifos_amd.DeviceBuffer(nb), ifos_amd.DeviceBuffer(nb)
+        import ctypes as C
+        ifos_amd._check(ifos_amd.lib().ifos_memcpy_d2d(C.c_void_p(c.ptr), C.c_void_p(a.ptr), nb))
+        log(f"[bench] copy probe: {args.copy_probe} MB device-to-device")
+        a.free()
+        c.free()
+    if args.kernels_only:
+        eng.set_model(**w.start)
+        for kern in (0, 1):
+            ms = eng.time_kernel(kern, reps=20)
+            gbs = eng.kernel_bytes_per_cell(kern) * cfg.NX * cfg.NY / (ms * 1e-3) / 1e9
+            log(f"[bench] kernel {kern}: {ms:.5f} ms/launch, {gbs:.1f} GB/s algorithmic")
+        eng.close()
+        return 0
     # observed data: forward runs on the true model (untimed), kept in HBM
     eng.set_model(**w.true)
     obs = []

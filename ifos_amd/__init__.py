@@ -70,6 +70,7 @@ def lib():
         L.ifos_dev_free.argtypes = [C.c_void_p]
         L.ifos_memcpy_h2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.ifos_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.ifos_memcpy_d2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.ifos_engine_create.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]
         L.ifos_engine_destroy.argtypes = [C.c_void_p]
         L.ifos_engine_destroy.restype = None

@@ -751,6 +751,12 @@ extern "C" int ifos_memcpy_d2h(void *dst, const void *src, size_t bytes)
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return 0;
 }
+extern "C" int ifos_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}
 
 // ---------------------------------------------------------------------------
 // kernel timing for the roofline figures

@@ -121,6 +121,7 @@ void *ifos_dev_alloc(size_t bytes);
 int  ifos_dev_free(void *p);
 int  ifos_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int  ifos_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int  ifos_memcpy_d2d(void *dst, const void *src, size_t bytes);
 /* Time one kernel of the engine on its own stream with HIP events:
  * kernel 0 = velocity update, 1 = stress update, 2 = post (surface/source/
  * receivers), 3 = correlation.  Runs reps launches on the live state, then
