@@ -22,6 +22,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def measured_traffic(kernel_prefix, workload):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes, each scaled by what the
+    same counter reads for a device-to-device copy of known size)."""
+    try:
+        import json as _j
+        d = _j.load(open(TRAFFIC_FILE))
+    except (OSError, ValueError):
+        return None
+    for k in d.get("kernels", []):
+        if k["kernel"].startswith(kernel_prefix) and k["workload"] == workload:
+            return k["hbm_bytes_per_launch"]
+    return None
 
 
 def log(*a):
@@ -99,6 +115,7 @@ def main():
         c.free()
     if args.kernels_only:
         eng.set_model(**w.start)
+        eng.set_shot(w.shots[0].srcpos, w.shots[0].recpos)
         for kern in (0, 1):
             ms = eng.time_kernel(kern, reps=20)
             gbs = eng.kernel_bytes_per_cell(kern) * cfg.NX * cfg.NY / (ms * 1e-3) / 1e9
@@ -144,9 +161,10 @@ def main():
     bytes_launch = bpc * cfg.NX * cfg.NY
     achieved = bytes_launch / (ms_k * 1e-3) / 1e9
     ms_v = eng.time_kernel(0, reps=50)
+    kname = "k_str_step"
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel="k_update_s_visc" if cfg.L else "k_update_s_elastic",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, w.name),
+                kernel=kname,
                 bytes_per_cell=bpc, cells=cfg.NX * cfg.NY, avg_ms=round(ms_k, 5),
                 velocity_kernel_avg_ms=round(ms_v, 5),
                 velocity_kernel_GBs=round(eng.kernel_bytes_per_cell(0) * cfg.NX * cfg.NY / (ms_v * 1e-3) / 1e9, 1))

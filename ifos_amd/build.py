@@ -31,7 +31,7 @@ HIP_FLAGS = [
 C_FLAGS = ["-O2", "-g", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-Wall",
            "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
-HIP_SOURCES = ["ifos_kernels.hip", "ifos_engine.hip", "ifos_dropin.hip"]
+HIP_SOURCES = ["ifos_kernels.hip", "ifos_step.hip", "ifos_engine.hip", "ifos_dropin.hip"]
 C_SOURCES = ["host/ifos_setup.c"]
 
 

@@ -285,6 +285,10 @@ int ifos_setup_materials(const ifos_config *c, const float *vp, const float *vs,
         m->d = (float *)malloc(n * L * sizeof(float));
         m->e = (float *)malloc(n * L * sizeof(float));
         m->dip = (float *)malloc(n * L * sizeof(float));
+        m->mu_r = (float *)malloc(n * sizeof(float));
+        m->pi_r = (float *)malloc(n * sizeof(float));
+        m->muipjp_r = (float *)malloc(n * sizeof(float));
+        m->tausipjp = (float *)malloc(n * sizeof(float));
         for (int j = 1; j <= NY; j++)
             for (int i = 1; i <= NX; i++) {
                 size_t k = (size_t)(j - 1) * NX + (i - 1);
@@ -295,6 +299,10 @@ int ifos_setup_materials(const ifos_config *c, const float *vp, const float *vs,
                 float mu = (u * u * rh) / (1.0 + sumu * ts);
                 float pi = (pv * pv * rh) / (1.0 + sumpi * tp);
                 float muipjp = m->uipjp[k] / (1.0 + sumu * tsipjp);
+                m->mu_r[k] = mu;
+                m->pi_r[k] = pi;
+                m->muipjp_r[k] = muipjp;
+                m->tausipjp[k] = tsipjp;
                 m->fipjp[k] = muipjp * c->DT * (1.0 + L * tsipjp);
                 m->f[k] = mu * c->DT * (1.0 + L * ts);
                 m->g[k] = pi * c->DT * (1.0 + L * tp);
@@ -313,5 +321,6 @@ void ifos_free_materials(ifos_materials *m)
 {
     free(m->rip); free(m->rjp); free(m->uipjp); free(m->muss); free(m->lamss);
     free(m->f); free(m->g); free(m->fipjp); free(m->d); free(m->e); free(m->dip);
+    free(m->mu_r); free(m->pi_r); free(m->muipjp_r); free(m->tausipjp);
     memset(m, 0, sizeof(*m));
 }

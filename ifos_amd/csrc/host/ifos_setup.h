@@ -19,6 +19,8 @@ typedef struct ifos_materials {
     float *fipjp;               /* viscoelastic                            */
     float *d, *e, *dip;         /* viscoelastic, [L][NY][NX]               */
     float *muss, *lamss;        /* moduli of the correlation / chain rule  */
+    float *mu_r, *pi_r, *muipjp_r, *tausipjp;  /* viscoelastic: relaxed moduli
+                                   and averaged tau of prepare_update_s      */
     float eta[IFOS_MAXL + 1], bip[IFOS_MAXL + 1], bjm[IFOS_MAXL + 1];
     float cip[IFOS_MAXL + 1], cjm[IFOS_MAXL + 1];
     float sumu, sumpi;

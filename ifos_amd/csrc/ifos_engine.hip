@@ -19,6 +19,7 @@
 #include "host/ifos_setup.h"
 #include "ifos2d_hip.h"
 #include "ifos_internal.h"
+#include "ifos_step.h"
 
 namespace ifos {
 void set_step_control(const int *nt_dev, int DTINV, int NT);
@@ -81,6 +82,7 @@ struct ifos_engine {
     float *vx, *vy, *sxx, *syy, *sxy, *ux, *uy, *uxy, *vxp1, *vyp1;
     float *rip, *rjp, *uipjp, *f, *g, *fipjp, *rho, *vs, *vp, *taus, *taup;
     float *r = nullptr, *p = nullptr, *q = nullptr, *d = nullptr, *e = nullptr, *dip = nullptr;
+    float *mu_r, *pi_r, *muipjp_r, *tausipjp;  // relaxed moduli for the in-kernel GSLS coefficients
     int nwave = 0;  // grids zeroed at the start of a run (wavefields + memory variables)
 
     // interior [NY][NX]
@@ -109,6 +111,12 @@ struct ifos_engine {
     float *seis = nullptr;  // vx, vy, diff_vx, diff_vy, obs_vx, obs_vy: 6*[cap_tr][NT]
     double *trace_sums = nullptr;  // l2, en [2][cap_tr]
 
+    // per-tile point lists (sources, receivers) for the fused step kernels
+    int nbx = 0, nby = 0;
+    int *tiles = nullptr;
+    size_t tiles_cap = 0;
+    int off_src_start = 0, off_src_idx = 0, off_rec_start = 0, off_rec_idx = 0;
+
     // snapshots
     float *snap = nullptr;
     size_t snap_bytes = 0;
@@ -118,6 +126,7 @@ struct ifos_engine {
     int gsteps[2] = {0, 0};
     int gkey[2][5] = {{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}};
     bool use_graphs = true;
+    int swizzle = 1;
 };
 
 #define ENG_TRY(expr)       \
@@ -154,6 +163,8 @@ extern "C" int ifos_engine_create(const ifos_config *cfg, int device, ifos_engin
     E->device = device;
     const char *ng = getenv("IFOS_NO_GRAPH");
     E->use_graphs = !(ng && ng[0] == '1');
+    const char *ns = getenv("IFOS_NO_SWIZZLE");
+    E->swizzle = (ns && ns[0] == '1') ? 0 : 1;
     auto fail = [&](int rc) {
         ifos_engine_destroy(E);
         return rc;
@@ -176,11 +187,13 @@ extern "C" int ifos_engine_create(const ifos_config *cfg, int device, ifos_engin
     E->off = (long)E->H * E->pitch + (A - 1);
     E->plane = (long)c.NX * c.NY;
     E->nsnap = c.NT / c.DTINV;
+    E->nbx = (c.NX + kTileX - 1) / kTileX;
+    E->nby = (c.NY + kTileY - 1) / kTileY;
     ifos_holberg(c.FDORDER, c.MAXRELERROR, E->hc);
 
-    // padded grids: 10 wavefields + 11 materials + 6*L memory planes
+    // padded grids: 10 wavefields + 3L memory planes + 15 materials + 3L coefficient planes
     const int L = c.L;
-    const int ngrid = 10 + 11 + 6 * L;
+    const int ngrid = 10 + 15 + 6 * L;
     if (hipMalloc(&E->slab, sizeof(float) * E->field * ngrid) != hipSuccess) {
         ifos_set_error("device allocation of %ld grids failed", (long)ngrid);
         return fail(-1);
@@ -202,6 +215,7 @@ extern "C" int ifos_engine_create(const ifos_config *cfg, int device, ifos_engin
     E->nwave = 10 + 3 * L;
     E->rip = next(); E->rjp = next(); E->uipjp = next(); E->f = next(); E->g = next(); E->fipjp = next();
     E->rho = next(); E->vs = next(); E->vp = next(); E->taus = next(); E->taup = next();
+    E->mu_r = next(); E->pi_r = next(); E->muipjp_r = next(); E->tausipjp = next();
     if (L) {
         E->d = next(); for (int l = 1; l < L; l++) next();
         E->e = next(); for (int l = 1; l < L; l++) next();
@@ -257,7 +271,7 @@ extern "C" void ifos_engine_destroy(ifos_engine *E)
     drop_graphs(E);
     hipFree(E->slab); hipFree(E->interior); hipFree(E->pml); hipFree(E->psi); hipFree(E->ctl);
     hipFree(E->src_i); hipFree(E->src_d); hipFree(E->sig); hipFree(E->rec_i); hipFree(E->seis);
-    hipFree(E->trace_sums); hipFree(E->snap);
+    hipFree(E->trace_sums); hipFree(E->snap); hipFree(E->tiles);
     if (E->stream) hipStreamDestroy(E->stream);
     ifos_free_materials(&E->mat);
     delete E;
@@ -294,6 +308,10 @@ extern "C" int ifos_engine_set_model(ifos_engine *E, const float *vp, const floa
         ENG_TRY(upload_grid(E, E->fipjp, m.fipjp));
         ENG_TRY(upload_grid(E, E->taus, taus));
         ENG_TRY(upload_grid(E, E->taup, taup));
+        ENG_TRY(upload_grid(E, E->mu_r, m.mu_r));
+        ENG_TRY(upload_grid(E, E->pi_r, m.pi_r));
+        ENG_TRY(upload_grid(E, E->muipjp_r, m.muipjp_r));
+        ENG_TRY(upload_grid(E, E->tausipjp, m.tausipjp));
         for (int l = 0; l < c.L; l++) {
             ENG_TRY(upload_grid(E, E->d + l * E->field, m.d + (size_t)l * E->plane));
             ENG_TRY(upload_grid(E, E->e + l * E->field, m.e + (size_t)l * E->plane));
@@ -384,6 +402,40 @@ extern "C" int ifos_engine_set_shot(ifos_engine *E, int nsrc, const float *srcpo
             if (seen[k] == seen[k - 1]) E->serial_adj = true;
         HIP_TRY(hipMemcpy(E->rec_i, ri.data(), sizeof(int) * ri.size(), hipMemcpyHostToDevice));
     }
+    // bucket sources and receivers by 64x8 tile, keeping their order
+    const int nblk = E->nbx * E->nby;
+    auto bucket = [&](int n, auto x_of, auto y_of, std::vector<int> &start, std::vector<int> &idx) {
+        start.assign(nblk + 1, 0);
+        idx.assign(n > 0 ? n : 1, 0);
+        std::vector<int> tile(n);
+        for (int q = 0; q < n; q++) {
+            tile[q] = ((y_of(q) - 1) / kTileY) * E->nbx + (x_of(q) - 1) / kTileX;
+            start[tile[q] + 1]++;
+        }
+        for (int b = 0; b < nblk; b++) start[b + 1] += start[b];
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int q = 0; q < n; q++) idx[fill[tile[q]]++] = q;
+    };
+    std::vector<int> s_start, s_idx, r_start, r_idx;
+    bucket(nsrc, [&](int q) { return (int)srcpos[(size_t)q * IFOS_SRC_STRIDE]; },
+           [&](int q) { return (int)srcpos[(size_t)q * IFOS_SRC_STRIDE + 1]; }, s_start, s_idx);
+    bucket(ntr, [&](int q) { return recpos[2 * q]; }, [&](int q) { return recpos[2 * q + 1]; }, r_start, r_idx);
+    std::vector<int> blob;
+    E->off_src_start = 0;
+    blob.insert(blob.end(), s_start.begin(), s_start.end());
+    E->off_src_idx = (int)blob.size();
+    blob.insert(blob.end(), s_idx.begin(), s_idx.end());
+    E->off_rec_start = (int)blob.size();
+    blob.insert(blob.end(), r_start.begin(), r_start.end());
+    E->off_rec_idx = (int)blob.size();
+    blob.insert(blob.end(), r_idx.begin(), r_idx.end());
+    if (blob.size() > E->tiles_cap) {
+        hipFree(E->tiles);
+        E->tiles_cap = blob.size() * 2;
+        HIP_TRY(hipMalloc(&E->tiles, sizeof(int) * E->tiles_cap));
+        drop_graphs(E);
+    }
+    HIP_TRY(hipMemcpy(E->tiles, blob.data(), sizeof(int) * blob.size(), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -409,131 +461,84 @@ static StepGeom geom(const ifos_engine *E)
 
 static float *seis_buf(const ifos_engine *E, int k) { return E->seis + (size_t)k * E->cap_tr * E->c.NT; }
 
-// one time step: sw = 0 forward, 1 adjoint; extra = snapshot (fwd) / correlation (adj)
-static void enqueue_step(ifos_engine *E, int sw, bool extra, bool snapshots)
+// argument block of the fused step kernels for this engine
+static StepArgs step_args(ifos_engine *E, int sw, bool snapshots)
 {
     const ifos_config &c = E->c;
-    const StepGeom g = geom(E);
-    int *ntd = E->ctl;
-    set_step_control(ntd, c.DTINV, c.NT);
-    hipStream_t s = E->stream;
-
-    VArgs va;
-    va.g = g; va.sw = sw; va.VELOCITY = c.VELOCITY;
-    va.sxx = E->sxx; va.syy = E->syy; va.sxy = E->sxy; va.rip = E->rip; va.rjp = E->rjp;
-    va.vx = E->vx; va.vy = E->vy;
-    const bool need_p1 = sw == 1 || snapshots;
-    va.vxp1 = need_p1 ? E->vxp1 : nullptr;
-    va.vyp1 = need_p1 ? E->vyp1 : nullptr;
-    va.psi_sxx_x = E->psi_sxx_x; va.psi_sxy_x = E->psi_sxy_x;
-    va.psi_syy_y = E->psi_syy_y; va.psi_sxy_y = E->psi_sxy_y;
-    launch_update_v(va, s);
-
-    PointArgs pa;
-    if (sw == 0) {
-        pa.n = E->nsrc; pa.ix = E->src_i; pa.iy = E->src_i + E->cap_src; pa.type = E->src_i + 2 * E->cap_src;
-        pa.sin_azi = E->src_d; pa.cos_azi = E->src_d + E->cap_src;
-        pa.sig = E->sig; pa.sig1 = E->sig; pa.ld = c.NT;
-        // point forces only; explosive sources are applied in the post kernel
-    } else {
-        pa.n = E->ntr; pa.ix = E->rec_i; pa.iy = E->rec_i + E->cap_tr; pa.type = nullptr;
-        pa.sin_azi = pa.cos_azi = nullptr;
-        pa.sig = seis_buf(E, 2); pa.sig1 = seis_buf(E, 3); pa.ld = c.NT;
+    StepArgs a;
+    memset(&a, 0, sizeof a);
+    a.g = geom(E);
+    a.L = c.L; a.VELOCITY = c.VELOCITY; a.ADJOINT_TYPE = c.ADJOINT_TYPE; a.NT = c.NT; a.DTINV = c.DTINV;
+    a.sw = sw;
+    a.explosive = sw == 0 ? E->explosive : 0;
+    a.plane = E->plane;
+    a.vx = E->vx; a.vy = E->vy; a.sxx = E->sxx; a.syy = E->syy; a.sxy = E->sxy;
+    a.vxp1 = E->vxp1; a.vyp1 = E->vyp1;
+    a.rip = E->rip; a.rjp = E->rjp;
+    a.f = E->f; a.gl = E->g; a.uipjp = E->uipjp;
+    a.mu = E->mu_r; a.pi = E->pi_r; a.muipjp = E->muipjp_r; a.taus = E->taus; a.taup = E->taup;
+    a.tausipjp = E->tausipjp;
+    a.r = E->r; a.p = E->p; a.q = E->q; a.ls = E->field;
+    for (int l = 0; l <= kMaxL; l++) {
+        const bool on = l >= 1 && l <= c.L;
+        a.eta[l] = on ? E->mat.eta[l] : 0.0f;
+        a.bip[l] = on ? E->mat.bip[l] : 0.0f;
+        a.bjm[l] = on ? E->mat.bjm[l] : 0.0f;
+        a.cip[l] = on ? E->mat.cip[l] : 0.0f;
+        a.cjm[l] = on ? E->mat.cjm[l] : 0.0f;
     }
-    if (sw == 1 && E->serial_adj)
-        launch_inject_serial(pa, g, E->vx, E->vy, E->rip, E->rjp, sw, c.ADJOINT_TYPE, nullptr, 0, ntd, s);
-    else
-        launch_inject(pa, g, E->vx, E->vy, E->rip, E->rjp, sw, c.ADJOINT_TYPE, nullptr, 0, ntd, s);
-
-    const bool rates = sw == 0 && snapshots && c.VELOCITY == 1;
-    if (c.L == 0) {
-        SElArgs sa;
-        sa.g = g; sa.vx = E->vx; sa.vy = E->vy; sa.sxx = E->sxx; sa.syy = E->syy; sa.sxy = E->sxy;
-        sa.ux = rates ? E->ux : nullptr; sa.uy = rates ? E->uy : nullptr; sa.uxy = rates ? E->uxy : nullptr;
-        sa.f = E->f; sa.gl = E->g; sa.uipjp = E->uipjp; sa.rho = sa.vs = sa.vp = nullptr;
-        sa.psi_vxx = E->psi_vxx; sa.psi_vyx = E->psi_vyx; sa.psi_vyy = E->psi_vyy; sa.psi_vxy = E->psi_vxy;
-        launch_update_s_elastic(sa, s);
-    } else {
-        SViscArgs sa;
-        sa.g = g; sa.L = c.L; sa.vx = E->vx; sa.vy = E->vy; sa.sxx = E->sxx; sa.syy = E->syy; sa.sxy = E->sxy;
-        sa.ux = rates ? E->ux : nullptr; sa.uy = rates ? E->uy : nullptr; sa.uxy = rates ? E->uxy : nullptr;
-        sa.fipjp = E->fipjp; sa.f = E->f; sa.gv = E->g;
-        sa.r = E->r; sa.p = E->p; sa.q = E->q; sa.d = E->d; sa.e = E->e; sa.dip = E->dip;
-        sa.cs = 1; sa.ls = (int)E->field;
-        for (int l = 0; l <= kMaxL; l++) {
-            sa.bip[l] = l <= c.L ? E->mat.bip[l] : 0.0f;
-            sa.bjm[l] = l <= c.L ? E->mat.bjm[l] : 0.0f;
-            sa.cip[l] = l <= c.L ? E->mat.cip[l] : 0.0f;
-            sa.cjm[l] = l <= c.L ? E->mat.cjm[l] : 0.0f;
-        }
-        sa.psi_vxx = E->psi_vxx; sa.psi_vyx = E->psi_vyx; sa.psi_vyy = E->psi_vyy; sa.psi_vxy = E->psi_vxy;
-        launch_update_s_visc(sa, s);
-    }
-
-    PostArgs po;
-    po.g = g; po.L = c.L; po.sw = sw; po.explosive = (sw == 0) ? E->explosive : 0; po.NT = c.NT;
-    po.src.n = E->nsrc; po.src.ix = E->src_i; po.src.iy = E->src_i + E->cap_src; po.src.type = nullptr;
-    po.src.sin_azi = po.src.cos_azi = nullptr; po.src.sig = E->sig; po.src.sig1 = E->sig; po.src.ld = c.NT;
-    po.ntr = sw == 0 ? E->ntr : 0;
-    po.rx = E->rec_i; po.ry = E->rec_i + E->cap_tr;
-    po.seis_vx = seis_buf(E, 0); po.seis_vy = seis_buf(E, 1);
-    po.vx = E->vx; po.vy = E->vy; po.sxx = E->sxx; po.syy = E->syy; po.sxy = E->sxy;
-    // stress rates are only consumed by VELOCITY=1 snapshots; their mirror
-    // rows above the surface are never read
-    po.ux = rates ? E->ux : nullptr; po.uy = rates ? E->uy : nullptr; po.uxy = nullptr;
-    po.f = E->f; po.gl = E->g; po.rho = E->rho; po.vs = E->vs; po.vp = E->vp; po.taus = E->taus; po.taup = E->taup;
-    po.p = E->p; po.q = E->q; po.cs = 1; po.ls = (int)E->field;
-    po.sumu = E->mat.sumu; po.sumpi = E->mat.sumpi;
-    for (int l = 0; l <= kMaxL; l++) po.eta[l] = l <= c.L ? E->mat.eta[l] : 0.0f;
-    po.psi_vxxs = E->psi_vxxs;
-    po.nt_dev = ntd; po.nt = 0; po.nan_flag = E->ctl + 1;
-    if (c.L) { po.f = nullptr; po.gl = nullptr; }
-    launch_post(po, s);
-
-    if (extra && sw == 0 && snapshots) {
-        SnapArgs na;
-        na.NX = c.NX; na.NY = c.NY; na.pitch = E->pitch; na.DTINV = c.DTINV;
-        const float *src[5] = {E->vxp1, E->vyp1, c.VELOCITY ? E->ux : E->sxx, c.VELOCITY ? E->uy : E->syy,
-                               c.VELOCITY ? E->uxy : E->sxy};
-        for (int k = 0; k < 5; k++) {
-            na.a[k] = src[k];
-            na.base[k] = E->snap + (size_t)k * E->nsnap * E->plane;
-        }
-        na.nt_dev = ntd; na.nt = 0;
-        launch_snapshot(na, s);
-    }
-    if (extra && sw == 1) {
-        CorrArgs ca;
-        ca.NX = c.NX; ca.NY = c.NY; ca.pitch = E->pitch; ca.NT = c.NT; ca.DTINV = c.DTINV;
-        ca.vxp1 = E->vxp1; ca.vyp1 = E->vyp1; ca.sxx = E->sxx; ca.syy = E->syy; ca.sxy = E->sxy;
-        for (int k = 0; k < 5; k++) ca.F[k] = E->snap + (size_t)k * E->nsnap * E->plane;
-        ca.nt_dev = ntd; ca.nt = 0;
-        ca.muss = E->muss; ca.lamss = E->lamss; ca.vs = E->vs_i;
-        ca.wc_rho = E->wc_rho; ca.wc = E->wc; ca.wc_u = E->wc_u;
-        launch_correlate(ca, s);
-    }
+    a.psi_sxx_x = E->psi_sxx_x; a.psi_sxy_x = E->psi_sxy_x; a.psi_syy_y = E->psi_syy_y; a.psi_sxy_y = E->psi_sxy_y;
+    a.psi_vxx = E->psi_vxx; a.psi_vyx = E->psi_vyx; a.psi_vyy = E->psi_vyy; a.psi_vxy = E->psi_vxy;
+    a.psi_vxxs = E->psi_vxxs;
+    TileList src;
+    src.start = E->tiles + E->off_src_start; src.idx = E->tiles + E->off_src_idx;
+    src.ix = E->src_i; src.iy = E->src_i + E->cap_src; src.type = E->src_i + 2 * E->cap_src;
+    src.sin_azi = E->src_d; src.cos_azi = E->src_d + E->cap_src;
+    src.sig = E->sig; src.sig1 = E->sig;
+    TileList rec;
+    rec.start = E->tiles + E->off_rec_start; rec.idx = E->tiles + E->off_rec_idx;
+    rec.ix = E->rec_i; rec.iy = E->rec_i + E->cap_tr; rec.type = nullptr;
+    rec.sin_azi = rec.cos_azi = nullptr;
+    rec.sig = seis_buf(E, 2); rec.sig1 = seis_buf(E, 3);  // adjoint sources
+    a.inj = sw == 0 ? src : rec;
+    a.exp = src;
+    a.rec = rec;
+    if (snapshots || sw == 1)
+        for (int k = 0; k < 5; k++) a.snap[k] = E->snap + (size_t)k * E->nsnap * E->plane;
+    a.muss = E->muss; a.lamss = E->lamss; a.vs_i = E->vs_i;
+    a.wc_rho = E->wc_rho; a.wc = E->wc; a.wc_u = E->wc_u;
+    a.seis_vx = seis_buf(E, 0); a.seis_vy = seis_buf(E, 1);
+    a.base = E->ctl;
+    a.nan_flag = E->ctl + 1;
+    a.nbx = E->nbx;
+    a.nby = E->nby;
+    a.swizzle = E->swizzle;
+    return a;
 }
 
-// Steps within a unit: forward units start at nt = 1 (mod DTINV) with the
-// snapshot at the first step; adjoint units end with the correlation, since
-// (NT - nt) % DTINV == 0 <=> nt % DTINV == 0 when DTINV divides NT.
-static bool extra_at(const ifos_engine *E, int sw, int k) { return sw == 0 ? k % E->c.DTINV == 0 : (k + 1) % E->c.DTINV == 0; }
-
+// NT steps as graph units of `unit` steps: kernels take their position k in
+// the unit and read the number of steps already done from a device counter,
+// which a one-thread kernel advances at the end of each unit.
 static int run_loop(ifos_engine *E, int sw, bool snapshots)
 {
     const ifos_config &c = E->c;
     const int NT = c.NT;
     int unit = c.DTINV;
     while (unit < 32 && NT % (unit * 2) == 0) unit *= 2;
-    const int kind = sw;  // graph slot
-    const int key[5] = {E->nsrc, E->ntr, snapshots ? 1 : 0, E->serial_adj ? 1 : 0, E->explosive};
+    const int kind = sw;
+    const int key[5] = {E->nsrc, E->ntr, snapshots ? 1 : 0, E->explosive, unit};
+    StepArgs a = step_args(E, sw, snapshots);
     if (E->use_graphs) {
         if (!E->gexec[kind] || E->gsteps[kind] != unit || memcmp(E->gkey[kind], key, sizeof key)) {
             if (E->gexec[kind]) hipGraphExecDestroy(E->gexec[kind]);
             E->gexec[kind] = nullptr;
             hipGraph_t graph;
             HIP_TRY(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < unit; k++) enqueue_step(E, sw, extra_at(E, sw, k), snapshots);
+            for (int k = 0; k < unit; k++) {
+                launch_vel_step(a, k, E->stream);
+                launch_str_step(a, k, E->stream);
+            }
+            launch_tick(E->ctl, unit, E->stream);
             HIP_TRY(hipStreamEndCapture(E->stream, &graph));
             hipError_t ie = hipGraphInstantiate(&E->gexec[kind], graph, nullptr, nullptr, 0);
             hipGraphDestroy(graph);
@@ -547,7 +552,11 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
         }
         for (int u = 0; u < NT / unit; u++) HIP_TRY(hipGraphLaunch(E->gexec[kind], E->stream));
     } else {
-        for (int nt = 0; nt < NT; nt++) enqueue_step(E, sw, extra_at(E, sw, nt % c.DTINV), snapshots);
+        a.base = nullptr;  // the step number is passed directly
+        for (int nt = 1; nt <= NT; nt++) {
+            launch_vel_step(a, nt, E->stream);
+            launch_str_step(a, nt, E->stream);
+        }
     }
     HIP_TRY(hipGetLastError());
     return 0;
@@ -592,6 +601,7 @@ extern "C" int ifos_engine_forward(ifos_engine *E, int store_snapshots)
 {
     HIP_TRY(hipSetDevice(E->device));
     if (!E->model_set) return ifos_set_error("set_model must be called before forward"), -2;
+    if (!E->tiles) return ifos_set_error("set_shot must be called before forward"), -2;
     if (store_snapshots) ENG_TRY(ensure_snapshots(E));
     ENG_TRY(zero_state(E));
     ENG_TRY(run_loop(E, 0, store_snapshots != 0));
@@ -675,6 +685,7 @@ extern "C" int ifos_engine_adjoint(ifos_engine *E)
 {
     HIP_TRY(hipSetDevice(E->device));
     if (!E->snap) return ifos_set_error("adjoint needs a forward run with snapshots"), -2;
+    if (!E->tiles || !E->model_set) return ifos_set_error("adjoint needs set_model and set_shot"), -2;
     ENG_TRY(zero_state(E));
     HIP_TRY(hipMemsetAsync(E->wc, 0, sizeof(float) * 3 * E->plane, E->stream));  // wc, wc_u, wc_rho
     ENG_TRY(run_loop(E, 1, false));
@@ -765,60 +776,32 @@ extern "C" double ifos_engine_kernel_bytes_per_cell(ifos_engine *E, int kernel)
 {
     const int L = E->c.L;
     switch (kernel) {
-    case 0: return 4.0 * (5 + 2 + 2);  // read sxx,syy,sxy,rip,rjp,vx,vy; write vx,vy
+    case 0: return 4.0 * (3 + 2 + 2 + 2);  // read sxx,syy,sxy,rip,rjp,vx,vy; write vx,vy
     case 1:
         if (L == 0) return 4.0 * (2 + 3 + 3 + 3);  // read vx,vy,sxx,syy,sxy,f,g,uipjp; write sxx,syy,sxy
-        return 4.0 * (2 + 3 + 3 + 6 * L + 3 + 3 * L);  // + r,p,q rw, d,e,dip, fipjp,f,g
-    case 3: return 4.0 * (5 + 5 + 6 + 3);  // adjoint fields, snapshots, acc rw, moduli
+        // read vx,vy, sxx,syy,sxy, 3L memory variables, 6 moduli/tau; write 3 stresses, 3L memory variables
+        return 4.0 * (2 + 3 + 3 * L + 6 + 3 + 3 * L);
     default: return 0.0;
     }
 }
 
+// kernel 0: velocity step, 1: stress step (forward, no snapshot), on the
+// live state with the step counter bypassed (nt = 2)
 extern "C" int ifos_engine_time_kernel(ifos_engine *E, int kernel, int reps, double *ms_avg)
 {
     HIP_TRY(hipSetDevice(E->device));
-    const ifos_config &c = E->c;
-    const StepGeom g = geom(E);
-    set_step_control(nullptr, c.DTINV, c.NT);
+    if (!E->tiles || !E->model_set) return ifos_set_error("time_kernel needs set_model and set_shot"), -2;
+    StepArgs a = step_args(E, 0, false);
+    a.base = nullptr;
+    a.nan_flag = nullptr;
     hipEvent_t t0, t1;
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
     auto one = [&]() {
-        if (kernel == 0) {
-            VArgs va;
-            va.g = g; va.sw = 0; va.VELOCITY = c.VELOCITY;
-            va.sxx = E->sxx; va.syy = E->syy; va.sxy = E->sxy; va.rip = E->rip; va.rjp = E->rjp;
-            va.vx = E->vx; va.vy = E->vy; va.vxp1 = nullptr; va.vyp1 = nullptr;
-            va.psi_sxx_x = E->psi_sxx_x; va.psi_sxy_x = E->psi_sxy_x;
-            va.psi_syy_y = E->psi_syy_y; va.psi_sxy_y = E->psi_sxy_y;
-            launch_update_v(va, E->stream);
-        } else if (kernel == 1) {
-            if (c.L == 0) {
-                SElArgs sa;
-                sa.g = g; sa.vx = E->vx; sa.vy = E->vy; sa.sxx = E->sxx; sa.syy = E->syy; sa.sxy = E->sxy;
-                sa.ux = sa.uy = sa.uxy = nullptr;
-                sa.f = E->f; sa.gl = E->g; sa.uipjp = E->uipjp; sa.rho = sa.vs = sa.vp = nullptr;
-                sa.psi_vxx = E->psi_vxx; sa.psi_vyx = E->psi_vyx; sa.psi_vyy = E->psi_vyy; sa.psi_vxy = E->psi_vxy;
-                launch_update_s_elastic(sa, E->stream);
-            } else {
-                SViscArgs sa;
-                sa.g = g; sa.L = c.L; sa.vx = E->vx; sa.vy = E->vy; sa.sxx = E->sxx; sa.syy = E->syy; sa.sxy = E->sxy;
-                sa.ux = sa.uy = sa.uxy = nullptr;
-                sa.fipjp = E->fipjp; sa.f = E->f; sa.gv = E->g;
-                sa.r = E->r; sa.p = E->p; sa.q = E->q; sa.d = E->d; sa.e = E->e; sa.dip = E->dip;
-                sa.cs = 1; sa.ls = (int)E->field;
-                for (int l = 0; l <= kMaxL; l++) {
-                    sa.bip[l] = l <= c.L ? E->mat.bip[l] : 0.0f;
-                    sa.bjm[l] = l <= c.L ? E->mat.bjm[l] : 0.0f;
-                    sa.cip[l] = l <= c.L ? E->mat.cip[l] : 0.0f;
-                    sa.cjm[l] = l <= c.L ? E->mat.cjm[l] : 0.0f;
-                }
-                sa.psi_vxx = E->psi_vxx; sa.psi_vyx = E->psi_vyx; sa.psi_vyy = E->psi_vyy; sa.psi_vxy = E->psi_vxy;
-                launch_update_s_visc(sa, E->stream);
-            }
-        }
+        if (kernel == 0) launch_vel_step(a, 2, E->stream);
+        else launch_str_step(a, 2, E->stream);
     };
-    one();  // warm
+    one();
     HIP_TRY(hipEventRecord(t0, E->stream));
     for (int k = 0; k < reps; k++) one();
     HIP_TRY(hipEventRecord(t1, E->stream));
