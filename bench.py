@@ -55,7 +55,7 @@ def build_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(w, steps=40):
+def cpu_baseline(w, steps=300):
     """Reference kernels (oracle/_ref) on the host: `steps` forward time steps
     of the same grid, scaled to shot-gradients/s as 1 / (2 * NT * t_step)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -85,7 +85,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="layered_visc")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the velocity and stress kernels on the live model (for counter runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
