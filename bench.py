@@ -86,6 +86,11 @@ def main():
     ap.add_argument("--workload", default="layered_visc")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--batch", type=int, default=1,
+                    help="shots advanced together per kernel launch (1: the one-shot-at-a-time path)")
+    ap.add_argument("--no-autotune", action="store_true", help="batched runs: keep the default launch shape")
+    ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"],
+                    help="snapshot storage (fp32 keeps the gradient bit-identical to the reference)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the velocity and stress kernels on the live model (for counter runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
@@ -99,12 +104,14 @@ def main():
     import ifos_amd
     w = build_workload(args.workload)
     cfg = w.cfg
-    nshots_needed = args.warmup + args.steps
-    shots = [w.shots[k % len(w.shots)] for k in range(nshots_needed)]
+    S = max(1, args.batch)
+    nbatches = args.warmup + args.steps
+    shots = [w.shots[k % len(w.shots)] for k in range(nbatches * S)]
+    batches = [shots[b * S:(b + 1) * S] for b in range(nbatches)]
     log(f"[bench] workload {w.name}: {cfg.NX}x{cfg.NY} L={cfg.L} FDORDER={cfg.FDORDER} NT={cfg.NT} "
-        f"DTINV={cfg.DTINV}, {len(shots)} shots")
+        f"DTINV={cfg.DTINV}, {len(shots)} shots in batches of {S}, {args.snap} snapshots")
 
-    eng = ifos_amd.Engine(cfg)
+    eng = ifos_amd.Engine(cfg, slots=S)
     if args.copy_probe:
         nb = args.copy_probe * (1 << 20)
         a, c = ifos_amd.DeviceBuffer(nb), ifos_amd.DeviceBuffer(nb)
@@ -115,31 +122,39 @@ def main():
         c.free()
     if args.kernels_only:
         eng.set_model(**w.start)
-        eng.set_shot(w.shots[0].srcpos, w.shots[0].recpos)
+        eng.set_shots([(s.srcpos, s.recpos) for s in batches[0]])
         for kern in (0, 1):
             ms = eng.time_kernel(kern, reps=20)
-            gbs = eng.kernel_bytes_per_cell(kern) * cfg.NX * cfg.NY / (ms * 1e-3) / 1e9
+            gbs = eng.kernel_bytes_per_cell(kern) * cfg.NX * cfg.NY * S / (ms * 1e-3) / 1e9
             log(f"[bench] kernel {kern}: {ms:.5f} ms/launch, {gbs:.1f} GB/s algorithmic")
         eng.close()
         return 0
     # observed data: forward runs on the true model (untimed), kept in HBM
     eng.set_model(**w.true)
     obs = []
-    for k, s in enumerate(shots):
-        eng.set_shot(s.srcpos, s.recpos)
+    for k, batch in enumerate(batches):
+        eng.set_shots([(s.srcpos, s.recpos) for s in batch])
         eng.forward(snapshots=False)
         vx, vy = eng.seismograms()
         vx[:, 0] = 0.0
         vy[:, 0] = 0.0
         obs.append((ifos_amd.DeviceBuffer.from_array(vx), ifos_amd.DeviceBuffer.from_array(vy)))
-        log(f"[bench] observed data for shot {k + 1}/{len(shots)}")
+        log(f"[bench] observed data for batch {k + 1}/{len(batches)}")
     eng.set_model(**w.start)
     eng.reset_gradient()
+    eng.set_snapshot_precision(args.snap)
+    shape = None
+    if S > 1 and not args.no_autotune:
+        eng.set_shots([(s.srcpos, s.recpos) for s in batches[0]])
+        shape = eng.autotune(reps=5)
+        log(f"[bench] autotune: {shape[0]} shots per workgroup, grid {shape[1] or 'one per item'}, "
+            f"{shape[2]:.4f} ms per step")
 
     def step(k):
-        s = shots[k]
-        eng.set_shot(s.srcpos, s.recpos)
-        return eng.shot_gradient(obs[k][0], obs[k][1], kill=s.kill)
+        batch = batches[k]
+        eng.set_shots([(s.srcpos, s.recpos) for s in batch])
+        kill = np.concatenate([s.kill if s.kill is not None else np.zeros(len(s.recpos), np.int32) for s in batch])
+        return eng.shot_gradient(obs[k][0], obs[k][1], kill=kill if cfg.TRKILL else None)
 
     for k in range(args.warmup):
         step(k)
@@ -150,7 +165,7 @@ def main():
         log(f"[bench] step {k - args.warmup + 1}/{args.steps} done")
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
-    rate = args.steps / elapsed
+    rate = args.steps * S / elapsed
     cell_updates = 2.0 * cfg.NX * cfg.NY * cfg.NT  # forward + adjoint per shot-gradient
     gcells = cell_updates * rate / 1e9
 
@@ -158,16 +173,16 @@ def main():
     kern = 1
     ms_k = eng.time_kernel(kern, reps=50)
     bpc = eng.kernel_bytes_per_cell(kern)
-    bytes_launch = bpc * cfg.NX * cfg.NY
+    bytes_launch = bpc * cfg.NX * cfg.NY * S
     achieved = bytes_launch / (ms_k * 1e-3) / 1e9
     ms_v = eng.time_kernel(0, reps=50)
     kname = "k_str_step"
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, w.name),
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, w.name) if S == 1 else None,
                 kernel=kname,
                 bytes_per_cell=bpc, cells=cfg.NX * cfg.NY, avg_ms=round(ms_k, 5),
                 velocity_kernel_avg_ms=round(ms_v, 5),
-                velocity_kernel_GBs=round(eng.kernel_bytes_per_cell(0) * cfg.NX * cfg.NY / (ms_v * 1e-3) / 1e9, 1))
+                velocity_kernel_GBs=round(eng.kernel_bytes_per_cell(0) * cfg.NX * cfg.NY * S / (ms_v * 1e-3) / 1e9, 1))
     eng.close()
 
     cpu = None
@@ -191,10 +206,13 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded layered model with faults; observed traces from the true model)",
         "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "dh": cfg.DH, "FDORDER": cfg.FDORDER, "L": cfg.L,
-                   "NT": cfg.NT, "DTINV": cfg.DTINV, "shots_per_step": 1, "receivers": int(len(w.shots[0].recpos)),
-                   "parallelism": "one GPU, one shot at a time"},
+                   "NT": cfg.NT, "DTINV": cfg.DTINV, "snapshots": args.snap, "shots_per_step": S, "receivers": int(len(w.shots[0].recpos)),
+                   "parallelism": "one GPU, one shot at a time" if S == 1 else
+                   f"one GPU, {S} shots per launch" + (f" ({shape[0]} per workgroup, grid {shape[1]})" if shape else "")},
         "gcell_updates_per_s": round(gcells, 3),
-        "gradient_parity": "bit-identical to the reference FP32 kernels (tests/test_gpu_parity.py)",
+        "gradient_parity": ("bit-identical to the reference FP32 kernels (tests/test_gpu_parity.py)" if args.snap == "fp32"
+                            else f"{args.snap} snapshots: bit-identical to the oracle with the same quantisation, "
+                                 f"<=1e-3 rel. L2 vs the FP64 build (tests/test_gpu_precision.py)"),
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -72,13 +72,17 @@ def lib():
         L.ifos_memcpy_d2h.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.ifos_memcpy_d2d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.ifos_engine_create.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(C.c_void_p)]
+        L.ifos_engine_create_batch.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.ifos_engine_destroy.argtypes = [C.c_void_p]
         L.ifos_engine_destroy.restype = None
         L.ifos_engine_kernel_bytes_per_cell.restype = C.c_double
         L.ifos_engine_kernel_bytes_per_cell.argtypes = [C.c_void_p, C.c_int]
         for n in ["set_model", "set_shot", "forward", "seismograms", "fields", "residual", "adjoint",
-                  "shot_gradient", "gradient", "reset_gradient", "time_kernel", "nan_step"]:
+                  "shot_gradient", "gradient", "reset_gradient", "time_kernel", "nan_step",
+                  "set_snapshot_precision", "set_shots", "shot_misfit", "set_launch", "get_launch", "autotune"]:
             getattr(L, "ifos_engine_" + n).restype = C.c_int
+        L.ifos_snapshot_bytes.restype = C.c_size_t
+        L.ifos_snapshot_bytes.argtypes = [C.POINTER(Config), C.c_int]
         _LIB = L
     return _LIB
 
@@ -131,13 +135,18 @@ class Engine:
     (src/IFOS2D.c:1076-2724): set_model -> set_shot -> forward / residual /
     adjoint -> gradient."""
 
-    def __init__(self, config, device=0):
+    def __init__(self, config, device=0, slots=1):
         self.cfg = config
+        self.slots = slots
         h = C.c_void_p()
-        _check(lib().ifos_engine_create(C.byref(config), device, C.byref(h)))
+        if slots == 1:
+            _check(lib().ifos_engine_create(C.byref(config), device, C.byref(h)))
+        else:
+            _check(lib().ifos_engine_create_batch(C.byref(config), device, slots, C.byref(h)))
         self.h = h
         self.ntr = 0
         self.nsrc = 0
+        self.nshots = 0
 
     def close(self):
         if self.h:
@@ -158,7 +167,44 @@ class Engine:
         srcpos = _f32(np.asarray(srcpos, np.float32).reshape(-1, SRC_STRIDE))
         recpos = np.ascontiguousarray(np.asarray(recpos, np.int32).reshape(-1, 2))
         self.nsrc, self.ntr = srcpos.shape[0], recpos.shape[0]
+        self.nshots = 1
+        self.trace_counts = [self.ntr]
         _check(lib().ifos_engine_set_shot(self.h, self.nsrc, _fp(srcpos), self.ntr, _fp(recpos, C.c_int)))
+
+    def set_shots(self, shots):
+        """Several shots at once, one per slot: a list of (srcpos, recpos).
+        Seismograms, observed traces and kill masks are then the slots'
+        traces concatenated in slot order."""
+        srcs = [np.asarray(sp, np.float32).reshape(-1, SRC_STRIDE) for sp, _ in shots]
+        recs = [np.asarray(rp, np.int32).reshape(-1, 2) for _, rp in shots]
+        ns = np.array([a.shape[0] for a in srcs], np.int32)
+        nr = np.array([a.shape[0] for a in recs], np.int32)
+        srcpos = _f32(np.concatenate(srcs) if len(srcs) else np.zeros((0, SRC_STRIDE), np.float32))
+        recpos = np.ascontiguousarray(np.concatenate(recs), np.int32)
+        self.nsrc, self.ntr, self.nshots = int(ns.sum()), int(nr.sum()), len(shots)
+        self.trace_counts = [int(v) for v in nr]
+        _check(lib().ifos_engine_set_shots(self.h, len(shots), _fp(ns, C.c_int), _fp(srcpos), _fp(nr, C.c_int),
+                                           _fp(recpos, C.c_int)))
+
+    def shot_misfit(self):
+        """(L2, energy) per shot of the last residual, FP64."""
+        out = np.zeros((self.nshots, 2), np.float64)
+        _check(lib().ifos_engine_shot_misfit(self.h, _fp(out, C.c_double)))
+        return out
+
+    def set_launch(self, spb=1, grid=0):
+        _check(lib().ifos_engine_set_launch(self.h, spb, grid))
+
+    def get_launch(self):
+        a, b = C.c_int(), C.c_int()
+        _check(lib().ifos_engine_get_launch(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, reps=5):
+        """Pick shots per workgroup and the persistent grid by timing; returns (spb, grid, ms per step)."""
+        a, b, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().ifos_engine_autotune(self.h, reps, C.byref(a), C.byref(b), C.byref(ms)))
+        return a.value, b.value, ms.value
 
     def forward(self, snapshots=False):
         _check(lib().ifos_engine_forward(self.h, int(bool(snapshots))))
@@ -212,3 +258,15 @@ class Engine:
 
     def kernel_bytes_per_cell(self, kernel):
         return lib().ifos_engine_kernel_bytes_per_cell(self.h, kernel)
+
+    def set_snapshot_precision(self, prec):
+        """0 / "fp32", 1 / "bf16" or 2 / "fp16" storage of the forward snapshots."""
+        _check(lib().ifos_engine_set_snapshot_precision(self.h, SNAP_PRECISIONS.get(prec, prec)))
+
+
+SNAP_PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
+
+
+def snapshot_bytes(config, prec=0):
+    """Device bytes of one shot's snapshot storage at that precision."""
+    return int(lib().ifos_snapshot_bytes(C.byref(config), SNAP_PRECISIONS.get(prec, prec)))

@@ -12,9 +12,10 @@ namespace ifos {
 // staggered derivatives of the stresses at a vx/vy node
 // (src/update_v_PML.c:358-381 for order 8; the other orders extend the sum)
 template <int N>
-__device__ __forceinline__ void stress_derivs(const float *hc, const float *sxx, const float *syy, const float *sxy,
+__device__ __forceinline__ void stress_derivs(const float *hc, const float *sxx_, const float *syy_, const float *sxy_,
                                               int P, float &sxx_x, float &sxy_x, float &sxy_y, float &syy_y)
 {
+    const auto sxx = gmem(sxx_), syy = gmem(syy_), sxy = gmem(sxy_);
     sxx_x = hc[1] * (sxx[1] - sxx[0]);
     sxy_x = hc[1] * (sxy[0] - sxy[-1]);
     sxy_y = hc[1] * (sxy[0] - sxy[-P]);
@@ -36,48 +37,49 @@ __device__ __forceinline__ void cpml_velocity(const StepGeom &g, int j, int i, f
     const Cpml &pm = g.pml;
     if (!g.BOUNDARY && i <= g.FW) {
         const int h = i - 1;
-        float *p1 = psi_sxx_x + (long)(j - 1) * g.px + h;
-        float *p2 = psi_sxy_x + (long)(j - 1) * g.px + h;
-        *p1 = pm.b_xh[h] * *p1 + pm.a_xh[h] * sxx_x;
-        sxx_x = sxx_x / pm.K_xh[h] + *p1;
-        *p2 = pm.b_x[h] * *p2 + pm.a_x[h] * sxy_x;
-        sxy_x = sxy_x / pm.K_x[h] + *p2;
+        auto p1 = gmem(psi_sxx_x) + (long)(j - 1) * g.px + h;
+        auto p2 = gmem(psi_sxy_x) + (long)(j - 1) * g.px + h;
+        *p1 = gmem(pm.b_xh)[h] * *p1 + gmem(pm.a_xh)[h] * sxx_x;
+        sxx_x = sxx_x / gmem(pm.K_xh)[h] + *p1;
+        *p2 = gmem(pm.b_x)[h] * *p2 + gmem(pm.a_x)[h] * sxy_x;
+        sxy_x = sxy_x / gmem(pm.K_x)[h] + *p2;
     }
     if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
         const int h = i - g.NX + 2 * g.FW - 1;
-        float *p1 = psi_sxx_x + (long)(j - 1) * g.px + h;
-        float *p2 = psi_sxy_x + (long)(j - 1) * g.px + h;
-        *p1 = pm.b_xh[h] * *p1 + pm.a_xh[h] * sxx_x;
-        sxx_x = sxx_x / pm.K_xh[h] + *p1;
-        *p2 = pm.b_x[h] * *p2 + pm.a_x[h] * sxy_x;
-        sxy_x = sxy_x / pm.K_x[h] + *p2;
+        auto p1 = gmem(psi_sxx_x) + (long)(j - 1) * g.px + h;
+        auto p2 = gmem(psi_sxy_x) + (long)(j - 1) * g.px + h;
+        *p1 = gmem(pm.b_xh)[h] * *p1 + gmem(pm.a_xh)[h] * sxx_x;
+        sxx_x = sxx_x / gmem(pm.K_xh)[h] + *p1;
+        *p2 = gmem(pm.b_x)[h] * *p2 + gmem(pm.a_x)[h] * sxy_x;
+        sxy_x = sxy_x / gmem(pm.K_x)[h] + *p2;
     }
     if (!g.FREE_SURF && j <= g.FW) {
         const int h = j - 1;
-        float *p1 = psi_syy_y + (long)h * g.py + (i - 1);
-        float *p2 = psi_sxy_y + (long)h * g.py + (i - 1);
-        *p1 = pm.b_yh[h] * *p1 + pm.a_yh[h] * syy_y;
-        syy_y = syy_y / pm.K_yh[h] + *p1;
-        *p2 = pm.b_y[h] * *p2 + pm.a_y[h] * sxy_y;
-        sxy_y = sxy_y / pm.K_y[h] + *p2;
+        auto p1 = gmem(psi_syy_y) + (long)h * g.py + (i - 1);
+        auto p2 = gmem(psi_sxy_y) + (long)h * g.py + (i - 1);
+        *p1 = gmem(pm.b_yh)[h] * *p1 + gmem(pm.a_yh)[h] * syy_y;
+        syy_y = syy_y / gmem(pm.K_yh)[h] + *p1;
+        *p2 = gmem(pm.b_y)[h] * *p2 + gmem(pm.a_y)[h] * sxy_y;
+        sxy_y = sxy_y / gmem(pm.K_y)[h] + *p2;
     }
     if (j >= g.NY - g.FW + 1) {
         const int h = j - g.NY + 2 * g.FW - 1;
-        float *p1 = psi_syy_y + (long)h * g.py + (i - 1);
-        float *p2 = psi_sxy_y + (long)h * g.py + (i - 1);
-        *p1 = pm.b_yh[h] * *p1 + pm.a_yh[h] * syy_y;
-        syy_y = syy_y / pm.K_yh[h] + *p1;
-        *p2 = pm.b_y[h] * *p2 + pm.a_y[h] * sxy_y;
-        sxy_y = sxy_y / pm.K_y[h] + *p2;
+        auto p1 = gmem(psi_syy_y) + (long)h * g.py + (i - 1);
+        auto p2 = gmem(psi_sxy_y) + (long)h * g.py + (i - 1);
+        *p1 = gmem(pm.b_yh)[h] * *p1 + gmem(pm.a_yh)[h] * syy_y;
+        syy_y = syy_y / gmem(pm.K_yh)[h] + *p1;
+        *p2 = gmem(pm.b_y)[h] * *p2 + gmem(pm.a_y)[h] * sxy_y;
+        sxy_y = sxy_y / gmem(pm.K_y)[h] + *p2;
     }
 }
 
 // staggered derivatives of the particle velocities at a stress node
 // (src/update_s_elastic_PML.c:340-360 for order 8), times `scale`
 template <int N>
-__device__ __forceinline__ void velocity_derivs(const float *hc, const float *vx, const float *vy, int P, float scale,
+__device__ __forceinline__ void velocity_derivs(const float *hc, const float *vx_, const float *vy_, int P, float scale,
                                                 float &vxx, float &vyx, float &vxy, float &vyy)
 {
+    const auto vx = gmem(vx_), vy = gmem(vy_);
     vxx = hc[1] * (vx[0] - vx[-1]);
     vyx = hc[1] * (vy[1] - vy[0]);
     vxy = hc[1] * (vx[P] - vx[0]);
@@ -103,39 +105,39 @@ __device__ __forceinline__ void cpml_stress(const StepGeom &g, int j, int i, flo
     const Cpml &pm = g.pml;
     if (!g.BOUNDARY && i <= g.FW) {
         const int h = i - 1;
-        float *p1 = psi_vxx + (long)(j - 1) * g.px + h;
-        float *p2 = psi_vyx + (long)(j - 1) * g.px + h;
-        *p1 = pm.b_x[h] * *p1 + pm.a_x[h] * vxx;
-        vxx = vxx / pm.K_x[h] + *p1;
-        *p2 = pm.b_xh[h] * *p2 + pm.a_xh[h] * vyx;
-        vyx = vyx / pm.K_xh[h] + *p2;
+        auto p1 = gmem(psi_vxx) + (long)(j - 1) * g.px + h;
+        auto p2 = gmem(psi_vyx) + (long)(j - 1) * g.px + h;
+        *p1 = gmem(pm.b_x)[h] * *p1 + gmem(pm.a_x)[h] * vxx;
+        vxx = vxx / gmem(pm.K_x)[h] + *p1;
+        *p2 = gmem(pm.b_xh)[h] * *p2 + gmem(pm.a_xh)[h] * vyx;
+        vyx = vyx / gmem(pm.K_xh)[h] + *p2;
     }
     if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
         const int h = i - g.NX + 2 * g.FW - 1;
-        float *p1 = psi_vxx + (long)(j - 1) * g.px + h;
-        float *p2 = psi_vyx + (long)(j - 1) * g.px + h;
-        *p1 = pm.b_x[h] * *p1 + pm.a_x[h] * vxx;
-        vxx = vxx / pm.K_x[h] + *p1;
-        *p2 = pm.b_xh[h] * *p2 + pm.a_xh[h] * vyx;
-        vyx = vyx / pm.K_xh[h] + *p2;
+        auto p1 = gmem(psi_vxx) + (long)(j - 1) * g.px + h;
+        auto p2 = gmem(psi_vyx) + (long)(j - 1) * g.px + h;
+        *p1 = gmem(pm.b_x)[h] * *p1 + gmem(pm.a_x)[h] * vxx;
+        vxx = vxx / gmem(pm.K_x)[h] + *p1;
+        *p2 = gmem(pm.b_xh)[h] * *p2 + gmem(pm.a_xh)[h] * vyx;
+        vyx = vyx / gmem(pm.K_xh)[h] + *p2;
     }
     if (!g.FREE_SURF && j <= g.FW) {
         const int h = j - 1;
-        float *p1 = psi_vyy + (long)h * g.py + (i - 1);
-        float *p2 = psi_vxy + (long)h * g.py + (i - 1);
-        *p1 = pm.b_y[h] * *p1 + pm.a_y[h] * vyy;
-        *p2 = pm.b_yh[h] * *p2 + pm.a_yh[h] * vxy;
-        vyy = vyy / pm.K_y[h] + *p1;
-        vxy = vxy / pm.K_yh[h] + *p2;
+        auto p1 = gmem(psi_vyy) + (long)h * g.py + (i - 1);
+        auto p2 = gmem(psi_vxy) + (long)h * g.py + (i - 1);
+        *p1 = gmem(pm.b_y)[h] * *p1 + gmem(pm.a_y)[h] * vyy;
+        *p2 = gmem(pm.b_yh)[h] * *p2 + gmem(pm.a_yh)[h] * vxy;
+        vyy = vyy / gmem(pm.K_y)[h] + *p1;
+        vxy = vxy / gmem(pm.K_yh)[h] + *p2;
     }
     if (j >= g.NY - g.FW + 1) {
         const int h = j - g.NY + 2 * g.FW - 1;
-        float *p1 = psi_vyy + (long)h * g.py + (i - 1);
-        float *p2 = psi_vxy + (long)h * g.py + (i - 1);
-        *p1 = pm.b_y[h] * *p1 + pm.a_y[h] * vyy;
-        vyy = vyy / pm.K_y[h] + *p1;
-        *p2 = pm.b_yh[h] * *p2 + pm.a_yh[h] * vxy;
-        vxy = vxy / pm.K_yh[h] + *p2;
+        auto p1 = gmem(psi_vyy) + (long)h * g.py + (i - 1);
+        auto p2 = gmem(psi_vxy) + (long)h * g.py + (i - 1);
+        *p1 = gmem(pm.b_y)[h] * *p1 + gmem(pm.a_y)[h] * vyy;
+        vyy = vyy / gmem(pm.K_y)[h] + *p1;
+        *p2 = gmem(pm.b_yh)[h] * *p2 + gmem(pm.a_yh)[h] * vxy;
+        vxy = vxy / gmem(pm.K_yh)[h] + *p2;
     }
 }
 
@@ -241,8 +243,9 @@ __device__ __forceinline__ void stress_visc(const ViscCell &c, const float *eta,
 }
 
 // explosive source amplitude at step nt (src/psource.c:46-49)
-__device__ __forceinline__ float explosive_amp(const float *s /* 1-based */, int nt, int NT, float DH)
+__device__ __forceinline__ float explosive_amp(const float *s_ /* 1-based */, int nt, int NT, float DH)
 {
+    const auto s = gmem(s_);
     float amp = 0.0f;
     const double den = 2.0 * (double)DH * (double)DH;
     if (nt == 1 && NT > 1) amp = (float)((double)s[nt + 1] / den);
@@ -255,9 +258,10 @@ __device__ __forceinline__ float add_half(float v, float amp) { return (float)((
 // horizontal derivatives on the free-surface row and the surface CPML
 // (src/surface_elastic_PML.c:66-91); vx, vy point at (1,i)
 template <int N>
-__device__ __forceinline__ void surface_derivs(const StepGeom &g, int i, const float *vx, const float *vy, int P,
+__device__ __forceinline__ void surface_derivs(const StepGeom &g, int i, const float *vx_, const float *vy_, int P,
                                                float *psi_vxxs, float &vxx, float &vyy)
 {
+    const auto vx = gmem(vx_), vy = gmem(vy_);
     vxx = 0.0f;
     vyy = 0.0f;
 #pragma unroll
@@ -271,15 +275,15 @@ __device__ __forceinline__ void surface_derivs(const StepGeom &g, int i, const f
     const Cpml &pm = g.pml;
     if (!g.BOUNDARY && i <= g.FW) {
         const int h = i - 1;
-        float *ps = psi_vxxs + h;
-        *ps = pm.b_x[h] * *ps + pm.a_x[h] * vxx;
-        vxx = vxx / pm.K_x[h] + *ps;
+        auto ps = gmem(psi_vxxs) + h;
+        *ps = gmem(pm.b_x)[h] * *ps + gmem(pm.a_x)[h] * vxx;
+        vxx = vxx / gmem(pm.K_x)[h] + *ps;
     }
     if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
         const int h = i - g.NX + 2 * g.FW - 1;
-        float *ps = psi_vxxs + h;
-        *ps = pm.b_x[h] * *ps + pm.a_x[h] * vxx;
-        vxx = vxx / pm.K_x[h] + *ps;
+        auto ps = gmem(psi_vxxs) + h;
+        *ps = gmem(pm.b_x)[h] * *ps + gmem(pm.a_x)[h] * vxx;
+        vxx = vxx / gmem(pm.K_x)[h] + *ps;
     }
 }
 

@@ -2,8 +2,10 @@
 // (include/ifos2d_hip.h).  One process drives one GPU; all state of a shot
 // stays in HBM: wavefields, CPML memory, materials, the forward snapshots
 // (slot-major [k][j][i]), seismograms, adjoint sources and the gradient
-// accumulators.  The time loop is recorded once per geometry as HIP graphs
-// of U = k*DTINV steps and replayed NT/U times; kernels read the step number
+// accumulators.  An engine holds S shot slots of one model (S = 1 for the
+// one-shot-at-a-time path); every step kernel launch advances all active
+// slots.  The time loop is recorded once per geometry as HIP graphs of
+// U = k*DTINV steps and replayed NT/U times; kernels read the step number
 // from a device counter so one graph serves every block of steps.
 #include <hip/hip_runtime.h>
 
@@ -76,55 +78,69 @@ struct ifos_engine {
     long plane = 0;       // NX*NY
     int nsnap = 0;
     float hc[7];
+    int slots = 1;        // shot slots held
+    int nslots = 0;       // slots in use by the current shots
 
-    // padded grids (origin pointers)
+    // per-slot padded grids (origin pointers of slot 0; slot s adds s*ws)
     float *slab = nullptr;
     float *vx, *vy, *sxx, *syy, *sxy, *ux, *uy, *uxy, *vxp1, *vyp1;
+    float *r = nullptr, *p = nullptr, *q = nullptr;
+    long ws = 0;          // floats per slot: 10 wavefields + 3L memory variables
+    // shared material grids
+    float *mslab = nullptr;
     float *rip, *rjp, *uipjp, *f, *g, *fipjp, *rho, *vs, *vp, *taus, *taup;
-    float *r = nullptr, *p = nullptr, *q = nullptr, *d = nullptr, *e = nullptr, *dip = nullptr;
+    float *d = nullptr, *e = nullptr, *dip = nullptr;
     float *mu_r, *pi_r, *muipjp_r, *tausipjp;  // relaxed moduli for the in-kernel GSLS coefficients
-    int nwave = 0;  // grids zeroed at the start of a run (wavefields + memory variables)
 
     // interior [NY][NX]
     float *interior = nullptr;
-    float *muss, *lamss, *vs_i, *vp_i, *rho_i, *wc, *wc_u, *wc_rho, *g_vp, *g_vs, *g_rho, *s_vp, *s_vs, *s_rho;
+    float *muss, *lamss, *vs_i, *vp_i, *rho_i, *g_vp, *g_vs, *g_rho, *s_vp, *s_vs, *s_rho;
+    float *wcbuf = nullptr;  // per slot: wc, wc_u, wc_rho (3 planes)
 
     // CPML
     float *pml = nullptr;   // 12 profiles
-    float *psi = nullptr;   // 5 x-strips then 4 y-strips
+    float *psi = nullptr;   // per slot: 5 x-strips then 4 y-strips
     float *psi_sxx_x, *psi_sxy_x, *psi_vxx, *psi_vyx, *psi_vxxs, *psi_syy_y, *psi_sxy_y, *psi_vyy, *psi_vxy;
-    size_t psi_bytes = 0;
+    long pss = 0;           // floats per slot
 
     // control
     int *ctl = nullptr;     // [0] step counter, [1] NaN step
     ifos_materials mat;
     bool model_set = false;
 
-    // shot
+    // shots: points of all slots in global arrays, slot s owning
+    // sources [src_off[s], src_off[s+1]) and traces [tr_off[s], tr_off[s+1])
     int nsrc = 0, ntr = 0, cap_src = 0, cap_tr = 0;
-    bool serial_adj = false;
-    int explosive = 0;
+    std::vector<int> src_off, tr_off;
+    int *explosive = nullptr;  // [slots] device
     int *src_i = nullptr;   // ix, iy, type  [3][cap_src]
     double *src_d = nullptr;  // sin, cos [2][cap_src]
     float *sig = nullptr;   // [cap_src][NT]
     int *rec_i = nullptr;   // rx, ry, kill [3][cap_tr]
     float *seis = nullptr;  // vx, vy, diff_vx, diff_vy, obs_vx, obs_vy: 6*[cap_tr][NT]
     double *trace_sums = nullptr;  // l2, en [2][cap_tr]
+    std::vector<double> shot_misfit;  // l2, en per shot of the last residual
 
-    // per-tile point lists (sources, receivers) for the fused step kernels
+    // per-slot, per-tile point lists (sources, receivers) for the step kernels
     int nbx = 0, nby = 0;
     int *tiles = nullptr;
     size_t tiles_cap = 0;
     int off_src_start = 0, off_src_idx = 0, off_rec_start = 0, off_rec_idx = 0;
+    LaunchShape shape;
+    StepArgs *dargs = nullptr;  // device copies of the argument blocks: forward, forward with
+                                // snapshots, adjoint, eager/timing
 
-    // snapshots
-    float *snap = nullptr;
+    // snapshots: per slot FP32 planes, or per-tile scales of all slots
+    // followed by the 16-bit codes of all slots
+    void *snap = nullptr;
     size_t snap_bytes = 0;
+    int snap_prec = 0;        // 0 FP32, 1 BF16, 2 FP16
+    bool snap_valid = false;  // the last forward stored snapshots for the current model and shot
 
     // graphs
     hipGraphExec_t gexec[2] = {nullptr, nullptr};  // forward(with snapshots), adjoint
     int gsteps[2] = {0, 0};
-    int gkey[2][5] = {{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}};
+    int gkey[2][6] = {{-1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1}};
     bool use_graphs = true;
     int swizzle = 1;
 };
@@ -154,13 +170,15 @@ static int validate(const ifos_config *c)
     return 0;
 }
 
-extern "C" int ifos_engine_create(const ifos_config *cfg, int device, ifos_engine **out)
+static int engine_create(const ifos_config *cfg, int device, int slots, ifos_engine **out)
 {
     *out = nullptr;
     ENG_TRY(validate(cfg));
+    if (slots < 1 || slots > 4096) return ifos_set_error("slots must be 1..4096 (got %d)", slots), -2;
     ifos_engine *E = new ifos_engine();
     E->c = *cfg;
     E->device = device;
+    E->slots = slots;
     const char *ng = getenv("IFOS_NO_GRAPH");
     E->use_graphs = !(ng && ng[0] == '1');
     const char *ns = getenv("IFOS_NO_SWIZZLE");
@@ -190,15 +208,17 @@ extern "C" int ifos_engine_create(const ifos_config *cfg, int device, ifos_engin
     E->nbx = (c.NX + kTileX - 1) / kTileX;
     E->nby = (c.NY + kTileY - 1) / kTileY;
     ifos_holberg(c.FDORDER, c.MAXRELERROR, E->hc);
-
-    // padded grids: 10 wavefields + 3L memory planes + 15 materials + 3L coefficient planes
     const int L = c.L;
-    const int ngrid = 10 + 15 + 6 * L;
-    if (hipMalloc(&E->slab, sizeof(float) * E->field * ngrid) != hipSuccess) {
-        ifos_set_error("device allocation of %ld grids failed", (long)ngrid);
-        return fail(-1);
+
+    // per slot: 10 wavefields + 3L memory planes
+    const int nwave = 10 + 3 * L;
+    E->ws = (long)nwave * E->field;
+    const size_t wbytes = sizeof(float) * (size_t)E->ws * slots;
+    if (hipMalloc(&E->slab, wbytes) != hipSuccess) {
+        ifos_set_error("device allocation of %d x %d wavefield grids failed", slots, nwave);
+        return fail(-3);
     }
-    hipMemset(E->slab, 0, sizeof(float) * E->field * ngrid);
+    hipMemset(E->slab, 0, wbytes);
     float *gp = E->slab + E->off;
     auto next = [&]() {
         float *r = gp;
@@ -212,7 +232,14 @@ extern "C" int ifos_engine_create(const ifos_config *cfg, int device, ifos_engin
         E->p = next(); for (int l = 1; l < L; l++) next();
         E->q = next(); for (int l = 1; l < L; l++) next();
     }
-    E->nwave = 10 + 3 * L;
+    // shared: 15 material grids + 3L coefficient planes
+    const int nmat = 15 + 3 * L;
+    if (hipMalloc(&E->mslab, sizeof(float) * E->field * nmat) != hipSuccess) {
+        ifos_set_error("device allocation of %d material grids failed", nmat);
+        return fail(-3);
+    }
+    hipMemset(E->mslab, 0, sizeof(float) * E->field * nmat);
+    gp = E->mslab + E->off;
     E->rip = next(); E->rjp = next(); E->uipjp = next(); E->f = next(); E->g = next(); E->fipjp = next();
     E->rho = next(); E->vs = next(); E->vp = next(); E->taus = next(); E->taup = next();
     E->mu_r = next(); E->pi_r = next(); E->muipjp_r = next(); E->tausipjp = next();
@@ -222,36 +249,50 @@ extern "C" int ifos_engine_create(const ifos_config *cfg, int device, ifos_engin
         E->dip = next(); for (int l = 1; l < L; l++) next();
     }
 
-    // interior arrays
-    const int nint = 14;
-    if (hipMalloc(&E->interior, sizeof(float) * E->plane * nint) != hipSuccess) {
+    // interior arrays: 11 shared + 3 correlation planes per slot
+    const int nint = 11;
+    if (hipMalloc(&E->interior, sizeof(float) * E->plane * nint) != hipSuccess ||
+        hipMalloc(&E->wcbuf, sizeof(float) * E->plane * 3 * slots) != hipSuccess) {
         ifos_set_error("device allocation of interior arrays failed");
-        return fail(-1);
+        return fail(-3);
     }
     hipMemset(E->interior, 0, sizeof(float) * E->plane * nint);
-    float **ia[nint] = {&E->muss, &E->lamss, &E->vs_i, &E->vp_i, &E->rho_i, &E->wc, &E->wc_u,
-                        &E->wc_rho, &E->g_vp, &E->g_vs, &E->g_rho, &E->s_vp, &E->s_vs, &E->s_rho};
+    float **ia[nint] = {&E->muss, &E->lamss, &E->vs_i, &E->vp_i, &E->rho_i, &E->g_vp, &E->g_vs, &E->g_rho,
+                        &E->s_vp, &E->s_vs, &E->s_rho};
     for (int k = 0; k < nint; k++) *ia[k] = E->interior + (long)k * E->plane;
 
-    // CPML profiles and memory
+    // CPML profiles and per-slot memory
     const int fw2 = c.FW > 0 ? 2 * c.FW : 1;
     std::vector<float> prof((size_t)12 * fw2, 0.0f);
     if (c.FW > 0) ifos_pml_profiles(&c, prof.data());
-    if (hipMalloc(&E->pml, sizeof(float) * prof.size()) != hipSuccess) return fail(-1);
+    if (hipMalloc(&E->pml, sizeof(float) * prof.size()) != hipSuccess) return fail(-3);
     hipMemcpy(E->pml, prof.data(), sizeof(float) * prof.size(), hipMemcpyHostToDevice);
     const size_t xs = (size_t)c.NY * fw2, ys = (size_t)fw2 * c.NX;
-    E->psi_bytes = sizeof(float) * (5 * xs + 4 * ys);
-    if (hipMalloc(&E->psi, E->psi_bytes) != hipSuccess) return fail(-1);
-    hipMemset(E->psi, 0, E->psi_bytes);
+    E->pss = (long)(5 * xs + 4 * ys);
+    if (hipMalloc(&E->psi, sizeof(float) * E->pss * slots) != hipSuccess) return fail(-3);
+    hipMemset(E->psi, 0, sizeof(float) * E->pss * slots);
     E->psi_sxx_x = E->psi; E->psi_sxy_x = E->psi + xs; E->psi_vxx = E->psi + 2 * xs;
     E->psi_vyx = E->psi + 3 * xs; E->psi_vxxs = E->psi + 4 * xs;
     float *yb = E->psi + 5 * xs;
     E->psi_syy_y = yb; E->psi_sxy_y = yb + ys; E->psi_vyy = yb + 2 * ys; E->psi_vxy = yb + 3 * ys;
 
-    if (hipMalloc(&E->ctl, sizeof(int) * 4) != hipSuccess) return fail(-1);
+    if (hipMalloc(&E->ctl, sizeof(int) * 4) != hipSuccess) return fail(-3);
     hipMemset(E->ctl, 0, sizeof(int) * 4);
+    if (hipMalloc(&E->explosive, sizeof(int) * slots) != hipSuccess) return fail(-3);
+    hipMemset(E->explosive, 0, sizeof(int) * slots);
+    if (hipMalloc(&E->dargs, sizeof(StepArgs) * 4) != hipSuccess) return fail(-3);
     *out = E;
     return 0;
+}
+
+extern "C" int ifos_engine_create(const ifos_config *cfg, int device, ifos_engine **out)
+{
+    return engine_create(cfg, device, 1, out);
+}
+
+extern "C" int ifos_engine_create_batch(const ifos_config *cfg, int device, int slots, ifos_engine **out)
+{
+    return engine_create(cfg, device, slots, out);
 }
 
 static void drop_graphs(ifos_engine *E)
@@ -259,7 +300,7 @@ static void drop_graphs(ifos_engine *E)
     for (int k = 0; k < 2; k++) {
         if (E->gexec[k]) hipGraphExecDestroy(E->gexec[k]);
         E->gexec[k] = nullptr;
-        for (int q = 0; q < 5; q++) E->gkey[k][q] = -1;
+        for (int q = 0; q < 6; q++) E->gkey[k][q] = -1;
     }
 }
 
@@ -269,7 +310,8 @@ extern "C" void ifos_engine_destroy(ifos_engine *E)
     hipSetDevice(E->device);
     if (E->stream) hipStreamSynchronize(E->stream);
     drop_graphs(E);
-    hipFree(E->slab); hipFree(E->interior); hipFree(E->pml); hipFree(E->psi); hipFree(E->ctl);
+    hipFree(E->slab); hipFree(E->mslab); hipFree(E->interior); hipFree(E->wcbuf); hipFree(E->pml);
+    hipFree(E->psi); hipFree(E->ctl); hipFree(E->explosive); hipFree(E->dargs);
     hipFree(E->src_i); hipFree(E->src_d); hipFree(E->sig); hipFree(E->rec_i); hipFree(E->seis);
     hipFree(E->trace_sums); hipFree(E->snap); hipFree(E->tiles);
     if (E->stream) hipStreamDestroy(E->stream);
@@ -293,6 +335,7 @@ extern "C" int ifos_engine_set_model(ifos_engine *E, const float *vp, const floa
     const ifos_config &c = E->c;
     if (c.L && (!taus || !taup)) return ifos_set_error("viscoelastic model needs taus and taup"), -2;
     HIP_TRY(hipStreamSynchronize(E->stream));
+    E->snap_valid = false;
     ifos_free_materials(&E->mat);
     ifos_setup_materials(&c, vp, vs, rho, taus, taup, &E->mat);
     ifos_materials &m = E->mat;
@@ -329,11 +372,22 @@ extern "C" int ifos_engine_set_model(ifos_engine *E, const float *vp, const floa
     return 0;
 }
 
-extern "C" int ifos_engine_set_shot(ifos_engine *E, int nsrc, const float *srcpos, int ntr, const int *recpos)
+// shots of the slots: slot s has nsrc[s] sources (rows of srcpos, all
+// slots' rows concatenated) and ntr[s] receivers (rows of recpos, likewise)
+extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc_of, const float *srcpos,
+                                     const int *ntr_of, const int *recpos)
 {
     HIP_TRY(hipSetDevice(E->device));
     const ifos_config &c = E->c;
-    if (nsrc < 0 || ntr < 0) return ifos_set_error("negative source or receiver count"), -2;
+    if (nshots < 1 || nshots > E->slots)
+        return ifos_set_error("%d shots for an engine of %d slots", nshots, E->slots), -2;
+    std::vector<int> so(nshots + 1, 0), to(nshots + 1, 0);
+    for (int k = 0; k < nshots; k++) {
+        if (nsrc_of[k] < 0 || ntr_of[k] < 0) return ifos_set_error("negative source or receiver count"), -2;
+        so[k + 1] = so[k] + nsrc_of[k];
+        to[k + 1] = to[k] + ntr_of[k];
+    }
+    const int nsrc = so[nshots], ntr = to[nshots];
     for (int l = 0; l < nsrc; l++) {
         int ix = (int)srcpos[l * IFOS_SRC_STRIDE], iy = (int)srcpos[l * IFOS_SRC_STRIDE + 1];
         if (ix < 1 || ix > c.NX || iy < 1 || iy > c.NY)
@@ -345,8 +399,10 @@ extern "C" int ifos_engine_set_shot(ifos_engine *E, int nsrc, const float *srcpo
             return ifos_set_error("receiver %d at (%d,%d) outside the grid", t + 1, ix, iy), -2;
     }
     HIP_TRY(hipStreamSynchronize(E->stream));
+    E->snap_valid = false;
     if (nsrc > E->cap_src) {
         hipFree(E->src_i); hipFree(E->src_d); hipFree(E->sig);
+        E->src_i = nullptr; E->src_d = nullptr; E->sig = nullptr;
         E->cap_src = nsrc;
         HIP_TRY(hipMalloc(&E->src_i, sizeof(int) * 3 * nsrc));
         HIP_TRY(hipMalloc(&E->src_d, sizeof(double) * 2 * nsrc));
@@ -355,6 +411,7 @@ extern "C" int ifos_engine_set_shot(ifos_engine *E, int nsrc, const float *srcpo
     }
     if (ntr > E->cap_tr) {
         hipFree(E->rec_i); hipFree(E->seis); hipFree(E->trace_sums);
+        E->rec_i = nullptr; E->seis = nullptr; E->trace_sums = nullptr;
         E->cap_tr = ntr;
         HIP_TRY(hipMalloc(&E->rec_i, sizeof(int) * 3 * ntr));
         HIP_TRY(hipMalloc(&E->seis, sizeof(float) * 6 * (size_t)ntr * c.NT));
@@ -362,64 +419,72 @@ extern "C" int ifos_engine_set_shot(ifos_engine *E, int nsrc, const float *srcpo
         HIP_TRY(hipMemset(E->seis, 0, sizeof(float) * 6 * (size_t)ntr * c.NT));
         drop_graphs(E);
     }
-    if (nsrc != E->nsrc || ntr != E->ntr) drop_graphs(E);
+    if (nsrc != E->nsrc || ntr != E->ntr || nshots != E->nslots) drop_graphs(E);
     E->nsrc = nsrc;
     E->ntr = ntr;
+    E->nslots = nshots;
+    E->src_off = so;
+    E->tr_off = to;
+    E->shot_misfit.assign(2 * (size_t)nshots, 0.0);
+    // SOURCE_TYPE is rewritten by update_v_PML from the last source before
+    // psource consults it (src/update_v_PML.c:558, src/IFOS2D.c:1758): per
+    // shot, the last source's type decides whether explosive sources act
+    std::vector<int> expl(nshots, 0);
+    for (int k = 0; k < nshots; k++)
+        expl[k] = nsrc_of[k] ? ((int)srcpos[(size_t)(so[k + 1] - 1) * IFOS_SRC_STRIDE + 7]) == 1 : c.SOURCE_TYPE == 1;
+    HIP_TRY(hipMemcpy(E->explosive, expl.data(), sizeof(int) * nshots, hipMemcpyHostToDevice));
     if (nsrc) {
         std::vector<int> si(3 * (size_t)E->cap_src, 0);
         std::vector<double> sd(2 * (size_t)E->cap_src, 0.0);
         for (int l = 0; l < nsrc; l++) {
-            const float *s = srcpos + (size_t)l * IFOS_SRC_STRIDE;
-            si[l] = (int)s[0];
-            si[E->cap_src + l] = (int)s[1];
-            si[2 * E->cap_src + l] = (int)s[7];
-            float azi_rad = s[6] * 3.141592653589793 / 180;  // src/update_v_PML.c:556
+            const float *sp = srcpos + (size_t)l * IFOS_SRC_STRIDE;
+            si[l] = (int)sp[0];
+            si[E->cap_src + l] = (int)sp[1];
+            si[2 * E->cap_src + l] = (int)sp[7];
+            float azi_rad = sp[6] * 3.141592653589793 / 180;  // src/update_v_PML.c:556
             sd[l] = sin(azi_rad);
             sd[E->cap_src + l] = cos(azi_rad);
         }
-        // SOURCE_TYPE is rewritten by update_v_PML from the last source before
-        // psource consults it (src/update_v_PML.c:558, src/IFOS2D.c:1758)
-        E->explosive = ((int)srcpos[(size_t)(nsrc - 1) * IFOS_SRC_STRIDE + 7]) == 1;
         std::vector<float> w((size_t)nsrc * c.NT);
         ifos_wavelet(&c, nsrc, srcpos, w.data());
         HIP_TRY(hipMemcpy(E->src_i, si.data(), sizeof(int) * si.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(E->src_d, sd.data(), sizeof(double) * sd.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(E->sig, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
-    } else {
-        E->explosive = c.SOURCE_TYPE == 1;
     }
     if (ntr) {
         std::vector<int> ri(3 * (size_t)E->cap_tr, 0);
-        std::vector<long> seen;
-        E->serial_adj = false;
         for (int t = 0; t < ntr; t++) {
             ri[t] = recpos[2 * t];
             ri[E->cap_tr + t] = recpos[2 * t + 1];
-            seen.push_back((long)recpos[2 * t + 1] * (c.NX + 1) + recpos[2 * t]);
         }
-        std::sort(seen.begin(), seen.end());
-        for (size_t k = 1; k < seen.size(); k++)
-            if (seen[k] == seen[k - 1]) E->serial_adj = true;
         HIP_TRY(hipMemcpy(E->rec_i, ri.data(), sizeof(int) * ri.size(), hipMemcpyHostToDevice));
     }
-    // bucket sources and receivers by 64x8 tile, keeping their order
+    // bucket each slot's sources and receivers by 64x8 tile, keeping their order
     const int nblk = E->nbx * E->nby;
-    auto bucket = [&](int n, auto x_of, auto y_of, std::vector<int> &start, std::vector<int> &idx) {
-        start.assign(nblk + 1, 0);
+    auto bucket = [&](const std::vector<int> &off, auto x_of, auto y_of, std::vector<int> &start,
+                      std::vector<int> &idx) {
+        const int n = off[nshots];
+        start.assign((size_t)nshots * (nblk + 1), 0);
         idx.assign(n > 0 ? n : 1, 0);
-        std::vector<int> tile(n);
-        for (int q = 0; q < n; q++) {
-            tile[q] = ((y_of(q) - 1) / kTileY) * E->nbx + (x_of(q) - 1) / kTileX;
-            start[tile[q] + 1]++;
+        int pos = 0;
+        for (int k = 0; k < nshots; k++) {
+            int *st = start.data() + (size_t)k * (nblk + 1);
+            std::vector<int> tile(off[k + 1] - off[k]);
+            for (int q = off[k]; q < off[k + 1]; q++) {
+                tile[q - off[k]] = ((y_of(q) - 1) / kTileY) * E->nbx + (x_of(q) - 1) / kTileX;
+                st[tile[q - off[k]] + 1]++;
+            }
+            st[0] = pos;
+            for (int b = 0; b < nblk; b++) st[b + 1] += st[b];
+            std::vector<int> fill(st, st + nblk);
+            for (int q = off[k]; q < off[k + 1]; q++) idx[fill[tile[q - off[k]]]++] = q;
+            pos = st[nblk];
         }
-        for (int b = 0; b < nblk; b++) start[b + 1] += start[b];
-        std::vector<int> fill(start.begin(), start.end() - 1);
-        for (int q = 0; q < n; q++) idx[fill[tile[q]]++] = q;
     };
     std::vector<int> s_start, s_idx, r_start, r_idx;
-    bucket(nsrc, [&](int q) { return (int)srcpos[(size_t)q * IFOS_SRC_STRIDE]; },
+    bucket(so, [&](int q) { return (int)srcpos[(size_t)q * IFOS_SRC_STRIDE]; },
            [&](int q) { return (int)srcpos[(size_t)q * IFOS_SRC_STRIDE + 1]; }, s_start, s_idx);
-    bucket(ntr, [&](int q) { return recpos[2 * q]; }, [&](int q) { return recpos[2 * q + 1]; }, r_start, r_idx);
+    bucket(to, [&](int q) { return recpos[2 * q]; }, [&](int q) { return recpos[2 * q + 1]; }, r_start, r_idx);
     std::vector<int> blob;
     E->off_src_start = 0;
     blob.insert(blob.end(), s_start.begin(), s_start.end());
@@ -431,12 +496,18 @@ extern "C" int ifos_engine_set_shot(ifos_engine *E, int nsrc, const float *srcpo
     blob.insert(blob.end(), r_idx.begin(), r_idx.end());
     if (blob.size() > E->tiles_cap) {
         hipFree(E->tiles);
+        E->tiles = nullptr;
         E->tiles_cap = blob.size() * 2;
         HIP_TRY(hipMalloc(&E->tiles, sizeof(int) * E->tiles_cap));
         drop_graphs(E);
     }
     HIP_TRY(hipMemcpy(E->tiles, blob.data(), sizeof(int) * blob.size(), hipMemcpyHostToDevice));
     return 0;
+}
+
+extern "C" int ifos_engine_set_shot(ifos_engine *E, int nsrc, const float *srcpos, int ntr, const int *recpos)
+{
+    return ifos_engine_set_shots(E, 1, &nsrc, srcpos, &ntr, recpos);
 }
 
 // ---------------------------------------------------------------------------
@@ -470,15 +541,16 @@ static StepArgs step_args(ifos_engine *E, int sw, bool snapshots)
     a.g = geom(E);
     a.L = c.L; a.VELOCITY = c.VELOCITY; a.ADJOINT_TYPE = c.ADJOINT_TYPE; a.NT = c.NT; a.DTINV = c.DTINV;
     a.sw = sw;
-    a.explosive = sw == 0 ? E->explosive : 0;
     a.plane = E->plane;
     a.vx = E->vx; a.vy = E->vy; a.sxx = E->sxx; a.syy = E->syy; a.sxy = E->sxy;
     a.vxp1 = E->vxp1; a.vyp1 = E->vyp1;
+    a.r = E->r; a.p = E->p; a.q = E->q;
+    a.ws = E->ws;
+    a.ls = E->field;
     a.rip = E->rip; a.rjp = E->rjp;
     a.f = E->f; a.gl = E->g; a.uipjp = E->uipjp;
     a.mu = E->mu_r; a.pi = E->pi_r; a.muipjp = E->muipjp_r; a.taus = E->taus; a.taup = E->taup;
     a.tausipjp = E->tausipjp;
-    a.r = E->r; a.p = E->p; a.q = E->q; a.ls = E->field;
     for (int l = 0; l <= kMaxL; l++) {
         const bool on = l >= 1 && l <= c.L;
         a.eta[l] = on ? E->mat.eta[l] : 0.0f;
@@ -490,6 +562,7 @@ static StepArgs step_args(ifos_engine *E, int sw, bool snapshots)
     a.psi_sxx_x = E->psi_sxx_x; a.psi_sxy_x = E->psi_sxy_x; a.psi_syy_y = E->psi_syy_y; a.psi_sxy_y = E->psi_sxy_y;
     a.psi_vxx = E->psi_vxx; a.psi_vyx = E->psi_vyx; a.psi_vyy = E->psi_vyy; a.psi_vxy = E->psi_vxy;
     a.psi_vxxs = E->psi_vxxs;
+    a.pss = E->pss;
     TileList src;
     src.start = E->tiles + E->off_src_start; src.idx = E->tiles + E->off_src_idx;
     src.ix = E->src_i; src.iy = E->src_i + E->cap_src; src.type = E->src_i + 2 * E->cap_src;
@@ -503,17 +576,46 @@ static StepArgs step_args(ifos_engine *E, int sw, bool snapshots)
     a.inj = sw == 0 ? src : rec;
     a.exp = src;
     a.rec = rec;
-    if (snapshots || sw == 1)
-        for (int k = 0; k < 5; k++) a.snap[k] = E->snap + (size_t)k * E->nsnap * E->plane;
+    a.explosive = E->explosive;
+    a.snap_prec = E->snap_prec;
+    if (snapshots || sw == 1) {
+        const size_t per = (size_t)E->nsnap * E->plane;
+        const size_t ntile = (size_t)E->nbx * E->nby;
+        a.sns = (long)(5 * per);
+        a.scs = (long)(5 * E->nsnap * ntile);
+        for (int k = 0; k < 5; k++) {
+            if (E->snap_prec == 0) {
+                a.snap[k] = (float *)E->snap + k * per;
+            } else {
+                // scales of all slots first (rounded up to 256 bytes), then the codes
+                const size_t sbytes = (sizeof(float) * 5 * E->nsnap * ntile * E->slots + 255) / 256 * 256;
+                a.sscale[k] = (float *)E->snap + k * E->nsnap * ntile;
+                a.snap16[k] = (uint16_t *)((char *)E->snap + sbytes) + k * per;
+            }
+        }
+    }
     a.muss = E->muss; a.lamss = E->lamss; a.vs_i = E->vs_i;
-    a.wc_rho = E->wc_rho; a.wc = E->wc; a.wc_u = E->wc_u;
+    a.wc = E->wcbuf; a.wc_u = E->wcbuf + E->plane; a.wc_rho = E->wcbuf + 2 * E->plane;
+    a.wcs = 3 * E->plane;
     a.seis_vx = seis_buf(E, 0); a.seis_vy = seis_buf(E, 1);
     a.base = E->ctl;
     a.nan_flag = E->ctl + 1;
     a.nbx = E->nbx;
     a.nby = E->nby;
+    a.nslots = E->nslots;
     a.swizzle = E->swizzle;
     return a;
+}
+
+// copy an argument block into device slot idx (0 forward, 1 forward with
+// snapshots, 2 adjoint, 3 eager runs and timing); kernels queued earlier on
+// the stream have finished when it returns, so a slot can be rewritten
+static int upload_args(ifos_engine *E, const StepArgs &a, int idx, const StepArgs **out)
+{
+    HIP_TRY(hipMemcpyAsync(E->dargs + idx, &a, sizeof a, hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    *out = E->dargs + idx;
+    return 0;
 }
 
 // NT steps as graph units of `unit` steps: kernels take their position k in
@@ -526,17 +628,20 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
     int unit = c.DTINV;
     while (unit < 32 && NT % (unit * 2) == 0) unit *= 2;
     const int kind = sw;
-    const int key[5] = {E->nsrc, E->ntr, snapshots ? 1 : 0, E->explosive, unit};
+    const int key[6] = {E->nsrc, E->ntr, snapshots ? 1 : 0, E->nslots, E->shape.spb * 65536 + E->shape.grid, unit};
     StepArgs a = step_args(E, sw, snapshots);
+    const LaunchShape ls = E->shape;
+    const StepArgs *da = nullptr;
     if (E->use_graphs) {
+        ENG_TRY(upload_args(E, a, sw == 1 ? 2 : (snapshots ? 1 : 0), &da));
         if (!E->gexec[kind] || E->gsteps[kind] != unit || memcmp(E->gkey[kind], key, sizeof key)) {
             if (E->gexec[kind]) hipGraphExecDestroy(E->gexec[kind]);
             E->gexec[kind] = nullptr;
             hipGraph_t graph;
             HIP_TRY(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
             for (int k = 0; k < unit; k++) {
-                launch_vel_step(a, k, E->stream);
-                launch_str_step(a, k, E->stream);
+                launch_vel_step(a, da, ls, k, E->stream);
+                launch_str_step(a, da, ls, k, E->stream);
             }
             launch_tick(E->ctl, unit, E->stream);
             HIP_TRY(hipStreamEndCapture(E->stream, &graph));
@@ -553,9 +658,10 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
         for (int u = 0; u < NT / unit; u++) HIP_TRY(hipGraphLaunch(E->gexec[kind], E->stream));
     } else {
         a.base = nullptr;  // the step number is passed directly
+        ENG_TRY(upload_args(E, a, 3, &da));
         for (int nt = 1; nt <= NT; nt++) {
-            launch_vel_step(a, nt, E->stream);
-            launch_str_step(a, nt, E->stream);
+            launch_vel_step(a, da, ls, nt, E->stream);
+            launch_str_step(a, da, ls, nt, E->stream);
         }
     }
     HIP_TRY(hipGetLastError());
@@ -564,15 +670,23 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
 
 static int zero_state(ifos_engine *E)
 {
-    HIP_TRY(hipMemsetAsync(E->slab, 0, sizeof(float) * E->field * E->nwave, E->stream));
-    HIP_TRY(hipMemsetAsync(E->psi, 0, E->psi_bytes, E->stream));
+    HIP_TRY(hipMemsetAsync(E->slab, 0, sizeof(float) * E->ws * E->nslots, E->stream));
+    HIP_TRY(hipMemsetAsync(E->psi, 0, sizeof(float) * E->pss * E->nslots, E->stream));
     HIP_TRY(hipMemsetAsync(E->ctl, 0, sizeof(int) * 2, E->stream));
     return 0;
 }
 
+static size_t snapshot_bytes(const ifos_config &c, int prec)
+{
+    const size_t plane = (size_t)c.NX * c.NY, nsnap = (size_t)(c.NT / c.DTINV);
+    const size_t ntile = (size_t)((c.NX + kTileX - 1) / kTileX) * ((c.NY + kTileY - 1) / kTileY);
+    if (prec == 0) return sizeof(float) * 5 * nsnap * plane;
+    return sizeof(uint16_t) * 5 * nsnap * plane + sizeof(float) * 5 * nsnap * ntile;
+}
+
 static int ensure_snapshots(ifos_engine *E)
 {
-    size_t need = sizeof(float) * 5 * (size_t)E->nsnap * E->plane;
+    const size_t need = snapshot_bytes(E->c, E->snap_prec) * E->slots + 256;
     if (E->snap && E->snap_bytes >= need) return 0;
     hipFree(E->snap);
     E->snap = nullptr;
@@ -603,9 +717,12 @@ extern "C" int ifos_engine_forward(ifos_engine *E, int store_snapshots)
     if (!E->model_set) return ifos_set_error("set_model must be called before forward"), -2;
     if (!E->tiles) return ifos_set_error("set_shot must be called before forward"), -2;
     if (store_snapshots) ENG_TRY(ensure_snapshots(E));
+    E->snap_valid = false;
     ENG_TRY(zero_state(E));
     ENG_TRY(run_loop(E, 0, store_snapshots != 0));
-    return check_nan(E);
+    ENG_TRY(check_nan(E));
+    E->snap_valid = store_snapshots != 0;
+    return 0;
 }
 
 extern "C" int ifos_engine_seismograms(ifos_engine *E, float *vx, float *vy)
@@ -651,6 +768,7 @@ extern "C" int ifos_engine_residual(ifos_engine *E, const float *obs_vx, const f
     }
     double l2 = 0.0, en = 0.0;
     std::vector<double> h(2 * (size_t)ntr);
+    std::fill(E->shot_misfit.begin(), E->shot_misfit.end(), 0.0);
     for (int comp = 0; comp < 2; comp++) {
         float *diff = seis_buf(E, 2 + comp);
         const bool active = comp == 0 ? (c.ADJOINT_TYPE == 1 || c.ADJOINT_TYPE == 3)
@@ -669,10 +787,13 @@ extern "C" int ifos_engine_residual(ifos_engine *E, const float *obs_vx, const f
         HIP_TRY(hipMemcpyAsync(h.data() + ntr, E->trace_sums + E->cap_tr, sizeof(double) * ntr,
                                hipMemcpyDeviceToHost, E->stream));
         HIP_TRY(hipStreamSynchronize(E->stream));
-        for (int t = 0; t < ntr; t++) {
-            l2 += h[t];
-            en += h[ntr + t];
-        }
+        for (int k = 0; k < E->nslots; k++)
+            for (int t = E->tr_off[k]; t < E->tr_off[k + 1]; t++) {
+                l2 += h[t];
+                en += h[ntr + t];
+                E->shot_misfit[2 * k] += h[t];
+                E->shot_misfit[2 * k + 1] += h[ntr + t];
+            }
     }
     if (misfit) {
         misfit[0] = l2;
@@ -684,15 +805,18 @@ extern "C" int ifos_engine_residual(ifos_engine *E, const float *obs_vx, const f
 extern "C" int ifos_engine_adjoint(ifos_engine *E)
 {
     HIP_TRY(hipSetDevice(E->device));
-    if (!E->snap) return ifos_set_error("adjoint needs a forward run with snapshots"), -2;
+    if (!E->snap || !E->snap_valid)
+        return ifos_set_error("adjoint needs a forward run with snapshots on the current model and shot"), -2;
     if (!E->tiles || !E->model_set) return ifos_set_error("adjoint needs set_model and set_shot"), -2;
     ENG_TRY(zero_state(E));
-    HIP_TRY(hipMemsetAsync(E->wc, 0, sizeof(float) * 3 * E->plane, E->stream));  // wc, wc_u, wc_rho
+    HIP_TRY(hipMemsetAsync(E->wcbuf, 0, sizeof(float) * 3 * E->plane * E->nslots, E->stream));
     ENG_TRY(run_loop(E, 1, false));
     GradArgs ga;
     ga.n = (int)E->plane; ga.DT = E->c.DT;
     ga.vp = E->vp_i; ga.vs = E->vs_i; ga.rho = E->rho_i;
-    ga.wc = E->wc; ga.wc_u = E->wc_u; ga.wc_rho = E->wc_rho;
+    ga.wc = E->wcbuf; ga.wc_u = E->wcbuf + E->plane; ga.wc_rho = E->wcbuf + 2 * E->plane;
+    ga.wcs = 3 * E->plane;
+    ga.nslots = E->nslots;
     ga.g_vp = E->g_vp; ga.g_vs = E->g_vs; ga.g_rho = E->g_rho;
     ga.s_vp = E->s_vp; ga.s_vs = E->s_vs; ga.s_rho = E->s_rho;
     launch_gradient(ga, E->stream);
@@ -724,6 +848,52 @@ extern "C" int ifos_engine_reset_gradient(ifos_engine *E)
     HIP_TRY(hipSetDevice(E->device));
     HIP_TRY(hipMemsetAsync(E->s_vp, 0, sizeof(float) * 3 * E->plane, E->stream));  // s_vp, s_vs, s_rho
     HIP_TRY(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+extern "C" int ifos_engine_set_snapshot_precision(ifos_engine *E, int prec)
+{
+    if (prec < 0 || prec > 2) return ifos_set_error("snapshot precision must be 0 (FP32), 1 (BF16) or 2 (FP16)"), -2;
+    HIP_TRY(hipSetDevice(E->device));
+    if (prec == E->snap_prec) return 0;
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    drop_graphs(E);
+    hipFree(E->snap);
+    E->snap = nullptr;
+    E->snap_bytes = 0;
+    E->snap_valid = false;
+    E->snap_prec = prec;
+    return 0;
+}
+
+extern "C" size_t ifos_snapshot_bytes(const ifos_config *c, int prec)
+{
+    if (validate(c) || prec < 0 || prec > 2) return 0;
+    return snapshot_bytes(*c, prec);
+}
+
+extern "C" int ifos_engine_shot_misfit(ifos_engine *E, double *per_shot)
+{
+    for (int k = 0; k < E->nslots; k++) {
+        per_shot[2 * k] = E->shot_misfit[2 * k];
+        per_shot[2 * k + 1] = E->shot_misfit[2 * k + 1];
+    }
+    return 0;
+}
+
+extern "C" int ifos_engine_set_launch(ifos_engine *E, int spb, int grid)
+{
+    if (spb != 1 && spb != 2 && spb != 4) return ifos_set_error("shots per workgroup must be 1, 2 or 4"), -2;
+    if (grid < 0 || grid % 8) return ifos_set_error("persistent grid must be 0 or a positive multiple of 8"), -2;
+    E->shape.spb = spb;
+    E->shape.grid = grid;
+    return 0;
+}
+
+extern "C" int ifos_engine_get_launch(ifos_engine *E, int *spb, int *grid)
+{
+    *spb = E->shape.spb;
+    *grid = E->shape.grid;
     return 0;
 }
 
@@ -794,12 +964,14 @@ extern "C" int ifos_engine_time_kernel(ifos_engine *E, int kernel, int reps, dou
     StepArgs a = step_args(E, 0, false);
     a.base = nullptr;
     a.nan_flag = nullptr;
+    const StepArgs *da = nullptr;
+    ENG_TRY(upload_args(E, a, 3, &da));
     hipEvent_t t0, t1;
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
     auto one = [&]() {
-        if (kernel == 0) launch_vel_step(a, 2, E->stream);
-        else launch_str_step(a, 2, E->stream);
+        if (kernel == 0) launch_vel_step(a, da, E->shape, 2, E->stream);
+        else launch_str_step(a, da, E->shape, 2, E->stream);
     };
     one();
     HIP_TRY(hipEventRecord(t0, E->stream));
@@ -811,5 +983,62 @@ extern "C" int ifos_engine_time_kernel(ifos_engine *E, int kernel, int reps, dou
     hipEventDestroy(t0);
     hipEventDestroy(t1);
     *ms_avg = ms / reps;
+    return 0;
+}
+
+// Launch-shape tuner: times one forward step (velocity + stress kernels, no
+// snapshot) of the current model and shots for every candidate shape --
+// 1, 2 or 4 shots per workgroup (up to the slots in use) times one
+// workgroup per work item or a persistent grid of 2, 3 or 4 workgroups per
+// compute unit -- keeps the fastest, and zeroes the wavefields it stepped.
+extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int *grid_out, double *ms_out)
+{
+    HIP_TRY(hipSetDevice(E->device));
+    if (!E->tiles || !E->model_set) return ifos_set_error("autotune needs set_model and set_shots"), -2;
+    if (reps < 1) reps = 1;
+    int ncu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, E->device));
+    StepArgs a = step_args(E, 0, false);
+    a.base = nullptr;
+    a.nan_flag = nullptr;
+    const StepArgs *da = nullptr;
+    ENG_TRY(upload_args(E, a, 3, &da));
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    double best = 1e30;
+    LaunchShape pick = E->shape;
+    for (int spb : {1, 2, 4}) {
+        if (spb > 1 && spb > E->nslots) break;
+        for (int per_cu : {0, 2, 3, 4}) {
+            LaunchShape ls;
+            ls.spb = spb;
+            ls.grid = per_cu ? (per_cu * ncu + 7) / 8 * 8 : 0;
+            launch_vel_step(a, da, ls, 2, E->stream);
+            launch_str_step(a, da, ls, 2, E->stream);
+            HIP_TRY(hipEventRecord(t0, E->stream));
+            for (int k = 0; k < reps; k++) {
+                launch_vel_step(a, da, ls, 2, E->stream);
+                launch_str_step(a, da, ls, 2, E->stream);
+            }
+            HIP_TRY(hipEventRecord(t1, E->stream));
+            HIP_TRY(hipEventSynchronize(t1));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+            if (ms / reps < best) {
+                best = ms / reps;
+                pick = ls;
+            }
+        }
+    }
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    E->shape = pick;
+    drop_graphs(E);
+    ENG_TRY(zero_state(E));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    if (spb_out) *spb_out = pick.spb;
+    if (grid_out) *grid_out = pick.grid;
+    if (ms_out) *ms_out = best;
     return 0;
 }

@@ -24,6 +24,23 @@ namespace ifos {
 
 constexpr int kMaxL = 8;
 
+// Every buffer the kernels touch is device global memory.  Pointers that
+// reach a kernel through memory (an argument block in device memory) are
+// generic to the compiler, which would then emit flat loads that share a
+// wait counter with the scalar loads of that block; these casts give global
+// loads and stores instead.
+#define IFOS_GAS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ IFOS_GAS T *gmem(T *p)
+{
+    return (IFOS_GAS T *)p;
+}
+template <class T>
+__device__ __forceinline__ const IFOS_GAS T *gmem(const T *p)
+{
+    return (const IFOS_GAS T *)p;
+}
+
 struct Cpml {
     const float *K_x, *a_x, *b_x, *K_xh, *a_xh, *b_xh;
     const float *K_y, *a_y, *b_y, *K_yh, *a_yh, *b_yh;
@@ -141,8 +158,10 @@ struct GradArgs {
     int n;                    // NX*NY
     float DT;
     const float *vp, *vs, *rho;       // [NY][NX]
-    const float *wc, *wc_u, *wc_rho;
-    float *g_vp, *g_vs, *g_rho;       // per-shot gradient
+    const float *wc, *wc_u, *wc_rho;  // slot 0; slot s adds s * wcs
+    long wcs;
+    int nslots;
+    float *g_vp, *g_vs, *g_rho;       // gradient of the current shots (summed in slot order)
     float *s_vp, *s_vs, *s_rho;       // running sums over shots (may be null)
 };
 

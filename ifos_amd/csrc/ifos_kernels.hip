@@ -530,23 +530,42 @@ IFOS_KERNEL void k_gradient(GradArgs a)
     if (e >= a.n) return;
     const float DT = a.DT;
     const float rh = a.rho[e], u = a.vs[e], pv = a.vp[e];
-    float wl = -DT * a.wc[e];
     const float muss = rh * u * u;
     const float lamss = (float)((double)(rh * pv * pv) - 2.0 * (double)muss);
-    wl = (float)((1.0 / (4.0 * (double)(lamss + muss) * (double)(lamss + muss))) * (double)wl);
-    const float gvp = (float)(2.0 * (double)pv * (double)rh * (double)wl);
-    const float wmu = -DT * a.wc_u[e];
-    const float gvs = (float)(-4.0 * (double)rh * (double)u * (double)wl + 2.0 * (double)rh * (double)u * (double)wmu);
-    const float wrs = -DT * a.wc_rho[e];
-    const float grho = (float)(((double)(pv * pv) - (2.0 * (double)u * (double)u)) * (double)wl + (double)(u * u * wmu) +
-                               (double)wrs);
+    float gvp = 0.0f, gvs = 0.0f, grho = 0.0f;
+    float tvp = 0.0f, tvs = 0.0f, trho = 0.0f;
+    if (a.s_vp) {
+        tvp = a.s_vp[e];
+        tvs = a.s_vs[e];
+        trho = a.s_rho[e];
+    }
+    // one shot after the other, in slot order, as the reference's shot loop
+    // adds them (src/IFOS2D.c:2703-2713)
+    for (int s = 0; s < a.nslots; s++) {
+        const long o = (long)s * a.wcs + e;
+        float wl = -DT * a.wc[o];
+        wl = (float)((1.0 / (4.0 * (double)(lamss + muss) * (double)(lamss + muss))) * (double)wl);
+        const float svp = (float)(2.0 * (double)pv * (double)rh * (double)wl);
+        const float wmu = -DT * a.wc_u[o];
+        const float svs =
+            (float)(-4.0 * (double)rh * (double)u * (double)wl + 2.0 * (double)rh * (double)u * (double)wmu);
+        const float wrs = -DT * a.wc_rho[o];
+        const float srho = (float)(((double)(pv * pv) - (2.0 * (double)u * (double)u)) * (double)wl +
+                                   (double)(u * u * wmu) + (double)wrs);
+        gvp = s ? gvp + svp : svp;
+        gvs = s ? gvs + svs : svs;
+        grho = s ? grho + srho : srho;
+        tvp = tvp + svp;
+        tvs = tvs + svs;
+        trho = trho + srho;
+    }
     a.g_vp[e] = gvp;
     a.g_vs[e] = gvs;
     a.g_rho[e] = grho;
     if (a.s_vp) {
-        a.s_vp[e] = a.s_vp[e] + gvp;
-        a.s_vs[e] = a.s_vs[e] + gvs;
-        a.s_rho[e] = a.s_rho[e] + grho;
+        a.s_vp[e] = tvp;
+        a.s_vs[e] = tvs;
+        a.s_rho[e] = trho;
     }
 }
 

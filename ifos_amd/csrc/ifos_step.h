@@ -1,4 +1,11 @@
 // ifos_step.h -- argument block of the fused time-step kernels.
+//
+// The kernels advance S shot slots of one model at once.  Every array that
+// belongs to a shot is laid out slot-major with a fixed stride, and the
+// pointers below address slot 0; slot s adds s * (its stride).  Sources and
+// receivers of all slots share global point arrays (seismograms and adjoint
+// sources are [trace][NT] over all slots' traces), reached through per-slot,
+// per-tile CSR lists.
 #pragma once
 #include "ifos_internal.h"
 
@@ -7,8 +14,9 @@ namespace ifos {
 constexpr int kTileX = 64;  // one wavefront across x
 constexpr int kTileY = 8;   // >= FDORDER/2 + 1 for the free surface
 
-// points of one kind bucketed by 64x8 tile: the entries of tile b are
-// idx[start[b] .. start[b+1]) (indices into ix/iy/...), in the caller's order
+// points of one kind bucketed by slot and 64x8 tile: the entries of tile b of
+// slot s are idx[start[s*(ntile+1) + b] .. start[s*(ntile+1) + b + 1])
+// (indices into ix/iy/...), in the caller's order
 struct TileList {
     const int *start, *idx;
     const int *ix, *iy, *type;
@@ -18,31 +26,61 @@ struct TileList {
 
 struct StepArgs {
     StepGeom g;
-    int L, VELOCITY, ADJOINT_TYPE, NT, DTINV, sw, explosive;
+    int L, VELOCITY, ADJOINT_TYPE, NT, DTINV, sw;
     long plane;                // NX*NY
+    // per-slot padded grids (stride ws floats between slots)
     float *vx, *vy, *sxx, *syy, *sxy;
     float *vxp1, *vyp1;        // adjoint only
+    float *r, *p, *q;
+    long ws;
+    long ls;                   // plane stride of the memory variables
+    // materials, shared by all slots
     const float *rip, *rjp;
     const float *f, *gl, *uipjp;                                 // elastic
     const float *mu, *pi, *muipjp, *taus, *taup, *tausipjp;      // viscoelastic
-    float *r, *p, *q;
-    long ls;                   // plane stride of the memory variables
     float eta[kMaxL + 1], bip[kMaxL + 1], bjm[kMaxL + 1], cip[kMaxL + 1], cjm[kMaxL + 1];
+    // per-slot CPML memory (stride pss floats)
     float *psi_sxx_x, *psi_sxy_x, *psi_syy_y, *psi_sxy_y;
     float *psi_vxx, *psi_vyx, *psi_vyy, *psi_vxy, *psi_vxxs;
+    long pss;
     TileList inj, exp, rec;
-    float *snap[5];            // rho_x, rho_y, x, y, u : [nsnap][NY][NX]; null: no snapshots
+    const int *explosive;      // [slot]: 1 when the slot's sources act as explosive (forward)
+    // forward snapshots rho_x, rho_y, x, y, u, each [nsnap][NY][NX] per slot:
+    // FP32 in snap (snap_prec 0), or 16-bit codes in snap16 (1: BF16,
+    // 2: FP16) of the value divided by a power-of-two scale, one scale per
+    // field, slot and tile in sscale[field][slot][tile]; all null: none.
+    // Slot strides: sns elements (snap / snap16), scs floats (sscale).
+    float *snap[5];
+    uint16_t *snap16[5];
+    float *sscale[5];
+    int snap_prec;
+    long sns, scs;
+    // correlation
     const float *muss, *lamss, *vs_i;
-    float *wc_rho, *wc, *wc_u;
-    float *seis_vx, *seis_vy;
-    const int *base;           // steps completed before the current graph unit
+    float *wc_rho, *wc, *wc_u;  // per slot, stride wcs floats
+    long wcs;
+    float *seis_vx, *seis_vy;   // [trace][NT], traces of all slots
+    const int *base;            // steps completed before the current graph unit
     int *nan_flag;
-    int nbx, nby;              // tiles in x and y
-    int swizzle;               // 1: XCD-contiguous tile order (see tile_of)
+    int nbx, nby;               // tiles in x and y
+    int nslots;                 // active shot slots
+    int swizzle;                // 1: XCD-contiguous work order (see item_of)
 };
 
-void launch_vel_step(const StepArgs &a, int k, hipStream_t s);
-void launch_str_step(const StepArgs &a, int k, hipStream_t s);
+// launch shape of the step kernels: spb slots per workgroup (1, 2 or 4; the
+// materials of a cell are loaded once for all of them) and, when grid > 0, a
+// persistent grid of that many workgroups (a multiple of 8) looping over the
+// work items; grid == 0 launches one workgroup per work item
+struct LaunchShape {
+    int spb = 1;
+    int grid = 0;
+};
+
+// a: host copy of the argument block (grid sizes), da: the same block in
+// device memory, which the kernels read (kept off the kernel-argument
+// registers, whose uniform values would not all fit)
+void launch_vel_step(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s);
+void launch_str_step(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s);
 void launch_tick(int *base, int by, hipStream_t s);
 
 }  // namespace ifos

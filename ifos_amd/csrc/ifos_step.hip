@@ -9,9 +9,12 @@
 //                snapshot store (src/IFOS2D.c:1805-1871) or zero-lag
 //                correlation (:2381-2419) + receiver sampling
 //                (src/seismo_ssg.c) + stability probe
-// Each thread owns one cell of a 64x8 tile; point sources and receivers are
-// bucketed per tile on the host (CSR lists in source order), so a thread only
-// scans the few points of its own tile.  The free surface needs the new
+// Each launch advances every active shot slot (one shot of the same model per
+// slot).  Work items are (tile, group of spb slots) pairs; a thread owns one
+// cell of a 64x8 tile and loops over the slots of its group, loading the
+// cell's materials once.  Point sources and receivers are bucketed per slot
+// and tile on the host (CSR lists in source order), so a thread only scans
+// the few points of its own tile.  The free surface needs the new
 // stresses of rows 2..N+1 of its column: tiles are 8 rows high and N <= 6, so
 // the top tile row holds them; they are exchanged through LDS after a barrier.
 // The arithmetic is the per-cell code of ifos_cellmath.h, bit-identical to
@@ -21,69 +24,179 @@
 
 namespace ifos {
 
-__device__ __forceinline__ int step_number(const StepArgs &a, int k) { return a.base ? *a.base + k + 1 : k; }
+__device__ __forceinline__ int step_number(const StepArgs &a, int k) { return a.base ? *gmem(a.base) + k + 1 : k; }
 
 // Workgroups are dealt round-robin to the 8 XCDs (workgroup h runs on XCD
-// h % 8), and each XCD has its own L2.  With swizzle on, XCD x gets one
-// contiguous run of tiles in row-major order, so the x- and y-halos a tile
-// shares with its neighbours are mostly fetched into the same L2 once.
-__device__ __forceinline__ int tile_of(const StepArgs &a, int h)
+// h % 8), and each XCD has its own L2.  With swizzle on, XCD x works through
+// one contiguous run [lo, hi) of the work items in row-major tile order, so
+// the halos neighbouring tiles share, and the materials the slot groups of a
+// tile share, are mostly fetched into one L2 once.
+__device__ __forceinline__ void xcd_chunk(int nitems, int x, int &lo, int &hi)
 {
-    const int nblk = a.nbx * a.nby;
-    if (!a.swizzle) return h;
-    const int per = nblk / 8, rem = nblk % 8, x = h % 8, loc = h / 8;
-    return x < rem ? x * (per + 1) + loc : rem * (per + 1) + (x - rem) * per + loc;
+    const int per = nitems / 8, rem = nitems % 8;
+    lo = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+    hi = lo + per + (x < rem ? 1 : 0);
+}
+
+// items [first, end) with the given stride for this workgroup: one item per
+// workgroup, or a persistent grid (gridDim.x a multiple of 8) looping over
+// the items of its XCD's run
+__device__ __forceinline__ void work_range(const StepArgs &a, int nitems, bool persistent, int &first, int &end,
+                                           int &stride)
+{
+    const int h = blockIdx.x;
+    int lo = 0, hi = nitems;
+    if (a.swizzle) xcd_chunk(nitems, h % 8, lo, hi);
+    if (!persistent) {
+        first = a.swizzle ? lo + h / 8 : h;
+        end = first + 1;
+        stride = 1;
+    } else if (a.swizzle) {
+        first = lo + h / 8;
+        end = hi;
+        stride = gridDim.x / 8;
+    } else {
+        first = h;
+        end = nitems;
+        stride = gridDim.x;
+    }
 }
 
 // ---------------------------------------------------------------------------
+// 16-bit snapshots.  A tile's values of one field are divided by 2^e, the
+// smallest power of two above their largest magnitude (exact in binary
+// floating point), and rounded to nearest-even BF16 or FP16.
+__device__ __forceinline__ bool has_snap(const StepArgs &a) { return a.snap[0] || a.snap16[0]; }
+
+__device__ __forceinline__ int scale_exp(float m)
+{
+    if (!(m > 0.0f) || isinf(m)) return 0;
+    return min(max(ilogbf(m) + 1, -120), 120);
+}
+
+__device__ __forceinline__ uint16_t enc16(float x, float inv_scale, int prec)
+{
+    const float y = x * inv_scale;
+    if (prec == 1) {
+        uint32_t u = __float_as_uint(y);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        return (uint16_t)(u >> 16);
+    }
+    return __builtin_bit_cast(uint16_t, (_Float16)y);
+}
+
+__device__ __forceinline__ float dec16(uint16_t h, float scale, int prec)
+{
+    if (prec == 1) return __uint_as_float((uint32_t)h << 16) * scale;
+    return (float)__builtin_bit_cast(_Float16, h) * scale;
+}
+
+// largest |v| over the 64x8 block, for NF values at once; every thread of the
+// block must call it (it holds two barriers)
+template <int NF>
+__device__ __forceinline__ void block_absmax(float (&v)[NF], float (*lds)[kTileY])
+{
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        float m = fabsf(v[f]);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (threadIdx.x == 0) lds[f][threadIdx.y] = m;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        float m = lds[f][0];
+#pragma unroll
+        for (int w = 1; w < kTileY; w++) m = fmaxf(m, lds[f][w]);
+        v[f] = m;
+    }
+    __syncthreads();
+}
+
+// store NF snapshot fields (first field index f0) of one cell of slot s for
+// snapshot slot k; with 16-bit storage every thread of the block must call it
+template <int NF>
+__device__ __forceinline__ void snap_store(const StepArgs &a, int s, int f0, long k, long e, int blk, bool live,
+                                           const float (&val)[NF], float (*lds)[kTileY])
+{
+    const long o = (long)s * a.sns + k * a.plane + e;
+    if (a.snap_prec == 0) {
+        if (live)
+#pragma unroll
+            for (int f = 0; f < NF; f++) gmem(a.snap[f0 + f])[o] = val[f];
+        return;
+    }
+    float m[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) m[f] = live ? val[f] : 0.0f;
+    block_absmax<NF>(m, lds);
+    const long ntile = (long)a.nbx * a.nby;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const int ex = scale_exp(m[f]);
+        if (live) gmem(a.snap16[f0 + f])[o] = enc16(val[f], ldexpf(1.0f, -ex), a.snap_prec);
+        if (threadIdx.x == 0 && threadIdx.y == 0) gmem(a.sscale[f0 + f])[(long)s * a.scs + k * ntile + blk] = ldexpf(1.0f, ex);
+    }
+}
+
+__device__ __forceinline__ float snap_load(const StepArgs &a, int s, int f, long k, long e, int blk)
+{
+    const long o = (long)s * a.sns + k * a.plane + e;
+    if (a.snap_prec == 0) return gmem(a.snap[f])[o];
+    const float sc = gmem(a.sscale[f])[(long)s * a.scs + k * (long)a.nbx * a.nby + blk];
+    return dec16(gmem(a.snap16[f])[o], sc, a.snap_prec);
+}
+
+// ---------------------------------------------------------------------------
+// velocity step of one cell of slot s
 template <int N>
-__global__ __launch_bounds__(512) void k_vel_step(StepArgs a, int k)
+__device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int i, int j, bool live, long c, long e,
+                                         int nt, float rip, float rjp, float (*s_red)[kTileY])
 {
     const StepGeom &g = a.g;
-    const int blk = tile_of(a, blockIdx.x);
-    const int bx = blk % a.nbx, by = blk / a.nbx;
-    const int i = bx * kTileX + threadIdx.x + 1;
-    const int j = by * kTileY + threadIdx.y + 1;
-    if (i > g.NX || j > g.NY) return;
-    const int nt = step_number(a, k);
     const int P = g.pitch;
-    const long c = (long)j * P + i;
-    float sxx_x, sxy_x, sxy_y, syy_y;
-    stress_derivs<N>(g.hc, a.sxx + c, a.syy + c, a.sxy + c, P, sxx_x, sxy_x, sxy_y, syy_y);
-    cpml_velocity(g, j, i, a.psi_sxx_x, a.psi_sxy_x, a.psi_syy_y, a.psi_sxy_y, sxx_x, sxy_x, sxy_y, syy_y);
-    const float rip = a.rip[c], rjp = a.rjp[c];
-    const float vx0 = a.vx[c], vy0 = a.vy[c];
-    const long e = (long)(j - 1) * g.NX + (i - 1);
-    if (a.sw == 0) {
-        if (a.snap[0] && (nt - 1) % a.DTINV == 0) {
-            const long o = (long)((nt - 1) / a.DTINV) * a.plane + e;
-            a.snap[0][o] = vel_rate(rip, sxx_x + sxy_y, g.DH);
-            a.snap[1][o] = vel_rate(rjp, sxy_x + syy_y, g.DH);
-        }
-    } else if (a.VELOCITY == 0) {
-        a.vxp1[c] += vx0 * g.DT;
-        a.vyp1[c] += vy0 * g.DT;
-    } else if ((a.NT - nt) % a.DTINV == 0) {
-        a.vxp1[c] = vx0;
-        a.vyp1[c] = vy0;
+    const long wo = (long)s * a.ws, po = (long)s * a.pss;
+    const auto vxs = gmem(a.vx) + wo, vys = gmem(a.vy) + wo;
+    float sxx_x = 0.0f, sxy_x = 0.0f, sxy_y = 0.0f, syy_y = 0.0f, vx0 = 0.0f, vy0 = 0.0f;
+    if (live) {
+        stress_derivs<N>(g.hc, a.sxx + wo + c, a.syy + wo + c, a.sxy + wo + c, P, sxx_x, sxy_x, sxy_y, syy_y);
+        cpml_velocity(g, j, i, a.psi_sxx_x + po, a.psi_sxy_x + po, a.psi_syy_y + po, a.psi_sxy_y + po, sxx_x, sxy_x,
+                      sxy_y, syy_y);
+        vx0 = vxs[c];
+        vy0 = vys[c];
     }
+    if (a.sw == 0) {
+        if (has_snap(a) && (nt - 1) % a.DTINV == 0) {
+            const float val[2] = {vel_rate(rip, sxx_x + sxy_y, g.DH), vel_rate(rjp, sxy_x + syy_y, g.DH)};
+            snap_store<2>(a, s, 0, (nt - 1) / a.DTINV, e, blk, live, val, s_red);
+        }
+    } else if (!live) {
+    } else if (a.VELOCITY == 0) {
+        gmem(a.vxp1)[wo + c] += vx0 * g.DT;
+        gmem(a.vyp1)[wo + c] += vy0 * g.DT;
+    } else if ((a.NT - nt) % a.DTINV == 0) {
+        gmem(a.vxp1)[wo + c] = vx0;
+        gmem(a.vyp1)[wo + c] = vy0;
+    }
+    if (!live) return;
     float vx = vel_new(vx0, g.DT, rip, sxx_x + sxy_y, g.DH);
     float vy = vel_new(vy0, g.DT, rjp, sxy_x + syy_y, g.DH);
 
     // point forces (forward) / residual injection (adjoint) of this tile, in order
-    const int b0 = a.inj.start[blk], b1 = a.inj.start[blk + 1];
-    for (int q = b0; q < b1; q++) {
-        const int l = a.inj.idx[q];
-        if (a.inj.ix[l] != i || a.inj.iy[l] != j) continue;
-        const float s0 = a.inj.sig[(long)l * a.NT + nt - 1];
-        const float s1 = a.inj.sig1[(long)l * a.NT + nt - 1];
+    const auto st = gmem(a.inj.start) + (long)s * (a.nbx * a.nby + 1);
+    for (int q = st[blk]; q < st[blk + 1]; q++) {
+        const int l = gmem(a.inj.idx)[q];
+        if (gmem(a.inj.ix)[l] != i || gmem(a.inj.iy)[l] != j) continue;
+        const float s0 = gmem(a.inj.sig)[(long)l * a.NT + nt - 1];
+        const float s1 = gmem(a.inj.sig1)[(long)l * a.NT + nt - 1];
         if (a.sw == 0) {
-            const int type = a.inj.type[l];
+            const int type = gmem(a.inj.type)[l];
             if (type == 2) vx += (g.DT * rip * s1) / (g.DH * g.DH);
             if (type == 3) vy += (g.DT * rjp * s1) / (g.DH * g.DH);
             if (type == 4) {
-                vx += (g.DT * rip * a.inj.sin_azi[l] * s0) / (g.DH * g.DH);
-                vy += (g.DT * rjp * a.inj.cos_azi[l] * s0) / (g.DH * g.DH);
+                vx += (g.DT * rip * gmem(a.inj.sin_azi)[l] * s0) / (g.DH * g.DH);
+                vy += (g.DT * rjp * gmem(a.inj.cos_azi)[l] * s0) / (g.DH * g.DH);
             }
         } else {
             if (a.ADJOINT_TYPE == 1) { vx += s0; vy += s1; }
@@ -91,67 +204,88 @@ __global__ __launch_bounds__(512) void k_vel_step(StepArgs a, int k)
             if (a.ADJOINT_TYPE == 3) vx += s0;
         }
     }
-    a.vx[c] = vx;
-    a.vy[c] = vy;
+    vxs[c] = vx;
+    vys[c] = vy;
+}
+
+template <int N, int SPB>
+__global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ pa, int k, int persistent)
+{
+    const StepArgs &a = *pa;
+    __shared__ float s_red[2][kTileY];
+    const StepGeom &g = a.g;
+    const int ngroups = (a.nslots + SPB - 1) / SPB;
+    const int nt = step_number(a, k);
+    int first, end, stride;
+    work_range(a, a.nbx * a.nby * ngroups, persistent != 0, first, end, stride);
+    for (int it = first; it < end; it += stride) {
+        const int blk = it / ngroups, s0 = (it % ngroups) * SPB;
+        const int bx = blk % a.nbx, by = blk / a.nbx;
+        const int i = bx * kTileX + threadIdx.x + 1;
+        const int j = by * kTileY + threadIdx.y + 1;
+        const bool live = i <= g.NX && j <= g.NY;
+        const long c = (long)j * g.pitch + i;
+        const long e = (long)(j - 1) * g.NX + (i - 1);
+        const float rip = live ? gmem(a.rip)[c] : 0.0f, rjp = live ? gmem(a.rjp)[c] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < SPB; u++)
+            if (s0 + u < a.nslots) vel_cell<N>(a, s0 + u, blk, i, j, live, c, e, nt, rip, rjp, s_red);
+    }
 }
 
 // ---------------------------------------------------------------------------
+// per-cell material values of the stress step, loaded once per work item
+template <int L>
+struct StressMat {
+    float fe, ge, fipjp;  // elastic
+    ViscCell vc;          // viscoelastic
+};
+
+// stress step of one cell of slot s
 template <int N, int L>
-__global__ __launch_bounds__(512) void k_str_step(StepArgs a, int k)
+__device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int i, int j, bool live, bool top, long c,
+                                         long e, int nt, const StressMat<L> &M, float (*s_syy)[kTileX],
+                                         float (*s_sxy)[kTileX], float (*s_red)[kTileY])
 {
-    __shared__ float s_syy[kTileY][kTileX];
-    __shared__ float s_sxy[kTileY][kTileX];
     const StepGeom &g = a.g;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int blk = tile_of(a, blockIdx.x);
-    const int bx = blk % a.nbx, by = blk / a.nbx;
-    const int i = bx * kTileX + tx + 1;
-    const int j = by * kTileY + ty + 1;
-    const bool live = i <= g.NX && j <= g.NY;
-    const bool top = by == 0 && g.FREE_SURF;
-    const int nt = step_number(a, k);
     const int P = g.pitch;
-    const long c = (long)j * P + i;
-    const long e = (long)(j - 1) * g.NX + (i - 1);
-    const bool snap_step = a.sw == 0 && a.snap[0] && (nt - 1) % a.DTINV == 0;
+    const long wo = (long)s * a.ws, po = (long)s * a.pss;
+    const const auto vxs = gmem(a.vx) + wo, vys = gmem(a.vy) + wo;
+    const bool snap_step = a.sw == 0 && has_snap(a) && (nt - 1) % a.DTINV == 0;
     const bool corr_step = a.sw == 1 && (a.NT - nt) % a.DTINV == 0;
     const bool rates = snap_step && a.VELOCITY == 1;
 
     float sxx = 0.0f, syy = 0.0f, sxy = 0.0f, ux = 0.0f, uy = 0.0f, uxy = 0.0f;
     float rr[L > 0 ? L : 1], pp[L > 0 ? L : 1], qq[L > 0 ? L : 1];
-    ViscCell vc;
-    float fe = 0.0f, ge = 0.0f;
     if (live) {
         float vxx, vyx, vxy, vyy;
         const float scale = L ? (float)(1.0 / (double)g.DH) : g.DT / g.DH;
-        velocity_derivs<N>(g.hc, a.vx + c, a.vy + c, P, scale, vxx, vyx, vxy, vyy);
-        cpml_stress(g, j, i, a.psi_vxx, a.psi_vyx, a.psi_vyy, a.psi_vxy, vxx, vyx, vxy, vyy);
-        sxx = a.sxx[c];
-        syy = a.syy[c];
-        sxy = a.sxy[c];
+        velocity_derivs<N>(g.hc, a.vx + wo + c, a.vy + wo + c, P, scale, vxx, vyx, vxy, vyy);
+        cpml_stress(g, j, i, a.psi_vxx + po, a.psi_vyx + po, a.psi_vyy + po, a.psi_vxy + po, vxx, vyx, vxy, vyy);
+        sxx = gmem(a.sxx)[wo + c];
+        syy = gmem(a.syy)[wo + c];
+        sxy = gmem(a.sxy)[wo + c];
         if (L == 0) {
-            fe = a.f[c];
-            ge = a.gl[c];
-            const float fipjp = a.uipjp[c];
-            if (rates) rates_elastic(fe, ge, fipjp, g.DT, vxx, vyx, vxy, vyy, ux, uy, uxy);
-            stress_elastic(fe, ge, fipjp, vxx, vyx, vxy, vyy, sxx, syy, sxy);
+            if (rates) rates_elastic(M.fe, M.ge, M.fipjp, g.DT, vxx, vyx, vxy, vyy, ux, uy, uxy);
+            stress_elastic(M.fe, M.ge, M.fipjp, vxx, vyx, vxy, vyy, sxx, syy, sxy);
         } else {
-            visc_coeffs(L, g.DT, a.mu[c], a.pi[c], a.muipjp[c], a.taus[c], a.taup[c], a.tausipjp[c], vc);
 #pragma unroll
             for (int l = 0; l < L; l++) {
-                rr[l] = a.r[c + (long)l * a.ls];
-                pp[l] = a.p[c + (long)l * a.ls];
-                qq[l] = a.q[c + (long)l * a.ls];
+                rr[l] = gmem(a.r)[wo + c + (long)l * a.ls];
+                pp[l] = gmem(a.p)[wo + c + (long)l * a.ls];
+                qq[l] = gmem(a.q)[wo + c + (long)l * a.ls];
             }
-            stress_visc<(L > 0 ? L : 1)>(vc, a.eta, a.bip, a.bjm, a.cip, a.cjm, g.DT, vxx, vyx, vxy, vyy, sxx, syy,
+            stress_visc<(L > 0 ? L : 1)>(M.vc, a.eta, a.bip, a.bjm, a.cip, a.cjm, g.DT, vxx, vyx, vxy, vyy, sxx, syy,
                                        sxy, rr, pp, qq, rates, ux, uy, uxy);
         }
         // explosive sources of this tile, applied after the stress update
         // and before the free surface (src/IFOS2D.c:1758-1773)
-        if (a.sw == 0 && a.explosive) {
-            for (int q = a.exp.start[blk]; q < a.exp.start[blk + 1]; q++) {
-                const int l = a.exp.idx[q];
-                if (a.exp.ix[l] != i || a.exp.iy[l] != j) continue;
+        if (a.sw == 0 && gmem(a.explosive)[s]) {
+            const auto st = gmem(a.exp.start) + (long)s * (a.nbx * a.nby + 1);
+            for (int q = st[blk]; q < st[blk + 1]; q++) {
+                const int l = gmem(a.exp.idx)[q];
+                if (gmem(a.exp.ix)[l] != i || gmem(a.exp.iy)[l] != j) continue;
                 const float amp = explosive_amp(a.exp.sig + (long)l * a.NT - 1, nt, a.NT, g.DH);
                 sxx = add_half(sxx, amp);
                 syy = add_half(syy, amp);
@@ -173,57 +307,97 @@ __global__ __launch_bounds__(512) void k_str_step(StepArgs a, int k)
             }
 #pragma unroll
             for (int m = 1; m <= N; m++) {
-                a.syy[c - (long)m * P] = -s_syy[m][tx];
-                a.sxy[c - (long)m * P] = -s_sxy[m - 1][tx];
+                gmem(a.syy)[wo + c - (long)m * P] = -s_syy[m][tx];
+                gmem(a.sxy)[wo + c - (long)m * P] = -s_sxy[m - 1][tx];
             }
             float vxx, vyy;
-            surface_derivs<N>(g, i, a.vx + c, a.vy + c, P, a.psi_vxxs, vxx, vyy);
+            surface_derivs<N>(g, i, a.vx + wo + c, a.vy + wo + c, P, a.psi_vxxs + po, vxx, vyy);
             if (L == 0) {
-                surface_elastic_update((float)((double)fe * 2.0), ge, g.DT, vxx, vyy, sxx, rates, ux);
+                surface_elastic_update((float)((double)M.fe * 2.0), M.ge, g.DT, vxx, vyy, sxx, rates, ux);
             } else {
-                surface_visc_update<(L > 0 ? L : 1)>(vc.mu, vc.pi, vc.taus, vc.taup, a.eta, g.DT, vxx, vyy, sxx, pp,
-                                                     rates, ux);
+                surface_visc_update<(L > 0 ? L : 1)>(M.vc.mu, M.vc.pi, M.vc.taus, M.vc.taup, a.eta, g.DT, vxx, vyy,
+                                                     sxx, pp, rates, ux);
             }
         }
+        __syncthreads();  // the exchange rows are reused by the next slot
+    }
+    if (snap_step) {
+        const float val[3] = {a.VELOCITY ? ux : sxx, a.VELOCITY ? uy : syy, a.VELOCITY ? uxy : sxy};
+        snap_store<3>(a, s, 2, (nt - 1) / a.DTINV, e, blk, live, val, s_red);
     }
     if (!live) return;
-    a.sxx[c] = sxx;
-    a.syy[c] = syy;
-    a.sxy[c] = sxy;
+    gmem(a.sxx)[wo + c] = sxx;
+    gmem(a.syy)[wo + c] = syy;
+    gmem(a.sxy)[wo + c] = sxy;
     if (L) {
 #pragma unroll
         for (int l = 0; l < L; l++) {
-            a.r[c + (long)l * a.ls] = rr[l];
-            a.p[c + (long)l * a.ls] = pp[l];
-            a.q[c + (long)l * a.ls] = qq[l];
+            gmem(a.r)[wo + c + (long)l * a.ls] = rr[l];
+            gmem(a.p)[wo + c + (long)l * a.ls] = pp[l];
+            gmem(a.q)[wo + c + (long)l * a.ls] = qq[l];
         }
     }
-    if (snap_step) {
-        const long o = (long)((nt - 1) / a.DTINV) * a.plane + e;
-        a.snap[2][o] = a.VELOCITY ? ux : sxx;
-        a.snap[3][o] = a.VELOCITY ? uy : syy;
-        a.snap[4][o] = a.VELOCITY ? uxy : sxy;
-    }
     if (corr_step) {
-        const long o = (long)((a.NT - nt) / a.DTINV) * a.plane + e;
-        float wr = a.wc_rho[e], w = a.wc[e], wu = a.wc_u[e];
-        correlate_cell(a.vxp1[c], a.vyp1[c], sxx, syy, sxy, a.snap[0][o], a.snap[1][o], a.snap[2][o], a.snap[3][o],
-                       a.snap[4][o], a.muss[e], a.lamss[e], a.vs_i[e] > 0.0f, wr, w, wu);
-        a.wc_rho[e] = wr;
-        a.wc[e] = w;
-        a.wc_u[e] = wu;
+        const long ks = (a.NT - nt) / a.DTINV, wco = (long)s * a.wcs;
+        float wr = gmem(a.wc_rho)[wco + e], w = gmem(a.wc)[wco + e], wu = gmem(a.wc_u)[wco + e];
+        correlate_cell(gmem(a.vxp1)[wo + c], gmem(a.vyp1)[wo + c], sxx, syy, sxy, snap_load(a, s, 0, ks, e, blk),
+                       snap_load(a, s, 1, ks, e, blk), snap_load(a, s, 2, ks, e, blk), snap_load(a, s, 3, ks, e, blk),
+                       snap_load(a, s, 4, ks, e, blk), gmem(a.muss)[e], gmem(a.lamss)[e], gmem(a.vs_i)[e] > 0.0f, wr, w, wu);
+        gmem(a.wc_rho)[wco + e] = wr;
+        gmem(a.wc)[wco + e] = w;
+        gmem(a.wc_u)[wco + e] = wu;
     }
     if (a.sw == 0) {
-        for (int q = a.rec.start[blk]; q < a.rec.start[blk + 1]; q++) {
-            const int t = a.rec.idx[q];
-            if (a.rec.ix[t] != i || a.rec.iy[t] != j) continue;
-            a.seis_vx[(long)t * a.NT + nt - 1] = a.vx[c];
-            a.seis_vy[(long)t * a.NT + nt - 1] = a.vy[c];
+        const auto st = gmem(a.rec.start) + (long)s * (a.nbx * a.nby + 1);
+        for (int q = st[blk]; q < st[blk + 1]; q++) {
+            const int t = gmem(a.rec.idx)[q];
+            if (gmem(a.rec.ix)[t] != i || gmem(a.rec.iy)[t] != j) continue;
+            gmem(a.seis_vx)[(long)t * a.NT + nt - 1] = vxs[c];
+            gmem(a.seis_vy)[(long)t * a.NT + nt - 1] = vys[c];
         }
     }
     if (a.nan_flag && j == g.NY / 2 && i == g.NX / 2) {
-        const float v = a.vy[c];
-        if (isnan(v) && *a.nan_flag == 0) *a.nan_flag = nt;
+        const float v = vys[c];
+        if (isnan(v) && *gmem(a.nan_flag) == 0) *gmem(a.nan_flag) = nt;
+    }
+}
+
+template <int N, int L, int SPB>
+__global__ __launch_bounds__(512) void k_str_step(const StepArgs *__restrict__ pa, int k, int persistent)
+{
+    const StepArgs &a = *pa;
+    __shared__ float s_syy[kTileY][kTileX];
+    __shared__ float s_sxy[kTileY][kTileX];
+    __shared__ float s_red[3][kTileY];
+    const StepGeom &g = a.g;
+    const int ngroups = (a.nslots + SPB - 1) / SPB;
+    const int nt = step_number(a, k);
+    int first, end, stride;
+    work_range(a, a.nbx * a.nby * ngroups, persistent != 0, first, end, stride);
+    for (int it = first; it < end; it += stride) {
+        const int blk = it / ngroups, s0 = (it % ngroups) * SPB;
+        const int bx = blk % a.nbx, by = blk / a.nbx;
+        const int i = bx * kTileX + threadIdx.x + 1;
+        const int j = by * kTileY + threadIdx.y + 1;
+        const bool live = i <= g.NX && j <= g.NY;
+        const bool top = by == 0 && g.FREE_SURF;
+        const long c = (long)j * g.pitch + i;
+        const long e = (long)(j - 1) * g.NX + (i - 1);
+        StressMat<L> M;
+        M.fe = M.ge = M.fipjp = 0.0f;
+        M.vc = ViscCell{};
+        if (live) {
+            if (L == 0) {
+                M.fe = gmem(a.f)[c];
+                M.ge = gmem(a.gl)[c];
+                M.fipjp = gmem(a.uipjp)[c];
+            } else {
+                visc_coeffs(L, g.DT, gmem(a.mu)[c], gmem(a.pi)[c], gmem(a.muipjp)[c], gmem(a.taus)[c], gmem(a.taup)[c], gmem(a.tausipjp)[c], M.vc);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < SPB; u++)
+            if (s0 + u < a.nslots) str_cell<N, L>(a, s0 + u, blk, i, j, live, top, c, e, nt, M, s_syy, s_sxy, s_red);
     }
 }
 
@@ -233,48 +407,88 @@ __global__ void k_tick(int *base, int by)
 }
 
 // ---------------------------------------------------------------------------
-void launch_vel_step(const StepArgs &a, int k, hipStream_t s)
+static int grid_of(const StepArgs &a, const LaunchShape &ls)
 {
-    dim3 b(kTileX, kTileY), gr(a.nbx * a.nby);
+    const int nitems = a.nbx * a.nby * ((a.nslots + ls.spb - 1) / ls.spb);
+    return ls.grid > 0 ? std::min(ls.grid, (nitems + 7) / 8 * 8) : nitems;
+}
+
+template <int N, int SPB>
+static void vel_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
+{
+    const int gr = grid_of(a, ls);
+    hipLaunchKernelGGL((k_vel_step<N, SPB>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid > 0 ? 1 : 0);
+}
+
+template <int N>
+static void vel_n(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
+{
+    switch (ls.spb) {
+    case 1: vel_launch<N, 1>(a, da, ls, k, s); break;
+    case 2: vel_launch<N, 2>(a, da, ls, k, s); break;
+    case 4: vel_launch<N, 4>(a, da, ls, k, s); break;
+    default: break;
+    }
+}
+
+void launch_vel_step(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
+{
     switch (a.g.N) {
-    case 1: hipLaunchKernelGGL(k_vel_step<1>, gr, b, 0, s, a, k); break;
-    case 2: hipLaunchKernelGGL(k_vel_step<2>, gr, b, 0, s, a, k); break;
-    case 3: hipLaunchKernelGGL(k_vel_step<3>, gr, b, 0, s, a, k); break;
-    case 4: hipLaunchKernelGGL(k_vel_step<4>, gr, b, 0, s, a, k); break;
-    case 5: hipLaunchKernelGGL(k_vel_step<5>, gr, b, 0, s, a, k); break;
-    case 6: hipLaunchKernelGGL(k_vel_step<6>, gr, b, 0, s, a, k); break;
+    case 1: vel_n<1>(a, da, ls, k, s); break;
+    case 2: vel_n<2>(a, da, ls, k, s); break;
+    case 3: vel_n<3>(a, da, ls, k, s); break;
+    case 4: vel_n<4>(a, da, ls, k, s); break;
+    case 5: vel_n<5>(a, da, ls, k, s); break;
+    case 6: vel_n<6>(a, da, ls, k, s); break;
+    default: break;
+    }
+}
+
+template <int N, int L, int SPB>
+static void str_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
+{
+    const int gr = grid_of(a, ls);
+    hipLaunchKernelGGL((k_str_step<N, L, SPB>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid > 0 ? 1 : 0);
+}
+
+template <int N, int L>
+static void str_nl(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
+{
+    switch (ls.spb) {
+    case 1: str_launch<N, L, 1>(a, da, ls, k, s); break;
+    case 2: str_launch<N, L, 2>(a, da, ls, k, s); break;
+    case 4: str_launch<N, L, 4>(a, da, ls, k, s); break;
     default: break;
     }
 }
 
 template <int N>
-static void str_n(const StepArgs &a, int k, hipStream_t s)
+static void str_n(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
 {
-    dim3 b(kTileX, kTileY), gr(a.nbx * a.nby);
     switch (a.L) {
-    case 0: hipLaunchKernelGGL((k_str_step<N, 0>), gr, b, 0, s, a, k); break;
-    case 1: hipLaunchKernelGGL((k_str_step<N, 1>), gr, b, 0, s, a, k); break;
-    case 2: hipLaunchKernelGGL((k_str_step<N, 2>), gr, b, 0, s, a, k); break;
-    case 3: hipLaunchKernelGGL((k_str_step<N, 3>), gr, b, 0, s, a, k); break;
-    case 4: hipLaunchKernelGGL((k_str_step<N, 4>), gr, b, 0, s, a, k); break;
-    case 5: hipLaunchKernelGGL((k_str_step<N, 5>), gr, b, 0, s, a, k); break;
+    case 0: str_nl<N, 0>(a, da, ls, k, s); break;
+    case 1: str_nl<N, 1>(a, da, ls, k, s); break;
+    case 2: str_nl<N, 2>(a, da, ls, k, s); break;
+    case 3: str_nl<N, 3>(a, da, ls, k, s); break;
+    case 4: str_nl<N, 4>(a, da, ls, k, s); break;
+    case 5: str_nl<N, 5>(a, da, ls, k, s); break;
     default: break;
     }
 }
 
-void launch_str_step(const StepArgs &a, int k, hipStream_t s)
+void launch_str_step(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
 {
     switch (a.g.N) {
-    case 1: str_n<1>(a, k, s); break;
-    case 2: str_n<2>(a, k, s); break;
-    case 3: str_n<3>(a, k, s); break;
-    case 4: str_n<4>(a, k, s); break;
-    case 5: str_n<5>(a, k, s); break;
-    case 6: str_n<6>(a, k, s); break;
+    case 1: str_n<1>(a, da, ls, k, s); break;
+    case 2: str_n<2>(a, da, ls, k, s); break;
+    case 3: str_n<3>(a, da, ls, k, s); break;
+    case 4: str_n<4>(a, da, ls, k, s); break;
+    case 5: str_n<5>(a, da, ls, k, s); break;
+    case 6: str_n<6>(a, da, ls, k, s); break;
     default: break;
     }
 }
 
-void launch_tick(int *base, int by, hipStream_t s) { hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, base, by); }
+void launch_tick(int *base, int by, hipStream_t s) { hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s, base, by); }
 
 }  // namespace ifos

@@ -82,9 +82,15 @@ void ifos_wavelet(const ifos_config *c, int nsrc, const float *srcpos, float *ou
 typedef struct ifos_engine ifos_engine;
 
 /* Snapshot storage precision (forward wavefield kept for the correlation). */
-enum { IFOS_SNAP_FP32 = 0 };
+enum { IFOS_SNAP_FP32 = 0, IFOS_SNAP_BF16 = 1, IFOS_SNAP_FP16 = 2 };
 
 int  ifos_engine_create(const ifos_config *cfg, int device, ifos_engine **out);
+/* An engine with `slots` shot slots of one model: set_shots gives each slot
+ * its own shot, and every forward / adjoint advances all of them in the same
+ * kernel launches.  The reference loops over shots one at a time
+ * (src/IFOS2D.c:1076-2724); per shot the results are identical to that
+ * loop, and the gradient sum adds the shots in slot order as it does. */
+int  ifos_engine_create_batch(const ifos_config *cfg, int device, int slots, ifos_engine **out);
 void ifos_engine_destroy(ifos_engine *e);
 /* Models: host arrays [NY][NX] (x fastest); taus/taup ignored when L == 0.
  * Performs matcopy/av_mue/av_rho/av_tau/prepare_update_s
@@ -94,11 +100,29 @@ int  ifos_engine_set_model(ifos_engine *e, const float *vp, const float *vs, con
 /* One shot: srcpos[nsrc][IFOS_SRC_STRIDE] (grid indices), recpos[ntr][2]
  * (x index, y index).  Source wavelets are generated like wavelet(). */
 int  ifos_engine_set_shot(ifos_engine *e, int nsrc, const float *srcpos, int ntr, const int *recpos);
+/* nshots shots (<= slots): shot k has nsrc[k] sources and ntr[k] receivers;
+ * srcpos and recpos hold all shots' rows concatenated in shot order.  From
+ * then on seismograms, observed traces, kill masks and adjoint sources are
+ * [sum ntr][NT] in the same order.  set_shot is set_shots with one shot. */
+int  ifos_engine_set_shots(ifos_engine *e, int nshots, const int *nsrc, const float *srcpos, const int *ntr,
+                           const int *recpos);
+/* Per shot of the last residual: per_shot[2k] = L2, per_shot[2k+1] = energy (FP64). */
+int  ifos_engine_shot_misfit(ifos_engine *e, double *per_shot);
+/* Launch shape of the step kernels: spb = shots per workgroup (1, 2, 4; a
+ * cell's materials are loaded once for all of them), grid = 0 for one
+ * workgroup per (tile, shot group) or a multiple of 8 for a persistent grid
+ * of that many workgroups. */
+int  ifos_engine_set_launch(ifos_engine *e, int spb, int grid);
+int  ifos_engine_get_launch(ifos_engine *e, int *spb, int *grid);
+/* Time every candidate launch shape on one forward step of the current
+ * model and shots (reps launches each), keep the fastest, zero the stepped
+ * wavefields; reports the pick and its time per step. */
+int  ifos_engine_autotune(ifos_engine *e, int reps, int *spb, int *grid, double *ms_per_step);
 /* Forward run (src/IFOS2D.c:1651-1894); snapshots every DTINV steps if asked. */
 int  ifos_engine_forward(ifos_engine *e, int store_snapshots);
-/* Seismograms of the last forward run, [ntr][NT] each. */
+/* Seismograms of the last forward run, [ntr][NT] each (all shots' traces). */
 int  ifos_engine_seismograms(ifos_engine *e, float *vx, float *vy);
-/* Interior of vx, vy, sxx, syy, sxy after the last run, 5*[NY][NX]. */
+/* Interior of vx, vy, sxx, syy, sxy of slot 0 after the last run, 5*[NY][NX]. */
 int  ifos_engine_fields(ifos_engine *e, float *out5);
 /* Residuals/adjoint sources from observed traces [ntr][NT] (first sample
  * zeroed as src/inseis.c:75 delivers them), optional kill mask (ntr ints).
@@ -112,10 +136,21 @@ int  ifos_engine_adjoint(ifos_engine *e);
 /* forward(with snapshots) + residual + adjoint for the current shot */
 int  ifos_engine_shot_gradient(ifos_engine *e, const float *obs_vx, const float *obs_vy, const int *kill,
                                int obs_on_device, double *misfit);
-/* which = 0: gradient of the last shot, 1: sum over shots since the reset.
+/* which = 0: gradient of the current shots (their sum in slot order; the
+ * last shot's gradient for a one-slot engine), 1: sum over shots since the reset.
  * Arrays [NY][NX]: Vp, Vs, rho gradients. */
 int  ifos_engine_gradient(ifos_engine *e, int which, float *g_vp, float *g_vs, float *g_rho);
 int  ifos_engine_reset_gradient(ifos_engine *e);
+/* Storage of the forward snapshots (src/IFOS2D.c:1805-1871 keeps them FP32):
+ * 0 = FP32 (default; gradients bit-identical to the reference), 1 = BF16,
+ * 2 = FP16.  16-bit snapshots hold value / 2^e rounded to nearest-even, with
+ * 2^e the smallest power of two above the field's largest magnitude in each
+ * 64x8 tile of each snapshot slot; the correlation and gradient stay FP32.
+ * Changing it frees the snapshot storage. */
+int  ifos_engine_set_snapshot_precision(ifos_engine *e, int prec);
+/* Device bytes of the snapshot storage of one shot at that precision
+ * (0 for an invalid configuration). */
+size_t ifos_snapshot_bytes(const ifos_config *c, int prec);
 /* Device memory helpers for callers without a GPU runtime of their own. */
 void *ifos_dev_alloc(size_t bytes);
 int  ifos_dev_free(void *p);
@@ -123,9 +158,9 @@ int  ifos_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int  ifos_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int  ifos_memcpy_d2d(void *dst, const void *src, size_t bytes);
 /* Time one kernel of the engine on its own stream with HIP events:
- * kernel 0 = velocity update, 1 = stress update, 2 = post (surface/source/
- * receivers), 3 = correlation.  Runs reps launches on the live state, then
- * restores nothing: call between shots.  Returns the average in ms. */
+ * kernel 0 = fused velocity step, 1 = fused stress step (forward step
+ * without snapshot).  Runs reps launches on the live state, then restores
+ * nothing: call between shots.  Returns the average in ms. */
 int  ifos_engine_time_kernel(ifos_engine *e, int kernel, int reps, double *ms_avg);
 /* Algorithmic HBM bytes per cell of one launch of that kernel. */
 double ifos_engine_kernel_bytes_per_cell(ifos_engine *e, int kernel);

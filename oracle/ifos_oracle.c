@@ -30,6 +30,7 @@
  *    CPML stencil form as orders 2-8; "parity unpinned" against the
  *    reference for these two orders, pinned by polynomial exactness only.
  */
+#include <stdint.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -811,6 +812,82 @@ static void time_step(shot_t *S, int nt, int sw, int nsrc, const float *srcpos, 
     }
 }
 
+/* ---- 16-bit snapshot storage (no reference counterpart) ----------------- */
+/* Float to BF16: keep the top 16 bits, rounding the dropped 16 to nearest,
+ * ties to even.  Float to FP16: 10 mantissa bits, exponent bias 15; values
+ * below 2^-14 become subnormal multiples of 2^-24, again ties to even. */
+unsigned short orc_f32_to_16(float x, int prec)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if (prec == 1) {
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        return (unsigned short)(u >> 16);
+    }
+    uint32_t sign = (u >> 16) & 0x8000u, ax = u & 0x7FFFFFFFu;
+    if (ax >= 0x7F800000u) return (unsigned short)(sign | 0x7C00u | (ax > 0x7F800000u ? 0x200u : 0u));
+    int e = (int)(ax >> 23) - 127;
+    uint32_t m = (ax & 0x7FFFFFu) | 0x800000u;
+    if (ax < 0x00800000u) return (unsigned short)sign; /* float subnormals are far below 2^-25 */
+    if (e < -25) return (unsigned short)sign;
+    uint32_t q, rem, half;
+    if (e < -14) { /* half subnormal: q = m * 2^(e+1) */
+        int sh = -(e + 1);
+        q = m >> sh;
+        rem = m & ((1u << sh) - 1u);
+        half = 1u << (sh - 1);
+    } else {
+        if (e > 15) return (unsigned short)(sign | 0x7C00u);
+        q = ((uint32_t)(e + 15) << 10) | ((m >> 13) & 0x3FFu);
+        rem = m & 0x1FFFu;
+        half = 0x1000u;
+    }
+    if (rem > half || (rem == half && (q & 1u))) q++; /* a carry moves into the exponent */
+    if (q >= 0x7C00u) q = 0x7C00u;
+    return (unsigned short)(sign | q);
+}
+
+float orc_16_to_f32(unsigned short h, int prec)
+{
+    if (prec == 1) {
+        uint32_t u = (uint32_t)h << 16;
+        float f;
+        memcpy(&f, &u, 4);
+        return f;
+    }
+    int E = (h >> 10) & 0x1F, mant = h & 0x3FF;
+    float v;
+    if (E == 0) v = ldexpf((float)mant, -24);
+    else if (E == 31) v = mant ? NAN : INFINITY;
+    else v = ldexpf((float)(1024 + mant), E - 25);
+    return (h & 0x8000) ? -v : v;
+}
+
+/* one snapshot field of one slot, quantised per 64x8 tile with the scale
+ * 2^e, e = ilogb(max |v|) + 1 clamped to [-120, 120] (0 for an all-zero tile) */
+static void quantize_plane(const orc_params *p, real *plane)
+{
+    const int NX = p->NX, NY = p->NY;
+    for (int ty = 0; ty < NY; ty += 8)
+        for (int tx = 0; tx < NX; tx += 64) {
+            const int y1 = ty + 8 < NY ? ty + 8 : NY, x1 = tx + 64 < NX ? tx + 64 : NX;
+            float m = 0.0f;
+            for (int y = ty; y < y1; y++)
+                for (int x = tx; x < x1; x++) m = fmaxf(m, fabsf((float)plane[(size_t)y * NX + x]));
+            int ex = 0;
+            if (m > 0.0f && !isinf(m)) {
+                ex = ilogbf(m) + 1;
+                ex = ex < -120 ? -120 : (ex > 120 ? 120 : ex);
+            }
+            const float inv = ldexpf(1.0f, -ex), sc = ldexpf(1.0f, ex);
+            for (int y = ty; y < y1; y++)
+                for (int x = tx; x < x1; x++) {
+                    real *v = &plane[(size_t)y * NX + x];
+                    *v = orc_16_to_f32(orc_f32_to_16((float)*v * inv, p->SNAP_PREC), p->SNAP_PREC) * sc;
+                }
+        }
+}
+
 /* ---- forward run with optional snapshot store --------------------------- */
 typedef struct {
     int nsnap;       /* NT/DTINV rounded up                                  */
@@ -844,10 +921,12 @@ static void run_forward(shot_t *S, int nsrc, const float *srcpos, int ntr, const
                 size_t plane = (size_t)NX * NY, off = (size_t)(hin - 1) * plane;
                 real **src[5] = {S->vxp1, S->vyp1, p->VELOCITY ? S->ux : S->sxx, p->VELOCITY ? S->uy : S->syy,
                                  p->VELOCITY ? S->uxy : S->sxy};
-                for (int k = 0; k < 5; k++)
+                for (int k = 0; k < 5; k++) {
                     for (int j = 1; j <= NY; j++)
                         for (int i = 1; i <= NX; i++)
                             SN->F[k][off + (size_t)(j - 1) * NX + (i - 1)] = src[k][j][i];
+                    if (p->SNAP_PREC) quantize_plane(p, SN->F[k] + off);
+                }
             }
             hin++;
             hin1 += p->DTINV;

@@ -54,6 +54,9 @@ typedef struct orc_params {
     float F_REF;           /* reference frequency of the Q model           */
     float TAU;             /* not used when taus/taup are given            */
     float FL[8];           /* relaxation frequencies, FL[1..L] used        */
+    int SNAP_PREC;         /* snapshot storage: 0 FP32 (the reference), 1 BF16,
+                              2 FP16, each with a power-of-two max-abs scale per
+                              field, slot and 64x8 tile (no reference counterpart) */
 } orc_params;
 
 /* Sources: srcpos[l*8 + k] holds the reference's srcpos_loc[k+1][l+1]
@@ -66,6 +69,10 @@ void orc_holbergcoeff(int FDORDER, int MAXRELERROR, float hc[7]);
  * K_x,a_x,b_x,K_x_half,a_x_half,b_x_half,K_y,a_y,b_y,K_y_half,a_y_half,b_y_half
  * (12 arrays). */
 void orc_pml_pro(const orc_params *p, float *out12);
+/* 16-bit snapshot codes: prec 1 BF16, 2 FP16, round to nearest even
+ * (IEEE 754 roundTiesToEven, subnormals kept). */
+unsigned short orc_f32_to_16(float x, int prec);
+float orc_16_to_f32(unsigned short h, int prec);
 /* Source wavelets: out[l*NT + (nt-1)] = signals[l+1][nt]. */
 void orc_wavelet(const orc_params *p, int nsrc, const float *srcpos, float *out);
 

@@ -21,7 +21,7 @@ _FLT_FIELDS = ["DH", "DT", "VPPML", "FPML", "npower", "k_max_PML", "F_REF", "TAU
 
 class OrcParams(C.Structure):
     _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [(n, C.c_float) for n in _FLT_FIELDS]
-                + [("FL", C.c_float * 8)])
+                + [("FL", C.c_float * 8), ("SNAP_PREC", C.c_int)])
 
 
 def make_params(**kw):
@@ -201,3 +201,18 @@ def ref_gradient(p, model, srcpos, recpos, diff_vx, diff_vy):
                             _ptr(g[0]), _ptr(g[1]), _ptr(g[2]))
     assert rc == 0, rc
     return dict(seis_vx=svx, seis_vy=svy, grad_vp=g[0], grad_vs=g[1], grad_rho=g[2])
+
+
+def f32_to_16(x, prec):
+    """16-bit snapshot codes of the oracle (prec 1 BF16, 2 FP16)."""
+    L = oracle()
+    L.orc_f32_to_16.restype = C.c_ushort
+    L.orc_f32_to_16.argtypes = [C.c_float, C.c_int]
+    return np.array([L.orc_f32_to_16(float(v), prec) for v in np.asarray(x, np.float32).ravel()], np.uint16)
+
+
+def f16_to_32(h, prec):
+    L = oracle()
+    L.orc_16_to_f32.restype = C.c_float
+    L.orc_16_to_f32.argtypes = [C.c_ushort, C.c_int]
+    return np.array([L.orc_16_to_f32(int(v), prec) for v in np.asarray(h, np.uint16).ravel()], np.float32)
