@@ -78,6 +78,74 @@ def cpu_baseline(w, steps=300):
                        f"forward+adjoint of NT={cfg.NT}; snapshot and correlation work not counted")
 
 
+def serving(args):
+    """Poisson arrivals of mixed forward-only and gradient requests on the
+    request server (include/ifos2d_serve.h).  Grids, shot counts, orders,
+    physics and NT are drawn per request from a seeded generator over a
+    reduced version of the serving workload's ranges so the run fits a few
+    minutes; observed traces are zero (the residual is then the synthetic
+    trace itself, the same work).  Latency = finish - arrival."""
+    import ifos_amd
+    from ifos_amd import workloads
+    from ifos_amd.serve import FORWARD, GRADIENT, Server
+    rng = np.random.default_rng(args.seed)
+    n = max(1, int(round(args.rate * args.duration)))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n))
+    reqs = []
+    for k in range(n):
+        nx = int(rng.integers(300, args.max_nx + 1)) // 8 * 8
+        ny = int(nx * rng.uniform(0.4, 0.5)) // 8 * 8
+        w = workloads.small_case(seed=1000 + k, NX=nx, NY=ny, NT=int(rng.integers(args.min_nt, args.max_nt + 1)) // 4 * 4,
+                                 FDORDER=int(rng.choice([4, 8])), L=int(rng.choice([0, 2])), VELOCITY=1, DTINV=4,
+                                 FW=10)
+        nshots = int(rng.integers(1, args.max_shots + 1))
+        shots = []
+        for _ in range(nshots):
+            sp = np.array(w.shots[0].srcpos, np.float32).reshape(-1, 8).copy()
+            sp[:, 0] = rng.integers(w.cfg.FW + 3, w.cfg.NX - w.cfg.FW - 3)
+            shots.append((sp, w.shots[0].recpos))
+        kind = GRADIENT if rng.random() < 0.5 else FORWARD
+        ntr = sum(len(r) for _, r in shots)
+        obs = (np.zeros((ntr, w.cfg.NT), np.float32),) * 2 if kind == GRADIENT else None
+        reqs.append((arrivals[k], w, kind, shots, obs))
+    log(f"[serve] {n} requests over {arrivals[-1]:.1f} s")
+    srv = Server(hbm_budget=int(args.budget_gb * (1 << 30)), workers=args.workers, max_slots=args.max_slots)
+    t0 = time.perf_counter()
+    ids = []
+    for at, w, kind, shots, obs in reqs:
+        while time.perf_counter() - t0 < at:
+            time.sleep(min(0.002, at - (time.perf_counter() - t0)))
+        ids.append(srv.submit(kind, w.cfg, w.start, shots, obs=obs, snap=args.snap, model_id=id(w)))
+    done = []
+    for rid, (at, w, kind, shots, obs) in zip(ids, reqs):
+        srv.wait(rid)
+        r = srv.result(rid)
+        done.append((kind, len(shots), r["times"], 2.0 * w.cfg.NX * w.cfg.NY * w.cfg.NT * len(shots)))
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([t[2] - t[0] for _, _, t, _ in done])
+    wait = np.array([t[1] - t[0] for _, _, t, _ in done])
+    gshots = sum(ns for kind, ns, _, _ in done if kind == GRADIENT)
+    fshots = sum(ns for kind, ns, _, _ in done if kind == FORWARD)
+    cells = sum(c for kind, _, _, c in done if kind == GRADIENT) + sum(c / 2 for kind, _, _, c in done if kind == FORWARD)
+    out = {"metric": "shot-gradients/s & Gcell-updates/s under Poisson arrivals (serving)",
+           "value": round(gshots / wall, 4), "unit": "shot-gradients/s", "n_gpus": 1, "higher_is_better": True,
+           "data": "synthetic (seeded layered/anomaly models, zero observed traces)",
+           "config": {"workload": "serving", "requests": n, "rate_per_s": args.rate, "duration_s": args.duration,
+                      "grid_nx": [300, args.max_nx], "nt": [args.min_nt, args.max_nt], "shots": [1, args.max_shots],
+                      "workers": args.workers, "max_slots": args.max_slots, "budget_gb": args.budget_gb,
+                      "snapshots": args.snap},
+           "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p95": round(float(np.percentile(lat, 95)), 4),
+                         "p99": round(float(np.percentile(lat, 99)), 4), "max": round(float(lat.max()), 4)},
+           "queue_wait_s": {"p50": round(float(np.percentile(wait, 50)), 4),
+                            "p99": round(float(np.percentile(wait, 99)), 4)},
+           "forward_shots_per_s": round(fshots / wall, 4), "gcell_updates_per_s": round(cells / wall / 1e9, 3),
+           "wall_s": round(wall, 2), "server": st}
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,12 +163,25 @@ def main():
                     help="only time the velocity and stress kernels on the live model (for counter runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of this many MB (counter calibration)")
+    ap.add_argument("--serve", action="store_true", help="serving trace (Poisson arrivals) on the request server")
+    ap.add_argument("--rate", type=float, default=4.0, help="serving: requests per second")
+    ap.add_argument("--duration", type=float, default=15.0, help="serving: seconds of arrivals")
+    ap.add_argument("--max-nx", type=int, default=1200)
+    ap.add_argument("--min-nt", type=int, default=600)
+    ap.add_argument("--max-nt", type=int, default=2000)
+    ap.add_argument("--max-shots", type=int, default=8)
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--max-slots", type=int, default=4)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--seed", type=int, default=7)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (every box here has one GPU; "
               f"the multi-GPU plan is shot sharding with one gradient all-reduce per iteration)", file=sys.stderr)
         return 2
 
+    if args.serve:
+        return serving(args)
     import ifos_amd
     w = build_workload(args.workload)
     cfg = w.cfg

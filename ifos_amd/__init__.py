@@ -118,6 +118,17 @@ class DeviceBuffer:
         _check(lib().ifos_memcpy_h2d(C.c_void_p(b.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes))
         return b
 
+    def to_array(self, shape, dtype=np.float32):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().ifos_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes))
+        return out
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib().ifos_memcpy_h2d(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes))
+
     def free(self):
         if self.ptr:
             lib().ifos_dev_free(C.c_void_p(self.ptr))

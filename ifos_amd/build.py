@@ -31,8 +31,10 @@ HIP_FLAGS = [
 C_FLAGS = ["-O2", "-g", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-Wall",
            "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
-HIP_SOURCES = ["ifos_kernels.hip", "ifos_step.hip", "ifos_engine.hip", "ifos_dropin.hip"]
-C_SOURCES = ["host/ifos_setup.c"]
+HIP_SOURCES = ["ifos_kernels.hip", "ifos_step.hip", "ifos_engine.hip", "ifos_dropin.hip", "ifos_opt.hip"]
+C_SOURCES = ["host/ifos_setup.c", "host/ifos_formats.c"]
+CXX_SOURCES = ["ifos_serve.cpp"]
+CXX_FLAGS = ["-O2", "-g", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
 
 def _run(cmd):
@@ -53,6 +55,10 @@ def build(verbose=False):
     for src in C_SOURCES:
         o = os.path.join(OBJ, os.path.basename(src).replace(".c", ".o"))
         jobs.append(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
+    for src in CXX_SOURCES:
+        o = os.path.join(OBJ, os.path.basename(src).replace(".cpp", ".o"))
+        jobs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
@@ -76,7 +82,7 @@ def build(verbose=False):
         raise RuntimeError("compilation failed")
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs,
-          "-Wl,--no-undefined", "-lm"])
+          "-Wl,--no-undefined", "-lm", "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,310 @@
+/* ifos_formats.c -- the reference's parameter, model and seismogram file
+ * formats (include/ifos2d_formats.h). */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ifos2d_formats.h"
+
+void ifos_set_error(const char *fmt, ...);
+
+#define LINE_MAX_CHARS 4096
+
+struct ifos_json {
+    int n, cap;
+    char **name, **value;
+};
+
+/* the first blank-delimited word of s (sscanf "%s" in
+ * remove_blankspaces_around_string, src/json_parser.c:402-415) */
+static char *first_word(const char *s, size_t len)
+{
+    size_t a = 0;
+    while (a < len && (s[a] == ' ' || s[a] == '\t' || s[a] == '\n' || s[a] == '\r')) a++;
+    size_t b = a;
+    while (b < len && !(s[b] == ' ' || s[b] == '\t' || s[b] == '\n' || s[b] == '\r')) b++;
+    char *w = (char *)malloc(b - a + 1);
+    memcpy(w, s + a, b - a);
+    w[b - a] = '\0';
+    return w;
+}
+
+static int json_add(struct ifos_json *j, char *name, char *value)
+{
+    if (j->n == j->cap) {
+        j->cap = j->cap ? 2 * j->cap : 64;
+        j->name = (char **)realloc(j->name, sizeof(char *) * j->cap);
+        j->value = (char **)realloc(j->value, sizeof(char *) * j->cap);
+    }
+    j->name[j->n] = name;
+    j->value[j->n] = value;
+    j->n++;
+    return 0;
+}
+
+static int count_char(const char *s, char c)
+{
+    int n = 0;
+    for (; *s; s++) n += *s == c;
+    return n;
+}
+
+/* split the text between two quotes at commas into up to five words */
+static int split_commas(const char *s, size_t len, char **out, int max)
+{
+    int n = 0;
+    size_t a = 0;
+    for (size_t i = 0; i <= len; i++)
+        if (i == len || s[i] == ',') {
+            if (n == max) return -1;
+            out[n++] = first_word(s + a, i - a);
+            a = i + 1;
+        }
+    return n;
+}
+
+int ifos_json_load(const char *path, ifos_json **out)
+{
+    *out = NULL;
+    FILE *fp = fopen(path, "r");
+    if (!fp) return ifos_set_error("could not open input file '%s'", path), -2;
+    struct ifos_json *j = (struct ifos_json *)calloc(1, sizeof *j);
+    char line[LINE_MAX_CHARS];
+    int lineno = 0;
+    while (fgets(line, sizeof line, fp)) {
+        lineno++;
+        /* an object line has a colon and double quotes and is no comment
+         * (src/json_parser.c:59-60) */
+        if (!(strchr(line, ':') && strchr(line, '"')) || strstr(line, "comment") || strstr(line, "Comment")) continue;
+        if (count_char(line, '"') != 4) {
+            fclose(fp);
+            ifos_json_free(j);
+            return ifos_set_error("%s:%d: an object line needs two quoted strings", path, lineno), -2;
+        }
+        const char *q1 = strchr(line, '"'), *q2 = strchr(q1 + 1, '"');
+        const char *q3 = strchr(q2 + 1, '"'), *q4 = strchr(q3 + 1, '"');
+        const char *colon = strchr(q2 + 1, ':');
+        if (!colon || colon > q3) {
+            fclose(fp);
+            ifos_json_free(j);
+            return ifos_set_error("%s:%d: cannot read object name and object value", path, lineno), -2;
+        }
+        char *names[5] = {0}, *values[5] = {0};
+        const int nn = split_commas(q1 + 1, (size_t)(q2 - q1 - 1), names, 5);
+        const int nv = split_commas(q3 + 1, (size_t)(q4 - q3 - 1), values, 5);
+        if (nn < 1 || nn != nv) {
+            for (int k = 0; k < 5; k++) {
+                free(names[k]);
+                free(values[k]);
+            }
+            fclose(fp);
+            ifos_json_free(j);
+            return ifos_set_error("%s:%d: %d object names but %d values", path, lineno, nn, nv), -2;
+        }
+        for (int k = 0; k < nn; k++) json_add(j, names[k], values[k]);
+    }
+    fclose(fp);
+    *out = j;
+    return 0;
+}
+
+void ifos_json_free(ifos_json *j)
+{
+    if (!j) return;
+    for (int k = 0; k < j->n; k++) {
+        free(j->name[k]);
+        free(j->value[k]);
+    }
+    free(j->name);
+    free(j->value);
+    free(j);
+}
+
+int ifos_json_count(const ifos_json *j) { return j->n; }
+const char *ifos_json_name(const ifos_json *j, int k) { return k >= 0 && k < j->n ? j->name[k] : NULL; }
+const char *ifos_json_value(const ifos_json *j, int k) { return k >= 0 && k < j->n ? j->value[k] : NULL; }
+
+const char *ifos_json_get(const ifos_json *j, const char *name)
+{
+    for (int k = 0; k < j->n; k++)
+        if (strcmp(j->name[k], name) == 0) return j->value[k];
+    return NULL;
+}
+
+int ifos_json_float(const ifos_json *j, const char *name, float *out)
+{
+    const char *v = ifos_json_get(j, name);
+    if (!v) return 1;
+    char *end = NULL;
+    const double d = strtod(v, &end);
+    if (!*v || end == v || *end) return ifos_set_error("value of %s is not a number: '%s'", name, v), -2;
+    *out = (float)d;
+    return 0;
+}
+
+int ifos_json_int(const ifos_json *j, const char *name, int *out)
+{
+    const char *v = ifos_json_get(j, name);
+    if (!v) return 1;
+    char *end = NULL;
+    const double d = strtod(v, &end);
+    double ip;
+    if (!*v || end == v || *end || modf(d, &ip) != 0.0)
+        return ifos_set_error("value of %s is not an integer: '%s'", name, v), -2;
+    *out = (int)d;
+    return 0;
+}
+
+int ifos_config_from_json(const char *path, ifos_config *c)
+{
+    ifos_json *j = NULL;
+    int rc = ifos_json_load(path, &j);
+    if (rc) return rc;
+    struct { const char *k; int *p; int required; } ints[] = {
+        {"NX", &c->NX, 1}, {"NY", &c->NY, 1}, {"FDORDER", &c->FDORDER, 1}, {"MAXRELERROR", &c->MAXRELERROR, 0},
+        {"FW", &c->FW, 0}, {"FREE_SURF", &c->FREE_SURF, 0}, {"BOUNDARY", &c->BOUNDARY, 0}, {"L", &c->L, 0},
+        {"VELOCITY", &c->VELOCITY, 0}, {"ADJOINT_TYPE", &c->ADJOINT_TYPE, 0},
+        {"PARAMETERIZATION", &c->PARAMETERIZATION, 0}, {"DTINV", &c->DTINV, 0},
+        {"SOURCE_SHAPE", &c->SOURCE_SHAPE, 0}, {"SOURCE_TYPE", &c->SOURCE_TYPE, 0}, {"TRKILL", &c->TRKILL, 0}};
+    struct { const char *k; float *p; int required; } flts[] = {
+        {"DH", &c->DH, 1}, {"DT", &c->DT, 1}, {"VPPML", &c->VPPML, 0}, {"FPML", &c->FPML, 0},
+        {"npower", &c->npower, 0}, {"k_max_PML", &c->k_max_PML, 0}, {"TAU", &c->TAU, 0}, {"F_REF", &c->F_REF, 0}};
+    for (size_t k = 0; k < sizeof ints / sizeof ints[0]; k++) {
+        const int r = ifos_json_int(j, ints[k].k, ints[k].p);
+        if (r < 0 || (r == 1 && ints[k].required)) {
+            if (r == 1) ifos_set_error("variable %s could not be retrieved from %s", ints[k].k, path);
+            ifos_json_free(j);
+            return -2;
+        }
+    }
+    for (size_t k = 0; k < sizeof flts / sizeof flts[0]; k++) {
+        const int r = ifos_json_float(j, flts[k].k, flts[k].p);
+        if (r < 0 || (r == 1 && flts[k].required)) {
+            if (r == 1) ifos_set_error("variable %s could not be retrieved from %s", flts[k].k, path);
+            ifos_json_free(j);
+            return -2;
+        }
+    }
+    float TIME = 0.0f;
+    rc = ifos_json_float(j, "TIME", &TIME);
+    if (rc) {
+        if (rc == 1) ifos_set_error("variable TIME could not be retrieved from %s", path);
+        ifos_json_free(j);
+        return -2;
+    }
+    c->NT = (int)floor(TIME / c->DT + 0.5);  /* iround, src/fd.h:15 */
+    /* FL1..FL_L of the relaxation mechanisms (src/read_par_json.c:387-412) */
+    static const char *fl[5] = {"FL1", "FL2", "FL3", "FL4", "FL5"};
+    for (int l = 0; l < c->L && l < 5; l++) {
+        rc = ifos_json_float(j, fl[l], &c->FL[l + 1]);
+        if (rc) {
+            if (rc == 1) ifos_set_error("variable %s could not be retrieved from %s", fl[l], path);
+            ifos_json_free(j);
+            return -2;
+        }
+    }
+    ifos_json_free(j);
+    return 0;
+}
+
+int ifos_model_read(const char *path, int NX, int NY, float *m)
+{
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return ifos_set_error("could not open model file '%s'", path), -2;
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+            if (fread(&m[(size_t)j * NX + i], sizeof(float), 1, fp) != 1) {
+                fclose(fp);
+                return ifos_set_error("model file '%s' ends before %d x %d values", path, NX, NY), -2;
+            }
+    fclose(fp);
+    return 0;
+}
+
+int ifos_model_write(const char *path, int NX, int NY, const float *m)
+{
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return ifos_set_error("could not open '%s' for writing", path), -2;
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++) fwrite(&m[(size_t)j * NX + i], sizeof(float), 1, fp);
+    const int bad = ferror(fp);
+    fclose(fp);
+    return bad ? (ifos_set_error("write to '%s' failed", path), -1) : 0;
+}
+
+/* SEG-Y trace header byte offsets (the SU segy.h layout) */
+enum {
+    SU_TRACL = 0, SU_CDP = 20, SU_TRID = 28, SU_OFFSET = 36, SU_GELEV = 40, SU_SDEPTH = 48, SU_SWDEP = 60,
+    SU_SCALEL = 68, SU_SCALCO = 70, SU_SX = 72, SU_GX = 80, SU_NS = 114, SU_DT = 116, SU_D1 = 180, SU_HDR = 240
+};
+
+static void put_i32(unsigned char *h, int off, int32_t v) { memcpy(h + off, &v, 4); }
+static void put_i16(unsigned char *h, int off, int16_t v) { memcpy(h + off, &v, 2); }
+static int iround_d(double x) { return (int)floor(x + 0.5); }
+
+int ifos_su_write(const char *path, int ntr, int ns, const float *data, const int *recpos, float DH, float DT,
+                  int ndt, float xs, float ys, float refrec_x, float refrec_y)
+{
+    if (ns < 1 || ns > 65535) return ifos_set_error("SU traces hold 1..65535 samples (got %d)", ns), -2;
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return ifos_set_error("could not open '%s' for writing", path), -2;
+    const float xshift = 800.0f, yshift = 800.0f;  /* src/outseis_glob.c:33 */
+    const double PI = 3.1415926535897932384626433832795;
+    for (int t = 0; t < ntr; t++) {
+        unsigned char h[SU_HDR];
+        memset(h, 0, sizeof h);
+        const float xr = recpos[2 * t] * DH, yr = recpos[2 * t + 1] * DH;
+        const float x = xr - refrec_x, y = yr - refrec_y;
+        put_i32(h, SU_TRACL, t + 1);
+        put_i32(h, SU_CDP, t + 1);
+        put_i16(h, SU_TRID, 1);
+        put_i32(h, SU_OFFSET, iround_d(sqrt((xs - xr) * (xs - xr) + (ys - yr) * (ys - yr)) * 1000.0));
+        put_i32(h, SU_GELEV, iround_d(yr * 1000.0));
+        put_i32(h, SU_SDEPTH, iround_d(ys * 1000.0));
+        put_i32(h, SU_SWDEP, iround_d(((360.0 / (2.0 * PI)) * atan2(x - xshift, y - yshift)) * 1000.0));
+        put_i16(h, SU_SCALEL, -1000);
+        put_i16(h, SU_SCALCO, -1000);
+        put_i32(h, SU_SX, iround_d(xs * 1000.0));
+        put_i32(h, SU_GX, iround_d(xr * 1000.0));
+        const uint16_t nsu = (uint16_t)ns, dtu = (uint16_t)iround_d(((float)ndt * DT) * 1.0e6);
+        memcpy(h + SU_NS, &nsu, 2);
+        memcpy(h + SU_DT, &dtu, 2);
+        const float d1 = (float)dtu * 1.0e-6;
+        memcpy(h + SU_D1, &d1, 4);
+        fwrite(h, 1, sizeof h, fp);
+        fwrite(data + (size_t)t * ns, sizeof(float), (size_t)ns, fp);
+    }
+    const int bad = ferror(fp);
+    fclose(fp);
+    return bad ? (ifos_set_error("write to '%s' failed", path), -1) : 0;
+}
+
+int ifos_su_read(const char *path, int max_tr, int ns, float *data, int *ntr_out)
+{
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return ifos_set_error("could not open SU file '%s'", path), -2;
+    int t = 0;
+    unsigned char h[SU_HDR];
+    while (fread(h, 1, sizeof h, fp) == sizeof h) {
+        uint16_t nsu;
+        memcpy(&nsu, h + SU_NS, 2);
+        if ((int)nsu != ns) {
+            fclose(fp);
+            return ifos_set_error("'%s' trace %d has %d samples, expected %d", path, t + 1, (int)nsu, ns), -2;
+        }
+        if (t == max_tr) {
+            fclose(fp);
+            return ifos_set_error("'%s' holds more than %d traces", path, max_tr), -2;
+        }
+        if (fread(data + (size_t)t * ns, sizeof(float), (size_t)ns, fp) != (size_t)ns) {
+            fclose(fp);
+            return ifos_set_error("'%s' ends inside trace %d", path, t + 1), -2;
+        }
+        t++;
+    }
+    fclose(fp);
+    *ntr_out = t;
+    return 0;
+}

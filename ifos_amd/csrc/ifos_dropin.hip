@@ -40,7 +40,7 @@ void launch_inject_serial(const PointArgs &a, const StepGeom &g, float *vx, floa
                           const float *rjp, int sw, int ADJOINT_TYPE, const int *nt_dev, int nt, int *nt_tick,
                           hipStream_t s);
 }
-void ifos_set_error(const char *fmt, ...);
+extern "C" void ifos_set_error(const char *fmt, ...);
 
 // ---------------------------------------------------------------------------
 // reference globals read by the kernels (src/globvar.h)

@@ -37,7 +37,7 @@ using namespace ifos;
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
 
-void ifos_set_error(const char *fmt, ...)
+extern "C" void ifos_set_error(const char *fmt, ...)
 {
     char buf[1024];
     va_list ap;
@@ -629,8 +629,8 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
     while (unit < 32 && NT % (unit * 2) == 0) unit *= 2;
     const int kind = sw;
     const int key[6] = {E->nsrc, E->ntr, snapshots ? 1 : 0, E->nslots, E->shape.spb * 65536 + E->shape.grid, unit};
-    StepArgs a = step_args(E, sw, snapshots);
     const LaunchShape ls = E->shape;
+    StepArgs a = shaped_args(step_args(E, sw, snapshots), ls);
     const StepArgs *da = nullptr;
     if (E->use_graphs) {
         ENG_TRY(upload_args(E, a, sw == 1 ? 2 : (snapshots ? 1 : 0), &da));
@@ -866,6 +866,28 @@ extern "C" int ifos_engine_set_snapshot_precision(ifos_engine *E, int prec)
     return 0;
 }
 
+// device bytes an engine of `slots` slots allocates (the sizes of
+// engine_create and set_shots; snapshots when `snapshots` != 0)
+extern "C" size_t ifos_engine_bytes(const ifos_config *c, int slots, int snap_prec, int snapshots, int ntr_total,
+                                    int nsrc_total)
+{
+    if (validate(c) || slots < 1) return 0;
+    const int N = c->FDORDER / 2, H = N + 1, A = 16;
+    const size_t pitch = (size_t)((A + c->NX + H) + 63) / 64 * 64;
+    const size_t field = pitch * (size_t)(c->NY + 2 * H + 1);
+    const size_t plane = (size_t)c->NX * c->NY;
+    const int fw2 = c->FW > 0 ? 2 * c->FW : 1;
+    size_t b = sizeof(float) * field * (size_t)(10 + 3 * c->L) * slots;   // wavefields
+    b += sizeof(float) * field * (size_t)(15 + 3 * c->L);                 // materials
+    b += sizeof(float) * plane * (11 + 3 * (size_t)slots);                // interior + correlation
+    b += sizeof(float) * (size_t)(5 * c->NY * fw2 + 4 * fw2 * c->NX) * slots;  // CPML memory
+    b += sizeof(float) * (size_t)6 * ntr_total * c->NT;                   // seismograms etc.
+    b += (sizeof(float) * c->NT + 3 * sizeof(int) + 2 * sizeof(double)) * (size_t)nsrc_total;
+    b += (3 * sizeof(int) + 2 * sizeof(double)) * (size_t)ntr_total;
+    if (snapshots) b += snapshot_bytes(*c, snap_prec) * slots + 256;
+    return b;
+}
+
 extern "C" size_t ifos_snapshot_bytes(const ifos_config *c, int prec)
 {
     if (validate(c) || prec < 0 || prec > 2) return 0;
@@ -961,7 +983,7 @@ extern "C" int ifos_engine_time_kernel(ifos_engine *E, int kernel, int reps, dou
 {
     HIP_TRY(hipSetDevice(E->device));
     if (!E->tiles || !E->model_set) return ifos_set_error("time_kernel needs set_model and set_shot"), -2;
-    StepArgs a = step_args(E, 0, false);
+    StepArgs a = shaped_args(step_args(E, 0, false), E->shape);
     a.base = nullptr;
     a.nan_flag = nullptr;
     const StepArgs *da = nullptr;
@@ -998,11 +1020,10 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
     if (reps < 1) reps = 1;
     int ncu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, E->device));
-    StepArgs a = step_args(E, 0, false);
-    a.base = nullptr;
-    a.nan_flag = nullptr;
+    StepArgs a0 = step_args(E, 0, false);
+    a0.base = nullptr;
+    a0.nan_flag = nullptr;
     const StepArgs *da = nullptr;
-    ENG_TRY(upload_args(E, a, 3, &da));
     hipEvent_t t0, t1;
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
@@ -1014,6 +1035,8 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
             LaunchShape ls;
             ls.spb = spb;
             ls.grid = per_cu ? (per_cu * ncu + 7) / 8 * 8 : 0;
+            const StepArgs a = shaped_args(a0, ls);
+            ENG_TRY(upload_args(E, a, 3, &da));
             launch_vel_step(a, da, ls, 2, E->stream);
             launch_str_step(a, da, ls, 2, E->stream);
             HIP_TRY(hipEventRecord(t0, E->stream));

@@ -20,6 +20,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// message for ifos_last_error() (defined in ifos_engine.hip)
+extern "C" void ifos_set_error(const char *fmt, ...);
+
 namespace ifos {
 
 constexpr int kMaxL = 8;

@@ -64,7 +64,11 @@ struct StepArgs {
     int *nan_flag;
     int nbx, nby;               // tiles in x and y
     int nslots;                 // active shot slots
-    int swizzle;                // 1: XCD-contiguous work order (see item_of)
+    int swizzle;                // 1: XCD-contiguous work order (see work_range)
+    int ngroups;                // slot groups per tile (set by the launcher)
+    // division by nbx and by ngroups as multiply-high and shift (set by the
+    // launcher): the quotients of every work item without integer division
+    uint32_t nbx_mul, nbx_sh, ng_mul, ng_sh;
 };
 
 // launch shape of the step kernels: spb slots per workgroup (1, 2 or 4; the
@@ -82,5 +86,8 @@ struct LaunchShape {
 void launch_vel_step(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s);
 void launch_str_step(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s);
 void launch_tick(int *base, int by, hipStream_t s);
+// the argument block as the kernels of launch shape ls must see it (slot
+// groups and the division constants filled in): upload this one
+StepArgs shaped_args(const StepArgs &a, const LaunchShape &ls);
 
 }  // namespace ifos

@@ -38,6 +38,15 @@ __device__ __forceinline__ void xcd_chunk(int nitems, int x, int &lo, int &hi)
     hi = lo + per + (x < rem ? 1 : 0);
 }
 
+// n / d for 0 <= n < 2^31 from (mul, sh) of make_div: t = mulhi(n, mul),
+// q = (((n - t) >> 1) + t) >> sh; sh = 0 with mul = 0 marks d = 1
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t mul, uint32_t sh)
+{
+    if (mul == 0) return n;
+    const uint32_t t = __umulhi(n, mul);
+    return (((n - t) >> 1) + t) >> sh;
+}
+
 // items [first, end) with the given stride for this workgroup: one item per
 // workgroup, or a persistent grid (gridDim.x a multiple of 8) looping over
 // the items of its XCD's run
@@ -214,13 +223,13 @@ __global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ p
     const StepArgs &a = *pa;
     __shared__ float s_red[2][kTileY];
     const StepGeom &g = a.g;
-    const int ngroups = (a.nslots + SPB - 1) / SPB;
+    const int ngroups = a.ngroups;
     const int nt = step_number(a, k);
     int first, end, stride;
     work_range(a, a.nbx * a.nby * ngroups, persistent != 0, first, end, stride);
     for (int it = first; it < end; it += stride) {
-        const int blk = it / ngroups, s0 = (it % ngroups) * SPB;
-        const int bx = blk % a.nbx, by = blk / a.nbx;
+        const int blk = (int)fdiv(it, a.ng_mul, a.ng_sh), s0 = (it - blk * ngroups) * SPB;
+        const int by = (int)fdiv(blk, a.nbx_mul, a.nbx_sh), bx = blk - by * a.nbx;
         const int i = bx * kTileX + threadIdx.x + 1;
         const int j = by * kTileY + threadIdx.y + 1;
         const bool live = i <= g.NX && j <= g.NY;
@@ -370,13 +379,13 @@ __global__ __launch_bounds__(512) void k_str_step(const StepArgs *__restrict__ p
     __shared__ float s_sxy[kTileY][kTileX];
     __shared__ float s_red[3][kTileY];
     const StepGeom &g = a.g;
-    const int ngroups = (a.nslots + SPB - 1) / SPB;
+    const int ngroups = a.ngroups;
     const int nt = step_number(a, k);
     int first, end, stride;
     work_range(a, a.nbx * a.nby * ngroups, persistent != 0, first, end, stride);
     for (int it = first; it < end; it += stride) {
-        const int blk = it / ngroups, s0 = (it % ngroups) * SPB;
-        const int bx = blk % a.nbx, by = blk / a.nbx;
+        const int blk = (int)fdiv(it, a.ng_mul, a.ng_sh), s0 = (it - blk * ngroups) * SPB;
+        const int by = (int)fdiv(blk, a.nbx_mul, a.nbx_sh), bx = blk - by * a.nbx;
         const int i = bx * kTileX + threadIdx.x + 1;
         const int j = by * kTileY + threadIdx.y + 1;
         const bool live = i <= g.NX && j <= g.NY;
@@ -407,6 +416,32 @@ __global__ void k_tick(int *base, int by)
 }
 
 // ---------------------------------------------------------------------------
+// multiplier and shift of fdiv for divisor d >= 1 (the round-up method for
+// unsigned division by an invariant: l = ceil(log2 d),
+// mul = floor(2^32 (2^l - d) / d) + 1, shift l - 1)
+static void make_div(uint32_t d, uint32_t &mul, uint32_t &sh)
+{
+    if (d <= 1) {
+        mul = 0;
+        sh = 0;
+        return;
+    }
+    uint32_t l = 0;
+    while ((1ull << l) < d) l++;
+    mul = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+    sh = l - 1;
+}
+
+// the launcher's copy of the argument block with the per-shape fields set
+StepArgs shaped_args(const StepArgs &a0, const LaunchShape &ls)
+{
+    StepArgs a = a0;
+    a.ngroups = (a.nslots + ls.spb - 1) / ls.spb;
+    make_div((uint32_t)a.nbx, a.nbx_mul, a.nbx_sh);
+    make_div((uint32_t)a.ngroups, a.ng_mul, a.ng_sh);
+    return a;
+}
+
 static int grid_of(const StepArgs &a, const LaunchShape &ls)
 {
     const int nitems = a.nbx * a.nby * ((a.nslots + ls.spb - 1) / ls.spb);

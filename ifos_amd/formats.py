@@ -1,0 +1,76 @@
+"""ctypes mirror of include/ifos2d_formats.h: the reference's parameter
+(quasi-JSON), model (binary) and seismogram (SU) files."""
+import ctypes as C
+
+import numpy as np
+
+from . import Config, _check, lib
+
+
+def _L():
+    L = lib()
+    L.ifos_json_load.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+    L.ifos_json_free.argtypes = [C.c_void_p]
+    L.ifos_json_free.restype = None
+    L.ifos_json_count.argtypes = [C.c_void_p]
+    L.ifos_json_get.argtypes = [C.c_void_p, C.c_char_p]
+    L.ifos_json_get.restype = C.c_char_p
+    L.ifos_config_from_json.argtypes = [C.c_char_p, C.POINTER(Config)]
+    L.ifos_model_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    L.ifos_model_write.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    L.ifos_su_write.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int), C.c_float,
+                                C.c_float, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.ifos_su_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int)]
+    return L
+
+
+def read_json(path):
+    """All objects of a parameter file, in file order (name, value)."""
+    L = _L()
+    h = C.c_void_p()
+    _check(L.ifos_json_load(path.encode(), C.byref(h)))
+    try:
+        return {k.decode(): v.decode() for k, v in _pairs(L, h)}
+    finally:
+        L.ifos_json_free(h)
+
+
+def _pairs(L, h):
+    # the C side exposes lookup by name; walk the file order through the names it holds
+    n = L.ifos_json_count(h)
+    L.ifos_json_name.argtypes = [C.c_void_p, C.c_int]
+    L.ifos_json_name.restype = C.c_char_p
+    L.ifos_json_value.argtypes = [C.c_void_p, C.c_int]
+    L.ifos_json_value.restype = C.c_char_p
+    return [(L.ifos_json_name(h, k), L.ifos_json_value(h, k)) for k in range(n)]
+
+
+def config_from_json(path, base=None):
+    c = Config() if base is None else base
+    _check(_L().ifos_config_from_json(path.encode(), C.byref(c)))
+    return c
+
+
+def read_model(path, NX, NY):
+    m = np.zeros((NY, NX), np.float32)
+    _check(_L().ifos_model_read(path.encode(), NX, NY, m.ctypes.data_as(C.POINTER(C.c_float))))
+    return m
+
+
+def write_model(path, m):
+    m = np.ascontiguousarray(m, np.float32)
+    _check(_L().ifos_model_write(path.encode(), m.shape[1], m.shape[0], m.ctypes.data_as(C.POINTER(C.c_float))))
+
+
+def write_su(path, data, recpos, DH, DT, xs, ys, ndt=1, refrec=(0.0, 0.0)):
+    data = np.ascontiguousarray(data, np.float32)
+    rp = np.ascontiguousarray(recpos, np.int32)
+    _check(_L().ifos_su_write(path.encode(), data.shape[0], data.shape[1], data.ctypes.data_as(C.POINTER(C.c_float)),
+                              rp.ctypes.data_as(C.POINTER(C.c_int)), DH, DT, ndt, xs, ys, refrec[0], refrec[1]))
+
+
+def read_su(path, ns, max_tr=100000):
+    buf = np.zeros((max_tr, ns), np.float32)
+    n = C.c_int()
+    _check(_L().ifos_su_read(path.encode(), max_tr, ns, buf.ctypes.data_as(C.POINTER(C.c_float)), C.byref(n)))
+    return buf[:n.value].copy()

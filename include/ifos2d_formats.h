@@ -1,0 +1,67 @@
+/*
+ * ifos2d_formats.h -- the reference's file formats on the P-SV path, so a
+ * run of this library reads and writes what an IFOS2D run does:
+ *   - the quasi-JSON parameter file (src/json_parser.c, src/read_par_json.c),
+ *   - binary model files (src/readmod_elastic.c, src/writemod.c),
+ *   - SU seismograms (src/outseis_glob.c, SEIS_FORMAT 1) and their reading
+ *     back as observed data (src/inseis.c).
+ * Host-side C, plain pointers.  0 on success, negative on failure with the
+ * message in ifos_last_error().
+ */
+#ifndef IFOS2D_FORMATS_H
+#define IFOS2D_FORMATS_H
+
+#include "ifos2d_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ifos_json ifos_json;
+
+/* replaces read_objects_from_intputfile() src/json_parser.c:26: the
+ * "name" : "value" objects of the file, up to five per line
+ * ("A, B" : "1, 2"), lines naming "comment"/"Comment" skipped, names and
+ * values cut to their first blank-delimited word. */
+int  ifos_json_load(const char *path, ifos_json **out);
+void ifos_json_free(ifos_json *j);
+int  ifos_json_count(const ifos_json *j);
+/* name and value of object k (file order), or NULL */
+const char *ifos_json_name(const ifos_json *j, int k);
+const char *ifos_json_value(const ifos_json *j, int k);
+/* the value of the first object called `name`, or NULL */
+const char *ifos_json_get(const ifos_json *j, const char *name);
+/* numeric values as get_float_from_objectlist / get_int_from_objectlist
+ * (src/json_parser.c:264-385) read them: 0 found, 1 absent, -2 malformed */
+int  ifos_json_float(const ifos_json *j, const char *name, float *out);
+int  ifos_json_int(const ifos_json *j, const char *name, int *out);
+
+/* The run parameters of the P-SV path from a parameter file, as
+ * read_par_json (src/read_par_json.c) sets the globals: NX, NY, DH, DT,
+ * NT = iround(TIME/DT) (src/IFOS2D.c:257), FDORDER, MAXRELERROR, FW,
+ * FREE_SURF, BOUNDARY, VPPML, FPML, npower, k_max_PML, L, FL1..FL5, TAU,
+ * F_REF, VELOCITY, ADJOINT_TYPE, PARAMETERIZATION, DTINV, SOURCE_SHAPE,
+ * SOURCE_TYPE, TRKILL.  Absent optional keys keep the values *c holds. */
+int ifos_config_from_json(const char *path, ifos_config *c);
+
+/* Binary model files: NX*NY native floats written x-outer, y-inner
+ * (src/writemod.c:36-41); arrays here are [NY][NX], x fastest. */
+int ifos_model_read(const char *path, int NX, int NY, float *m);
+int ifos_model_write(const char *path, int NX, int NY, const float *m);
+
+/* SU seismograms as outseis_glob writes them for one shot (SEIS_FORMAT 1):
+ * per trace a 240-byte SEG-Y trace header (tracl, cdp, trid, offset,
+ * gelev, sdepth, swdep, scalel/scalco = -1000, sx, gx, ns, dt, d1 as there)
+ * followed by ns floats.  data [ntr][ns]; recpos [ntr][2] grid indices;
+ * xs, ys: source position in metres; ndt: the output sample stride (NDT). */
+int ifos_su_write(const char *path, int ntr, int ns, const float *data, const int *recpos, float DH, float DT,
+                  int ndt, float xs, float ys, float refrec_x, float refrec_y);
+/* Read an SU file's traces: ntr_out traces of ns samples each into data
+ * [max_tr][ns] (as inseis reads observed data); fails if the file's
+ * sample count differs from ns. */
+int ifos_su_read(const char *path, int max_tr, int ns, float *data, int *ntr_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -151,6 +151,12 @@ int  ifos_engine_set_snapshot_precision(ifos_engine *e, int prec);
 /* Device bytes of the snapshot storage of one shot at that precision
  * (0 for an invalid configuration). */
 size_t ifos_snapshot_bytes(const ifos_config *c, int prec);
+/* Device bytes an engine with `slots` slots holds for shots with ntr_total
+ * receivers and nsrc_total sources in all, with snapshot storage at that
+ * precision when snapshots != 0 (0 for an invalid configuration); the
+ * admission budget of the request server (ifos2d_serve.h) counts this. */
+size_t ifos_engine_bytes(const ifos_config *c, int slots, int snap_prec, int snapshots, int ntr_total,
+                         int nsrc_total);
 /* Device memory helpers for callers without a GPU runtime of their own. */
 void *ifos_dev_alloc(size_t bytes);
 int  ifos_dev_free(void *p);

@@ -1,0 +1,76 @@
+/*
+ * ifos2d_opt.h -- C ABI of the model-update side of an FWI iteration on the
+ * device: reductions, PCG direction, step-length parabola, bounded model
+ * update and L-BFGS.  All array arguments are device pointers (interior
+ * [NY][NX] arrays, x fastest); scalars come back to the host.
+ *
+ * Sums run in FP64 as deterministic two-pass reductions (the reference sums
+ * in FP32); the element-wise expressions are the reference's FP32 ones.
+ * Errors: 0 on success, negative on failure, message in ifos_last_error().
+ */
+#ifndef IFOS2D_OPT_H
+#define IFOS2D_OPT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ifos_opt ifos_opt;
+
+/* Parameters of calc_mat_change_test (src/calc_mat_change_test.c:26 and the
+ * globals it reads): the integers, then the floats, in this order. */
+typedef struct ifos_update_params {
+    int iter, INV_VP_ITER, INV_VS_ITER, INV_RHO_ITER, S, GRAD_METHOD, LBFGS_STEP_LENGTH, LBFGS_iter_start;
+    float eps_scale, VPUPPERLIM, VPLOWERLIM, VSUPPERLIM, VSLOWERLIM, RHOUPPERLIM, RHOLOWERLIM, VP_VS_RATIO;
+    float S_VP, S_VS, S_RHO;
+} ifos_update_params;
+
+/* Reduction scratch and a stream on one device. */
+int  ifos_opt_create(int device, ifos_opt **out);
+void ifos_opt_destroy(ifos_opt *o);
+
+/* FP64 sum of a[i]*b[i] (replaces the FP32 sums of src/PCG.c:170-174 and
+ * src/LBFGS.c:247-250, 279-282, 308-310). */
+int ifos_opt_dot(ifos_opt *o, const float *a, const float *b, long n, double *out);
+/* Largest a[i] (absolute = 0) or |a[i]| (absolute = 1), exact
+ * (src/calc_mat_change_test.c:85-150). */
+int ifos_opt_max(ifos_opt *o, const float *a, long n, int absolute, float *out);
+
+/* replaces PCG() src/PCG.c:27 for one parameter (tapers, filters and the
+ * jacobian files aside): dir = C*grad, conjugated with the Polak-Ribiere
+ * beta (FP64 sums, clipped at 0) against p_last (iteration 2) or c_last
+ * (later iterations) exactly when the reference conjugates.  p_last and
+ * c_last hold the previous preconditioned gradient and direction and are
+ * updated in place as the reference's _p.old / _c.old files are. */
+int ifos_opt_pcg(ifos_opt *o, long n, const float *grad, float C, float *p_last, float *c_last, int iter,
+                 int pcg_iter_start, float *dir, double *beta);
+
+/* replaces calc_opt_step() src/calc_opt_step.c:29 (host arrays: the
+ * reference's L2t[1..3], epst1[1..3]): the parabola through the three
+ * points, solved in FP64 on the device, with the same clamps. */
+int ifos_opt_step_length(ifos_opt *o, const float *L2t, const float *epst, float *opteps);
+
+/* replaces calc_mat_change_test() src/calc_mat_change_test.c:26,
+ * PARAMETERIZATION 1: models vp, vs, rho minus the scaled directions,
+ * bounded as there; vp0/vs0/rho0 are the starting models of the S == 1
+ * limits.  eps[3] (host) receives EPSILON, EPSILON_u, EPSILON_rho. */
+int ifos_opt_model_update(ifos_opt *o, long n, const ifos_update_params *u, const float *vp, const float *vs,
+                          const float *rho, const float *g_vp, const float *g_vs, const float *g_rho,
+                          const float *vp0, const float *vs0, const float *rho0, float *vp1, float *vs1,
+                          float *rho1, float *eps);
+
+/* L-BFGS memory: s_hist, y_hist [N_LBFGS][npar*ncell] (device), rho_hist
+ * [N_LBFGS] (host, FP64).  Parameters in the reference's order vs, rho, vp.
+ * replaces the s_LBFGS store of src/calc_mat_change_test.c:297-316:
+ * s_w = (m1 - m0) / avg per parameter, w = iter mod N (N for 0). */
+int ifos_opt_lbfgs_store_s(ifos_opt *o, long ncell, int npar, const float *const *m1, const float *const *m0,
+                           const float *avg, float *s_hist, int N_LBFGS, int iter);
+/* replaces lbfgs() src/LBFGS.c:28: grad[npar] (device, in/out) become the
+ * L-BFGS direction; work holds 2*npar*ncell floats. */
+int ifos_opt_lbfgs(ifos_opt *o, long ncell, int npar, float *const *grad, const float *avg, float *s_hist,
+                   float *y_hist, double *rho_hist, float *work, int N_LBFGS, int iteration, int lbfgs_iter_start);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -377,3 +377,112 @@ int ref_gradient(const int *ip, const float *fp, const float *FLv, const float *
     for (int k = 0; k < 5; k++) free(snap[k]);
     return 0;
 }
+
+/* ---------------------------------------------------------------------------
+ * Optimizer routines of the reference, called unmodified: calc_opt_step
+ * (+ solvelin), lbfgs_core and calc_mat_change_test in its test-step mode
+ * (itest = 1: computes the updated models, writes no files).
+ * ------------------------------------------------------------------------- */
+static FILE *harness_null = NULL;
+
+static void harness_setup_opt(int nx, int ny)
+{
+    int ip[16] = {0};
+    float fp[8] = {0};
+    ip[0] = nx; ip[1] = ny; ip[2] = 2; ip[10] = 1; ip[12] = 1;
+    harness_setup(ip, fp);
+    if (!harness_null) harness_null = fopen("/dev/null", "w");
+    FP = harness_null;
+    if (!FL) FL = vector(1, 8);
+}
+
+/* L2t[0..2], epst[0..2] are the reference's L2t[1..3], epst1[1..3] */
+float ref_calc_opt_step(const float *L2t3, const float *epst3)
+{
+    harness_setup_opt(1, 1);
+    float *L2t = vector(1, 4), *epst = vector(1, 3);
+    for (int k = 1; k <= 3; k++) {
+        L2t[k] = L2t3[k - 1];
+        epst[k] = epst3[k - 1];
+    }
+    L2t[4] = 0.0f;
+    float e = calc_opt_step(L2t, NULL, NULL, epst, 1, 1.0f);
+    free_vector(L2t, 1, 4);
+    free_vector(epst, 1, 3);
+    return e;
+}
+
+/* s, y: [N_LBFGS][n] (row w-1 = the reference's vector w), rho [N_LBFGS],
+ * q, r: [n]; alpha [N_LBFGS].  q is modified as in the reference. */
+void ref_lbfgs_core(int iteration, int N_LBFGS, int n, float *s, float *y, float *rho, float *q, float *alpha,
+                    float *r)
+{
+    harness_setup_opt(1, 1);
+    float **S = matrix(1, N_LBFGS, 1, n), **Y = matrix(1, N_LBFGS, 1, n);
+    float *R = vector(1, N_LBFGS), *Q = vector(1, n), *A = vector(1, N_LBFGS), *Rr = vector(1, n);
+    for (int w = 1; w <= N_LBFGS; w++) {
+        R[w] = rho[w - 1];
+        A[w] = alpha[w - 1];
+        for (int l = 1; l <= n; l++) {
+            S[w][l] = s[(size_t)(w - 1) * n + l - 1];
+            Y[w][l] = y[(size_t)(w - 1) * n + l - 1];
+        }
+    }
+    for (int l = 1; l <= n; l++) Q[l] = q[l - 1];
+    lbfgs_core(iteration, N_LBFGS, n, S, Y, R, Q, A, Rr);
+    for (int w = 1; w <= N_LBFGS; w++) {
+        rho[w - 1] = R[w];
+        alpha[w - 1] = A[w];
+    }
+    for (int l = 1; l <= n; l++) {
+        q[l - 1] = Q[l];
+        r[l - 1] = Rr[l];
+    }
+    free_matrix(S, 1, N_LBFGS, 1, n);
+    free_matrix(Y, 1, N_LBFGS, 1, n);
+    free_vector(R, 1, N_LBFGS);
+    free_vector(Q, 1, n);
+    free_vector(A, 1, N_LBFGS);
+    free_vector(Rr, 1, n);
+}
+
+/* Test-step model update (PARAMETERIZATION 1, itest = 1).  Arrays are
+ * interior [NY][NX]: models vp, vs, rho, directions g_vp, g_vs, g_rho,
+ * starting models (for the S == 1 limits); outputs vp1, vs1, rho1 and
+ * eps[3] = EPSILON, EPSILON_u, EPSILON_rho.
+ * ip: iter, INV_VP_ITER, INV_VS_ITER, INV_RHO_ITER, S, GRAD_METHOD,
+ *     LBFGS_STEP_LENGTH, LBFGS_iter_start
+ * fp: eps_scale, VPUPPERLIM, VPLOWERLIM, VSUPPERLIM, VSLOWERLIM, RHOUPPERLIM,
+ *     RHOLOWERLIM, VP_VS_RATIO, S_VP, S_VS, S_RHO */
+void ref_mat_change(int nx, int ny, const int *ip, const float *fp, const float *vp, const float *vs,
+                    const float *rho, const float *g_vp, const float *g_vs, const float *g_rho, const float *vp0,
+                    const float *vs0, const float *rho0, float *vp1, float *vs1, float *rho1, float *eps)
+{
+    harness_setup_opt(nx, ny);
+    INV_VP_ITER = ip[1]; INV_VS_ITER = ip[2]; INV_RHO_ITER = ip[3]; S = ip[4]; GRAD_METHOD = ip[5];
+    LBFGS_STEP_LENGTH = ip[6];
+    VPUPPERLIM = fp[1]; VPLOWERLIM = fp[2]; VSUPPERLIM = fp[3]; VSLOWERLIM = fp[4];
+    RHOUPPERLIM = fp[5]; RHOLOWERLIM = fp[6]; VP_VS_RATIO = fp[7]; S_VP = fp[8]; S_VS = fp[9]; S_RHO = fp[10];
+    float **a[12];
+    const float *src[12] = {g_vp, g_rho, g_vs, rho, NULL, vp, NULL, vs, NULL, vs0, vp0, rho0};
+    for (int k = 0; k < 12; k++) {
+        a[k] = matrix(1, ny, 1, nx);
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) a[k][j][i] = src[k] ? src[k][(size_t)(j - 1) * nx + i - 1] : 0.0f;
+    }
+    /* (waveconv, waveconv_rho, waveconv_u, rho, rhonp1, pi, pinp1, u, unp1, ...)
+     * as src/IFOS2D.c:3239 passes them */
+    calc_mat_change_test(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], ip[0], 1, 0, fp[0], 1, 1, a[9],
+                         a[10], a[11], 1, NULL, 5, 3, 1.0f, 1.0f, 1.0f, ip[7]);
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            const size_t e = (size_t)(j - 1) * nx + i - 1;
+            vp1[e] = a[6][j][i];
+            vs1[e] = a[8][j][i];
+            rho1[e] = a[4][j][i];
+        }
+    eps[0] = EPSILON;
+    eps[1] = EPSILON_u;
+    eps[2] = EPSILON_rho;
+    for (int k = 0; k < 12; k++) free_matrix(a[k], 1, ny, 1, nx);
+}

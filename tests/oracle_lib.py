@@ -216,3 +216,68 @@ def f16_to_32(h, prec):
     L.orc_16_to_f32.restype = C.c_float
     L.orc_16_to_f32.argtypes = [C.c_ushort, C.c_int]
     return np.array([L.orc_16_to_f32(int(v), prec) for v in np.asarray(h, np.uint16).ravel()], np.float32)
+
+
+# ---------------------------------------------------------------- optimizer --
+class UpdateParams(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("iter", "INV_VP_ITER", "INV_VS_ITER", "INV_RHO_ITER", "S", "GRAD_METHOD",
+                                        "LBFGS_STEP_LENGTH", "LBFGS_iter_start")]
+                + [(n, C.c_float) for n in ("eps_scale", "VPUPPERLIM", "VPLOWERLIM", "VSUPPERLIM", "VSLOWERLIM",
+                                            "RHOUPPERLIM", "RHOLOWERLIM", "VP_VS_RATIO", "S_VP", "S_VS", "S_RHO")])
+
+
+def update_params(**kw):
+    d = dict(iter=1, INV_VP_ITER=0, INV_VS_ITER=0, INV_RHO_ITER=0, S=0, GRAD_METHOD=1, LBFGS_STEP_LENGTH=1,
+             LBFGS_iter_start=1, eps_scale=0.01, VPUPPERLIM=1e9, VPLOWERLIM=0.0, VSUPPERLIM=1e9, VSLOWERLIM=0.0,
+             RHOUPPERLIM=1e9, RHOLOWERLIM=0.0, VP_VS_RATIO=0.0, S_VP=0.0, S_VS=0.0, S_RHO=0.0)
+    d.update(kw)
+    u = UpdateParams()
+    for k, v in d.items():
+        setattr(u, k, v)
+    return u
+
+
+def pcg(grad, C_, p_last, c_last, it, start, which="oracle"):
+    ny, nx = grad.shape
+    gp = np.zeros((ny, nx), np.float32)
+    d = np.zeros((ny, nx), np.float32)
+    L = oracle()
+    L.orc_pcg.restype = C.c_float
+    beta = L.orc_pcg(nx, ny, _ptr(_f32(grad)), C.c_float(C_), _ptr(_f32(p_last)), _ptr(_f32(c_last)), it, start,
+                     _ptr(gp), _ptr(d))
+    return beta, gp, d
+
+
+def opt_step(L2t, epst, which="oracle"):
+    a, b = _f32(np.asarray(L2t)), _f32(np.asarray(epst))
+    if which == "ref":
+        L = ref()
+        L.ref_calc_opt_step.restype = C.c_float
+        return L.ref_calc_opt_step(_ptr(a), _ptr(b))
+    L = oracle()
+    L.orc_opt_step.restype = C.c_float
+    return L.orc_opt_step(_ptr(a), _ptr(b))
+
+
+def mat_change(u, vp, vs, rho, g_vp, g_vs, g_rho, vp0, vs0, rho0, which="oracle"):
+    ny, nx = vp.shape
+    arrs = [_f32(a) for a in (vp, vs, rho, g_vp, g_vs, g_rho, vp0, vs0, rho0)]
+    out = [np.zeros((ny, nx), np.float32) for _ in range(3)]
+    eps = np.zeros(3, np.float32)
+    if which == "ref":
+        ip = (C.c_int * 8)(*[getattr(u, k) for k, _ in u._fields_[:8]])
+        fp = (C.c_float * 11)(*[getattr(u, k) for k, _ in u._fields_[8:]])
+        ref().ref_mat_change(nx, ny, ip, fp, *[_ptr(a) for a in arrs], *[_ptr(o) for o in out], _ptr(eps))
+    else:
+        oracle().orc_mat_change(nx, ny, C.byref(u), *[_ptr(a) for a in arrs], *[_ptr(o) for o in out], _ptr(eps))
+    return out[0], out[1], out[2], eps
+
+
+def lbfgs_core(iteration, s, y, rho, q, alpha, which="oracle"):
+    N, n = s.shape
+    s, y = _f32(s), _f32(y)
+    rho, q, alpha = _f32(rho).copy(), _f32(q).copy(), _f32(alpha).copy()
+    r = np.zeros(n, np.float32)
+    f = ref().ref_lbfgs_core if which == "ref" else oracle().orc_lbfgs_core
+    f(iteration, N, n, _ptr(s), _ptr(y), _ptr(rho), _ptr(q), _ptr(alpha), _ptr(r))
+    return r, rho, alpha, q

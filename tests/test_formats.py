@@ -1,0 +1,85 @@
+"""File formats of the reference (include/ifos2d_formats.h), host side:
+the quasi-JSON parameter file (the reference's own par/in_and_out/
+IFOS2D_FW.json, committed as tests/golden/IFOS2D_FW.json), binary model
+files and SU seismograms, checked against numpy readings of the bytes."""
+import os
+
+import numpy as np
+import pytest
+
+import ifos_amd
+from ifos_amd import formats
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_reference_parameter_file():
+    objs = formats.read_json(os.path.join(GOLDEN, "IFOS2D_FW.json"))
+    assert objs["NX"] == "500" and objs["NY"] == "100" and objs["DH"] == "0.2"
+    assert objs["FDORDER"] == "2" and objs["FW"] == "20" and objs["npower"] == "4.0"
+    # two objects on one line, names and values cut at the blanks
+    assert objs["REFRECX"] == "0.0" and objs["REFRECY"] == "0.0"
+    assert objs["XREC1"] == "6.0" and objs["YREC1"] == "0.2"
+    # comment lines are not objects
+    assert not any("comment" in v.lower() for v in objs.values())
+    c = formats.config_from_json(os.path.join(GOLDEN, "IFOS2D_FW.json"), ifos_amd.make_config())
+    assert (c.NX, c.NY, c.FDORDER, c.FW, c.FREE_SURF, c.BOUNDARY) == (500, 100, 2, 20, 1, 0)
+    assert c.DH == np.float32(0.2) and c.DT == np.float32(5e-5)
+    # NT = iround(TIME/DT) in FP32 (src/IFOS2D.c:257)
+    assert c.NT == int(np.floor(np.float32(0.5) / np.float32(5e-5) + 0.5)) == 10000
+    assert abs(c.VPPML - 600.0) < 1e-3 and abs(c.k_max_PML - 8.0) < 1e-6
+
+
+def test_parameter_file_errors(tmp_path):
+    p = tmp_path / "bad.json"
+    p.write_text('{\n"NX" : "10",\n"NY, DH" : "5",\n}\n')
+    with pytest.raises(ifos_amd.IfosError):
+        formats.read_json(str(p))
+    p.write_text('{\n"NX" : "1.5",\n"NY" : "10",\n"DH" : "1",\n"DT" : "1e-3",\n"TIME" : "0.1",\n"FDORDER" : "4"\n}\n')
+    with pytest.raises(ifos_amd.IfosError):
+        formats.config_from_json(str(p))
+    p.write_text('{\n"NX" : "10",\n"NY" : "10",\n"DH" : "1",\n"DT" : "1e-3",\n"FDORDER" : "4"\n}\n')
+    with pytest.raises(ifos_amd.IfosError):   # TIME missing
+        formats.config_from_json(str(p))
+
+
+def test_model_file_layout(tmp_path):
+    rng = np.random.default_rng(0)
+    m = rng.standard_normal((7, 11)).astype(np.float32)   # [NY][NX]
+    p = str(tmp_path / "m.vp")
+    formats.write_model(p, m)
+    raw = np.fromfile(p, np.float32)
+    # x outer, y inner (src/writemod.c): the file is the transpose, row-major
+    np.testing.assert_array_equal(raw, m.T.ravel())
+    np.testing.assert_array_equal(formats.read_model(p, 11, 7), m)
+    with pytest.raises(ifos_amd.IfosError):
+        formats.read_model(p, 12, 7)
+
+
+def test_su_roundtrip_and_headers(tmp_path):
+    rng = np.random.default_rng(1)
+    ntr, ns, DH, DT = 5, 300, 2.0, 2.5e-4
+    data = rng.standard_normal((ntr, ns)).astype(np.float32)
+    rec = np.array([[10 + 3 * k, 4] for k in range(ntr)], np.int32)
+    xs, ys = 52.0, 8.0
+    p = str(tmp_path / "s.su")
+    formats.write_su(p, data, rec, DH, DT, xs, ys, ndt=2)
+    raw = np.fromfile(p, np.uint8).reshape(ntr, 240 + 4 * ns)
+    hdr = raw[:, :240]
+    i32 = lambda off: hdr[:, off:off + 4].copy().view(np.int32).ravel()
+    i16 = lambda off: hdr[:, off:off + 2].copy().view(np.int16).ravel()
+    u16 = lambda off: hdr[:, off:off + 2].copy().view(np.uint16).ravel()
+    assert list(i32(0)) == [1, 2, 3, 4, 5] and list(i32(20)) == [1, 2, 3, 4, 5]     # tracl, cdp
+    assert list(i16(28)) == [1] * ntr                                                 # trid
+    xr = rec[:, 0] * np.float32(DH)
+    yr = rec[:, 1] * np.float32(DH)
+    off = np.floor(np.sqrt((xs - xr) ** 2 + (ys - yr) ** 2) * 1000.0 + 0.5)
+    np.testing.assert_array_equal(i32(36), off.astype(np.int32))                     # offset (mm)
+    np.testing.assert_array_equal(i32(72), np.full(ntr, 52000, np.int32))            # sx
+    np.testing.assert_array_equal(i32(80), (xr * 1000).astype(np.int32))             # gx
+    assert list(i16(68)) == [-1000] * ntr and list(i16(70)) == [-1000] * ntr         # scalel, scalco
+    assert list(u16(114)) == [ns] * ntr and list(u16(116)) == [500] * ntr            # ns, dt (us)
+    np.testing.assert_array_equal(raw[:, 240:].copy().view(np.float32), data)
+    np.testing.assert_array_equal(formats.read_su(p, ns), data)
+    with pytest.raises(ifos_amd.IfosError):
+        formats.read_su(p, ns + 1)
