@@ -79,9 +79,13 @@ def lib():
         L.ifos_engine_kernel_bytes_per_cell.argtypes = [C.c_void_p, C.c_int]
         for n in ["set_model", "set_shot", "forward", "seismograms", "fields", "residual", "adjoint",
                   "shot_gradient", "gradient", "reset_gradient", "time_kernel", "nan_step",
-                  "set_snapshot_precision", "set_shots", "shot_misfit", "set_launch", "get_launch", "autotune"]:
+                  "set_snapshot_precision", "set_shots", "shot_misfit", "set_launch", "get_launch", "autotune",
+                  "set_eprecond", "apply_eprecond", "energy_weight"]:
             getattr(L, "ifos_engine_" + n).restype = C.c_int
         L.ifos_snapshot_bytes.restype = C.c_size_t
+        L.ifos_engine_set_eprecond.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_int]
+        L.ifos_engine_bytes.restype = C.c_size_t
+        L.ifos_engine_bytes.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.ifos_snapshot_bytes.argtypes = [C.POINTER(Config), C.c_int]
         _LIB = L
     return _LIB
@@ -269,6 +273,18 @@ class Engine:
 
     def kernel_bytes_per_cell(self, kernel):
         return lib().ifos_engine_kernel_bytes_per_cell(self.h, kernel)
+
+    def set_eprecond(self, mode, epsilon_we=0.005, per_shot=False):
+        """Energy preconditioning of the gradient (EPRECOND 1 or 3; 0 off)."""
+        _check(lib().ifos_engine_set_eprecond(self.h, mode, epsilon_we, int(per_shot)))
+
+    def apply_eprecond(self):
+        _check(lib().ifos_engine_apply_eprecond(self.h))
+
+    def energy_weight(self, which=0):
+        out = np.zeros((self.cfg.NY, self.cfg.NX), np.float32)
+        _check(lib().ifos_engine_energy_weight(self.h, which, _fp(out)))
+        return out
 
     def set_snapshot_precision(self, prec):
         """0 / "fp32", 1 / "bf16" or 2 / "fp16" storage of the forward snapshots."""

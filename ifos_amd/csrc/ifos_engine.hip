@@ -127,6 +127,15 @@ struct ifos_engine {
     size_t tiles_cap = 0;
     int off_src_start = 0, off_src_idx = 0, off_rec_start = 0, off_rec_idx = 0;
     LaunchShape shape;
+    // energy preconditioner (src/eprecond.c, src/eprecond1.c): Ws and Wr per
+    // slot, the 1/We sum over shots, the mode-3 geometric factor
+    int ep_mode = 0, ep_per_shot = 0;
+    float ep_eps = 0.0f;
+    float *wbuf = nullptr;      // [slots][2][plane]: Ws then Wr (Ws becomes We after the adjoint)
+    float *we_sum = nullptr;    // [plane]
+    double *ep_geom = nullptr;  // [plane], mode 3
+    float *we_max = nullptr;    // [slots]
+    float *red = nullptr;       // reduction partials
     StepArgs *dargs = nullptr;  // device copies of the argument blocks: forward, forward with
                                 // snapshots, adjoint, eager/timing
 
@@ -312,6 +321,7 @@ extern "C" void ifos_engine_destroy(ifos_engine *E)
     drop_graphs(E);
     hipFree(E->slab); hipFree(E->mslab); hipFree(E->interior); hipFree(E->wcbuf); hipFree(E->pml);
     hipFree(E->psi); hipFree(E->ctl); hipFree(E->explosive); hipFree(E->dargs);
+    hipFree(E->wbuf); hipFree(E->we_sum); hipFree(E->ep_geom); hipFree(E->we_max); hipFree(E->red);
     hipFree(E->src_i); hipFree(E->src_d); hipFree(E->sig); hipFree(E->rec_i); hipFree(E->seis);
     hipFree(E->trace_sums); hipFree(E->snap); hipFree(E->tiles);
     if (E->stream) hipStreamDestroy(E->stream);
@@ -598,6 +608,12 @@ static StepArgs step_args(ifos_engine *E, int sw, bool snapshots)
     a.wc = E->wcbuf; a.wc_u = E->wcbuf + E->plane; a.wc_rho = E->wcbuf + 2 * E->plane;
     a.wcs = 3 * E->plane;
     a.seis_vx = seis_buf(E, 0); a.seis_vy = seis_buf(E, 1);
+    // energies for the preconditioner in gradient runs: Ws in the forward
+    // run that stores snapshots, Wr in the adjoint (EPRECOND 1 only)
+    if (E->ep_mode && ((sw == 0 && snapshots) || (sw == 1 && E->ep_mode == 1))) {
+        a.wacc = E->wbuf + (sw == 0 ? 0 : E->plane);
+        a.wacs = 2 * E->plane;
+    }
     a.base = E->ctl;
     a.nan_flag = E->ctl + 1;
     a.nbx = E->nbx;
@@ -719,6 +735,9 @@ extern "C" int ifos_engine_forward(ifos_engine *E, int store_snapshots)
     if (store_snapshots) ENG_TRY(ensure_snapshots(E));
     E->snap_valid = false;
     ENG_TRY(zero_state(E));
+    // energies of this shot's wavefields start at zero (src/IFOS2D.c:1623-1631)
+    if (E->ep_mode && store_snapshots)
+        HIP_TRY(hipMemsetAsync(E->wbuf, 0, sizeof(float) * 2 * E->plane * E->nslots, E->stream));
     ENG_TRY(run_loop(E, 0, store_snapshots != 0));
     ENG_TRY(check_nan(E));
     E->snap_valid = store_snapshots != 0;
@@ -802,6 +821,159 @@ extern "C" int ifos_engine_residual(ifos_engine *E, const float *obs_vx, const f
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// energy preconditioner (src/eprecond1.c:8-69, src/IFOS2D.c:2615-2654, 2737-2751)
+// ---------------------------------------------------------------------------
+// We of each slot, written over its Ws plane: sqrt(Ws*Wr) (mode 1) or
+// sqrt(Ws) times the geometric receiver term (mode 3)
+__global__ void k_we_form(long n, int nslots, int mode, float *wbuf_, const double *geom_)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    for (int s = 0; s < nslots; s++) {
+        auto W = gmem(wbuf_) + (long)s * 2 * n;
+        if (mode == 1) W[e] = (float)sqrt((double)(W[e] * W[n + e]));
+        else W[e] = (float)(sqrt((double)W[e]) * gmem(geom_)[e]);
+    }
+}
+
+// per-block maxima of |W| of one slot's We plane
+__global__ __launch_bounds__(256) void k_absmax_part(const float *w_, long n, float *part_)
+{
+    __shared__ float lds[4];
+    float m = 0.0f;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(gmem(w_)[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (threadIdx.x % 64 == 0) lds[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) gmem(part_)[blockIdx.x] = fmaxf(fmaxf(lds[0], lds[1]), fmaxf(lds[2], lds[3]));
+}
+
+// We + epsilon*max(We) per slot (eps from the device array, one per slot)
+__global__ void k_we_add(long n, int nslots, float *wbuf_, const float *eps_)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    for (int s = 0; s < nslots; s++) {
+        auto W = gmem(wbuf_) + (long)s * 2 * n;
+        W[e] = W[e] + gmem(eps_)[s];
+    }
+}
+
+// summed gradient times We_sum / max(We_sum); We_sum back to zero
+__global__ void k_apply_wesum(long n, float *we_sum_, float wmax, float *gvp_, float *gvs_, float *grho_)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    auto ws = gmem(we_sum_);
+    const float w = ws[e] / wmax;
+    gmem(gvp_)[e] = gmem(gvp_)[e] * w;
+    gmem(gvs_)[e] = gmem(gvs_)[e] * w;
+    gmem(grho_)[e] = gmem(grho_)[e] * w;
+    ws[e] = 0.0f;
+}
+
+static constexpr int kMaxParts = 512;
+
+static int plane_absmax(ifos_engine *E, const float *w, float *out)
+{
+    hipLaunchKernelGGL(k_absmax_part, dim3(kMaxParts), dim3(256), 0, E->stream, w, E->plane, E->red);
+    HIP_TRY(hipGetLastError());
+    std::vector<float> h(kMaxParts);
+    HIP_TRY(hipMemcpyAsync(h.data(), E->red, sizeof(float) * kMaxParts, hipMemcpyDeviceToHost, E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    float m = 0.0f;
+    for (float v : h) m = fmaxf(m, v);
+    *out = m;
+    return 0;
+}
+
+static int energy_weights(ifos_engine *E, GradArgs &ga)
+{
+    const long n = E->plane;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_we_form, dim3(nb), dim3(256), 0, E->stream, n, E->nslots, E->ep_mode, E->wbuf, E->ep_geom);
+    HIP_TRY(hipGetLastError());
+    std::vector<float> eps(E->nslots), wmax(E->nslots);
+    for (int s = 0; s < E->nslots; s++) {
+        float m;
+        ENG_TRY(plane_absmax(E, E->wbuf + (long)s * 2 * n, &m));
+        eps[s] = E->ep_eps * m;  // src/eprecond1.c:63-68
+    }
+    HIP_TRY(hipMemcpyAsync(E->we_max, eps.data(), sizeof(float) * E->nslots, hipMemcpyHostToDevice, E->stream));
+    hipLaunchKernelGGL(k_we_add, dim3(nb), dim3(256), 0, E->stream, n, E->nslots, E->wbuf, E->we_max);
+    HIP_TRY(hipGetLastError());
+    if (E->ep_per_shot) {
+        // global_maximum of the regularised weights (src/IFOS2D.c:2621)
+        for (int s = 0; s < E->nslots; s++) ENG_TRY(plane_absmax(E, E->wbuf + (long)s * 2 * n, &wmax[s]));
+        HIP_TRY(hipMemcpyAsync(E->we_max, wmax.data(), sizeof(float) * E->nslots, hipMemcpyHostToDevice, E->stream));
+        HIP_TRY(hipStreamSynchronize(E->stream));
+    }
+    ga.we = E->wbuf;
+    ga.wes = 2 * n;
+    ga.we_max = E->we_max;
+    ga.per_shot = E->ep_per_shot;
+    ga.we_sum = E->we_sum;
+    return 0;
+}
+
+extern "C" int ifos_engine_set_eprecond(ifos_engine *E, int mode, float epsilon_we, int per_shot)
+{
+    if (mode != 0 && mode != 1 && mode != 3) return ifos_set_error("EPRECOND must be 0, 1 or 3"), -2;
+    HIP_TRY(hipSetDevice(E->device));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    E->ep_eps = epsilon_we;
+    E->ep_per_shot = per_shot ? 1 : 0;
+    if (mode == E->ep_mode) return 0;
+    drop_graphs(E);
+    E->ep_mode = mode;
+    if (!mode || E->wbuf) return 0;
+    const size_t n = (size_t)E->plane;
+    if (hipMalloc(&E->wbuf, sizeof(float) * 2 * n * E->slots) != hipSuccess ||
+        hipMalloc(&E->we_sum, sizeof(float) * n) != hipSuccess ||
+        hipMalloc(&E->ep_geom, sizeof(double) * n) != hipSuccess ||
+        hipMalloc(&E->we_max, sizeof(float) * E->slots) != hipSuccess ||
+        hipMalloc(&E->red, sizeof(float) * kMaxParts) != hipSuccess) {
+        E->ep_mode = 0;
+        return ifos_set_error("device allocation for the energy preconditioner failed"), -3;
+    }
+    HIP_TRY(hipMemset(E->we_sum, 0, sizeof(float) * n));
+    // geometric receiver term of mode 3 (src/eprecond1.c:46-48), FP64 on the host
+    const ifos_config &c = E->c;
+    std::vector<double> g(n);
+    const float xmin = c.DH, xmax = c.NX * c.DH;
+    for (int j = 1; j <= c.NY; j++)
+        for (int i = 1; i <= c.NX; i++) {
+            const float x = i * c.DH, y = j * c.DH;
+            g[(size_t)(j - 1) * c.NX + (i - 1)] = asinh((xmax - x) / y) - asinh((xmin - x) / y);
+        }
+    HIP_TRY(hipMemcpy(E->ep_geom, g.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int ifos_engine_apply_eprecond(ifos_engine *E)
+{
+    HIP_TRY(hipSetDevice(E->device));
+    if (!E->ep_mode || E->ep_per_shot) return 0;
+    float m;
+    ENG_TRY(plane_absmax(E, E->we_sum, &m));  // global_maximum (src/IFOS2D.c:2740)
+    const unsigned nb = (unsigned)((E->plane + 255) / 256);
+    hipLaunchKernelGGL(k_apply_wesum, dim3(nb), dim3(256), 0, E->stream, E->plane, E->we_sum, m, E->s_vp, E->s_vs,
+                       E->s_rho);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+extern "C" int ifos_engine_energy_weight(ifos_engine *E, int which, float *out)
+{
+    HIP_TRY(hipSetDevice(E->device));
+    if (!E->ep_mode) return ifos_set_error("energy preconditioning is off"), -2;
+    HIP_TRY(hipMemcpy(out, which ? E->we_sum : E->wbuf, sizeof(float) * E->plane, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 extern "C" int ifos_engine_adjoint(ifos_engine *E)
 {
     HIP_TRY(hipSetDevice(E->device));
@@ -812,6 +984,8 @@ extern "C" int ifos_engine_adjoint(ifos_engine *E)
     HIP_TRY(hipMemsetAsync(E->wcbuf, 0, sizeof(float) * 3 * E->plane * E->nslots, E->stream));
     ENG_TRY(run_loop(E, 1, false));
     GradArgs ga;
+    memset(&ga, 0, sizeof ga);
+    if (E->ep_mode) ENG_TRY(energy_weights(E, ga));
     ga.n = (int)E->plane; ga.DT = E->c.DT;
     ga.vp = E->vp_i; ga.vs = E->vs_i; ga.rho = E->rho_i;
     ga.wc = E->wcbuf; ga.wc_u = E->wcbuf + E->plane; ga.wc_rho = E->wcbuf + 2 * E->plane;

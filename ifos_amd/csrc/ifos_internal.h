@@ -166,6 +166,13 @@ struct GradArgs {
     int nslots;
     float *g_vp, *g_vs, *g_rho;       // gradient of the current shots (summed in slot order)
     float *s_vp, *s_vs, *s_rho;       // running sums over shots (may be null)
+    // energy preconditioner: per-slot weights We (stride wes) and, per slot,
+    // the largest |We|; per_shot: divide each shot's gradient by We/We_max,
+    // else add 1/We to we_sum (null: no preconditioning)
+    const float *we, *we_max;
+    long wes;
+    int per_shot;
+    float *we_sum;
 };
 
 // adjoint sources from seismograms (src/calc_res.c, LNORM 2)

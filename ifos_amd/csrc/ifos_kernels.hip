@@ -545,13 +545,25 @@ IFOS_KERNEL void k_gradient(GradArgs a)
         const long o = (long)s * a.wcs + e;
         float wl = -DT * a.wc[o];
         wl = (float)((1.0 / (4.0 * (double)(lamss + muss) * (double)(lamss + muss))) * (double)wl);
-        const float svp = (float)(2.0 * (double)pv * (double)rh * (double)wl);
+        float svp = (float)(2.0 * (double)pv * (double)rh * (double)wl);
         const float wmu = -DT * a.wc_u[o];
-        const float svs =
+        float svs =
             (float)(-4.0 * (double)rh * (double)u * (double)wl + 2.0 * (double)rh * (double)u * (double)wmu);
         const float wrs = -DT * a.wc_rho[o];
-        const float srho = (float)(((double)(pv * pv) - (2.0 * (double)u * (double)u)) * (double)wl +
-                                   (double)(u * u * wmu) + (double)wrs);
+        float srho = (float)(((double)(pv * pv) - (2.0 * (double)u * (double)u)) * (double)wl +
+                             (double)(u * u * wmu) + (double)wrs);
+        // energy preconditioning of the shot (src/IFOS2D.c:2627-2654)
+        if (a.we) {
+            const float We = a.we[(long)s * a.wes + e];
+            if (a.per_shot) {
+                const float w = We / a.we_max[s];
+                svp = svp / (w);
+                svs = svs / (w);
+                srho = srho / (w);
+            } else {
+                a.we_sum[e] = (float)((double)a.we_sum[e] + 1.0 / (double)We);
+            }
+        }
         gvp = s ? gvp + svp : svp;
         gvs = s ? gvs + svs : svs;
         grho = s ? grho + srho : srho;

@@ -60,6 +60,8 @@ struct StepArgs {
     float *wc_rho, *wc, *wc_u;  // per slot, stride wcs floats
     long wcs;
     float *seis_vx, *seis_vy;   // [trace][NT], traces of all slots
+    float *wacc;                // energy accumulator of the preconditioner (per slot, stride wacs), or null
+    long wacs;
     const int *base;            // steps completed before the current graph unit
     int *nan_flag;
     int nbx, nby;               // tiles in x and y

@@ -215,6 +215,11 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     }
     vxs[c] = vx;
     vys[c] = vy;
+    // energy of the new particle velocities (eprecond, src/eprecond.c:8-21)
+    if (a.wacc) {
+        const auto W = gmem(a.wacc) + (long)s * a.wacs;
+        W[e] += ((vx * vx) + (vy * vy));
+    }
 }
 
 template <int N, int SPB>

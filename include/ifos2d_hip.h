@@ -141,6 +141,21 @@ int  ifos_engine_shot_gradient(ifos_engine *e, const float *obs_vx, const float 
  * Arrays [NY][NX]: Vp, Vs, rho gradients. */
 int  ifos_engine_gradient(ifos_engine *e, int which, float *g_vp, float *g_vs, float *g_rho);
 int  ifos_engine_reset_gradient(ifos_engine *e);
+/* Energy preconditioning of the gradient (EPRECOND, EPSILON_WE,
+ * EPRECOND_PER_SHOT of src/IFOS2D.c): mode 1 weights with
+ * sqrt(Ws*Wr), mode 3 with sqrt(Ws) times the receiver Green's-function
+ * term (src/eprecond1.c:8-69); Ws, Wr = sums over the time steps of
+ * vx^2 + vy^2 of the forward and adjoint wavefields (src/eprecond.c),
+ * accumulated inside the velocity step of gradient runs.  per_shot != 0
+ * divides each shot's gradient by We/max|We| (src/IFOS2D.c:2627-2638);
+ * otherwise 1/We is summed over shots and ifos_engine_apply_eprecond scales
+ * the summed gradient by that sum over its maximum (src/IFOS2D.c:2737-2751).
+ * mode 0 turns it off. */
+int  ifos_engine_set_eprecond(ifos_engine *e, int mode, float epsilon_we, int per_shot);
+int  ifos_engine_apply_eprecond(ifos_engine *e);
+/* which = 0: We of the first slot's shot of the last adjoint run; 1: the
+ * sum of 1/We since the last apply.  [NY][NX]. */
+int  ifos_engine_energy_weight(ifos_engine *e, int which, float *out);
 /* Storage of the forward snapshots (src/IFOS2D.c:1805-1871 keeps them FP32):
  * 0 = FP32 (default; gradients bit-identical to the reference), 1 = BF16,
  * 2 = FP16.  16-bit snapshots hold value / 2^e rounded to nearest-even, with

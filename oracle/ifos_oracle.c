@@ -894,8 +894,19 @@ typedef struct {
     real *F[5];      /* rho_x, rho_y, x, y, u : [nsnap][NY][NX]              */
 } snaps_t;
 
+/* energy of the particle velocities at the end of a time step
+ * (eprecond, src/eprecond.c:8-21, called at src/IFOS2D.c:1872-1874 and
+ * :2420-2425) */
+static void energy_add(const shot_t *S, real *W)
+{
+    const int NX = S->P.NX, NY = S->P.NY;
+    for (int j = 1; j <= NY; j++)
+        for (int i = 1; i <= NX; i++)
+            W[(size_t)(j - 1) * NX + (i - 1)] += ((S->vx[j][i] * S->vx[j][i]) + (S->vy[j][i] * S->vy[j][i]));
+}
+
 static void run_forward(shot_t *S, int nsrc, const float *srcpos, int ntr, const int *recpos, real *seis_vx,
-                        real *seis_vy, snaps_t *SN, int *dtinv_help)
+                        real *seis_vy, snaps_t *SN, int *dtinv_help, real *Ws)
 {
     const orc_params *p = &S->P;
     int NT = p->NT, NX = p->NX, NY = p->NY;
@@ -932,6 +943,7 @@ static void run_forward(shot_t *S, int nsrc, const float *srcpos, int ntr, const
             hin1 += p->DTINV;
             if (dtinv_help) dtinv_help[nt] = 1;
         }
+        if (Ws) energy_add(S, Ws);
     }
     free(sig);
     free(rsig);
@@ -944,7 +956,7 @@ int orc_forward(const orc_params *p, const float *vp, const float *vs, const flo
     if (p->FDORDER < 2 || p->FDORDER > 12 || (p->FDORDER & 1)) return -1;
     shot_t S;
     shot_init(&S, p, vp, vs, rho, taus, taup);
-    run_forward(&S, nsrc, srcpos, ntr, recpos, seis_vx, seis_vy, NULL, NULL);
+    run_forward(&S, nsrc, srcpos, ntr, recpos, seis_vx, seis_vy, NULL, NULL, NULL);
     if (final5) {
         real **src[5] = {S.vx, S.vy, S.sxx, S.syy, S.sxy};
         for (int k = 0; k < 5; k++)
@@ -1032,6 +1044,15 @@ int orc_gradient(const orc_params *p, const float *vp, const float *vs, const fl
                  const float *obs_vy, const int *kill, real *seis_vx, real *seis_vy, real *grad_vp, real *grad_vs,
                  real *grad_rho, double *misfit)
 {
+    return orc_gradient_we(p, vp, vs, rho, taus, taup, nsrc, srcpos, ntr, recpos, obs_vx, obs_vy, kill, seis_vx,
+                           seis_vy, grad_vp, grad_vs, grad_rho, misfit, NULL);
+}
+
+int orc_gradient_we(const orc_params *p, const float *vp, const float *vs, const float *rho, const float *taus,
+                    const float *taup, int nsrc, const float *srcpos, int ntr, const int *recpos,
+                    const float *obs_vx, const float *obs_vy, const int *kill, real *seis_vx, real *seis_vy,
+                    real *grad_vp, real *grad_vs, real *grad_rho, double *misfit, real *We_out)
+{
     if (p->FDORDER < 2 || p->FDORDER > 12 || (p->FDORDER & 1)) return -1;
     if (p->DTINV < 1) return -2;
     int NT = p->NT, NX = p->NX, NY = p->NY;
@@ -1042,7 +1063,12 @@ int orc_gradient(const orc_params *p, const float *vp, const float *vs, const fl
     size_t plane = (size_t)NX * NY;
     for (int k = 0; k < 5; k++) SN.F[k] = (real *)calloc((size_t)SN.nsnap * plane, sizeof(real));
     int *help = (int *)calloc((size_t)NT + 2, sizeof(int));
-    run_forward(&S, nsrc, srcpos, ntr, recpos, seis_vx, seis_vy, &SN, help);
+    /* energies of the forward and adjoint wavefields, reset per shot
+     * (src/IFOS2D.c:1623-1631) */
+    const int ep = p->EPRECOND == 1 || p->EPRECOND == 3;
+    real *Ws = ep ? (real *)calloc(plane, sizeof(real)) : NULL;
+    real *Wr = ep ? (real *)calloc(plane, sizeof(real)) : NULL;
+    run_forward(&S, nsrc, srcpos, ntr, recpos, seis_vx, seis_vy, &SN, help, Ws);
 
     /* residuals (src/IFOS2D.c:1965-2060): vx then vy, L2 carried over */
     real *dvx = (real *)calloc((size_t)ntr * NT + 1, sizeof(real));
@@ -1091,6 +1117,7 @@ int orc_gradient(const orc_params *p, const float *vp, const float *vs, const fl
                 }
             hin++;
         }
+        if (p->EPRECOND == 1) energy_add(&S, Wr);
     }
 
     /* chain rule to Vp, Vs, rho (src/IFOS2D.c:2471-2611, PARAMETERIZATION 1) */
@@ -1110,6 +1137,43 @@ int orc_gradient(const orc_params *p, const float *vp, const float *vs, const fl
             grad_rho[c] = ((pv * pv) - (2.0 * u * u)) * wl + u * u * wmu + wrs;
         }
 
+    /* energy weight of the shot (src/eprecond1.c:8-69) and its per-shot
+     * application (src/IFOS2D.c:2615-2638) */
+    if (ep) {
+        real *We = (real *)calloc(plane, sizeof(real));
+        float maxWe = 0.0f;
+        const float xmin = p->DH, xmax = NX * p->DH;
+        for (int j = 1; j <= NY; j++)
+            for (int i = 1; i <= NX; i++) {
+                const size_t c = (size_t)(j - 1) * NX + (i - 1);
+                if (p->EPRECOND == 1) {
+                    We[c] = sqrt(Ws[c] * Wr[c]);
+                } else {
+                    const float x = i * p->DH, y = j * p->DH;
+                    We[c] = sqrt(Ws[c]) * (asinh((xmax - x) / y) - asinh((xmin - x) / y));
+                }
+                if (We[c] > maxWe) maxWe = We[c];
+            }
+        for (size_t c = 0; c < plane; c++) We[c] = We[c] + (p->EPSILON_WE * maxWe);
+        if (We_out)
+            for (size_t c = 0; c < plane; c++) We_out[c] = We[c];
+        if (p->EPRECOND_PER_SHOT) {
+            /* global_maximum (src/matrix_operations.c:53-70): max |We| */
+            float We_max = 0.0f;
+            for (size_t c = 0; c < plane; c++)
+                if (c == 0 || We_max == 0.0) We_max = fabs(We[c]);
+                else if (fabs(We[c]) > We_max) We_max = fabs(We[c]);
+            for (size_t c = 0; c < plane; c++) {
+                We[c] = We[c] / We_max;
+                grad_vp[c] = grad_vp[c] / (We[c]);
+                grad_vs[c] = grad_vs[c] / (We[c]);
+                grad_rho[c] = grad_rho[c] / (We[c]);
+            }
+        }
+        free(We);
+    }
+    free(Ws);
+    free(Wr);
     free(wc); free(wc_u); free(wc_rho); free(bpos); free(dvx); free(dvy); free(help);
     for (int k = 0; k < 5; k++) free(SN.F[k]);
     shot_free(&S);

@@ -57,6 +57,11 @@ typedef struct orc_params {
     int SNAP_PREC;         /* snapshot storage: 0 FP32 (the reference), 1 BF16,
                               2 FP16, each with a power-of-two max-abs scale per
                               field, slot and 64x8 tile (no reference counterpart) */
+    int EPRECOND;          /* energy preconditioning: 0 off, 1 sqrt(Ws*Wr), 3
+                              sqrt(Ws) times the receiver Green's-function term
+                              (src/eprecond1.c)                              */
+    int EPRECOND_PER_SHOT; /* 1: divide each shot's gradient by We/max(We)    */
+    float EPSILON_WE;      /* regularisation of We, relative to max(We)      */
 } orc_params;
 
 /* Sources: srcpos[l*8 + k] holds the reference's srcpos_loc[k+1][l+1]
@@ -101,6 +106,17 @@ int orc_gradient(const orc_params *p, const float *vp, const float *vs,
                  real *seis_vx, real *seis_vy,
                  real *grad_vp, real *grad_vs, real *grad_rho,
                  double *misfit);
+
+/* orc_gradient, and with EPRECOND set the shot's energy weight We [NY][NX]
+ * (after the regularisation of src/eprecond1.c:63-68, before the per-shot
+ * normalisation) in We_out. */
+int orc_gradient_we(const orc_params *p, const float *vp, const float *vs,
+                    const float *rho, const float *taus, const float *taup,
+                    int nsrc, const float *srcpos, int ntr, const int *recpos,
+                    const float *obs_vx, const float *obs_vy, const int *kill,
+                    real *seis_vx, real *seis_vy,
+                    real *grad_vp, real *grad_vs, real *grad_rho,
+                    double *misfit, real *We_out);
 
 /* Residual only (calc_res, LNORM=2): diff[ntr][NT] time-reversed adjoint
  * source, returns the FP32 running L2 (all traces counted). */

@@ -245,6 +245,9 @@ static void copy_interior(float **a, float *out)
 
 /* forward run: seismograms [ntr][NT], optional final wavefields, optional
  * snapshots (5 fields x nsnap planes of [NY][NX]) */
+/* energy accumulators of the preconditioner (set by ref_gradient_we) */
+static float **harness_Ws = NULL, **harness_Wr = NULL;
+
 static void run_forward(run_t *R, int nsrc, const float *srcpos, int ntr, const int *recpos, float *seis_vx,
                         float *seis_vy, float **snap, int *help, int dtinv)
 {
@@ -274,6 +277,7 @@ static void run_forward(run_t *R, int nsrc, const float *srcpos, int ntr, const 
             hin1 += dtinv;
             help[nt] = 1;
         }
+        if (harness_Ws) eprecond(harness_Ws, R->pvx, R->pvy);  /* src/IFOS2D.c:1872-1874 */
     }
     for (int t = 1; t <= ntr; t++)
         for (int nt = 1; nt <= NT; nt++) {
@@ -307,12 +311,40 @@ int ref_forward(const int *ip, const float *fp, const float *FLv, const float *v
 /* forward with snapshots, then the adjoint run driven by the given
  * time-reversed residuals diff_vx/diff_vy [ntr][NT], correlation and chain
  * rule.  Outputs per-shot gradients [NY][NX]. */
+int ref_gradient_we(const int *ip, const float *fp, const float *FLv, const float *vp, const float *vs,
+                    const float *rho, const float *taus, const float *taup, int nsrc, const float *srcpos, int ntr,
+                    const int *recpos, const float *diff_vx, const float *diff_vy, float *seis_vx, float *seis_vy,
+                    float *grad_vp, float *grad_vs, float *grad_rho, int ep_mode, float epsilon_we, int per_shot,
+                    float *We_out);
+
 int ref_gradient(const int *ip, const float *fp, const float *FLv, const float *vp, const float *vs,
                  const float *rho, const float *taus, const float *taup, int nsrc, const float *srcpos, int ntr,
                  const int *recpos, const float *diff_vx, const float *diff_vy, float *seis_vx, float *seis_vy,
                  float *grad_vp, float *grad_vs, float *grad_rho)
 {
+    return ref_gradient_we(ip, fp, FLv, vp, vs, rho, taus, taup, nsrc, srcpos, ntr, recpos, diff_vx, diff_vy,
+                           seis_vx, seis_vy, grad_vp, grad_vs, grad_rho, 0, 0.0f, 0, NULL);
+}
+
+/* with energy preconditioning: the reference's eprecond() accumulates the
+ * forward (and for EPRECOND 1 the adjoint) energies every time step,
+ * eprecond1() forms We, and the per-shot division of src/IFOS2D.c:2627-2638
+ * follows; We_out receives We after eprecond1. */
+int ref_gradient_we(const int *ip, const float *fp, const float *FLv, const float *vp, const float *vs,
+                    const float *rho, const float *taus, const float *taup, int nsrc, const float *srcpos, int ntr,
+                    const int *recpos, const float *diff_vx, const float *diff_vy, float *seis_vx, float *seis_vy,
+                    float *grad_vp, float *grad_vs, float *grad_rho, int ep_mode, float epsilon_we, int per_shot,
+                    float *We_out)
+{
     harness_setup(ip, fp);
+    EPRECOND = ep_mode;
+    const int nd = FDORDER / 2 + 1;
+    float **We = NULL;
+    if (ep_mode == 1 || ep_mode == 3) {
+        harness_Ws = matrix(-nd + 1, NY + nd, -nd + 1, NX + nd);
+        harness_Wr = matrix(-nd + 1, NY + nd, -nd + 1, NX + nd);
+        We = matrix(-nd + 1, NY + nd, -nd + 1, NX + nd);
+    }
     int DTINV = ip[11];
     run_t R;
     run_init(&R, vp, vs, rho, taus, taup, FLv);
@@ -360,6 +392,7 @@ int ref_gradient(const int *ip, const float *fp, const float *FLv, const float *
                 }
             hin++;
         }
+        if (ep_mode == 1) eprecond(harness_Wr, R.pvx, R.pvy);  /* src/IFOS2D.c:2420-2425 */
     }
     for (int j = 1; j <= NY; j++)
         for (int i = 1; i <= NX; i++) {
@@ -373,6 +406,27 @@ int ref_gradient(const int *ip, const float *fp, const float *FLv, const float *
             float wrs = -DT * wcr[c];
             grad_rho[c] = ((pv * pv) - (2.0 * u * u)) * wl + u * u * wmu + wrs;
         }
+    if (We) {
+        eprecond1(We, harness_Ws, harness_Wr, epsilon_we);
+        for (int j = 1; j <= NY; j++)
+            for (int i = 1; i <= NX; i++)
+                if (We_out) We_out[(size_t)(j - 1) * NX + (i - 1)] = We[j][i];
+        if (per_shot) {
+            const float We_max = global_maximum(We);
+            for (int j = 1; j <= NY; j++)
+                for (int i = 1; i <= NX; i++) {
+                    size_t c = (size_t)(j - 1) * NX + (i - 1);
+                    We[j][i] = We[j][i] / We_max;
+                    grad_vp[c] = grad_vp[c] / (We[j][i]);
+                    grad_vs[c] = grad_vs[c] / (We[j][i]);
+                    grad_rho[c] = grad_rho[c] / (We[j][i]);
+                }
+        }
+        free_matrix(harness_Ws, -nd + 1, NY + nd, -nd + 1, NX + nd);
+        free_matrix(harness_Wr, -nd + 1, NY + nd, -nd + 1, NX + nd);
+        free_matrix(We, -nd + 1, NY + nd, -nd + 1, NX + nd);
+        harness_Ws = harness_Wr = NULL;
+    }
     free(wc); free(wcu); free(wcr); free(help);
     for (int k = 0; k < 5; k++) free(snap[k]);
     return 0;

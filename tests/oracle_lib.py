@@ -21,7 +21,8 @@ _FLT_FIELDS = ["DH", "DT", "VPPML", "FPML", "npower", "k_max_PML", "F_REF", "TAU
 
 class OrcParams(C.Structure):
     _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [(n, C.c_float) for n in _FLT_FIELDS]
-                + [("FL", C.c_float * 8), ("SNAP_PREC", C.c_int)])
+                + [("FL", C.c_float * 8), ("SNAP_PREC", C.c_int), ("EPRECOND", C.c_int),
+                   ("EPRECOND_PER_SHOT", C.c_int), ("EPSILON_WE", C.c_float)])
 
 
 def make_params(**kw):
@@ -281,3 +282,46 @@ def lbfgs_core(iteration, s, y, rho, q, alpha, which="oracle"):
     f = ref().ref_lbfgs_core if which == "ref" else oracle().orc_lbfgs_core
     f(iteration, N, n, _ptr(s), _ptr(y), _ptr(rho), _ptr(q), _ptr(alpha), _ptr(r))
     return r, rho, alpha, q
+
+
+def gradient_we(p, model, srcpos, recpos, obs_vx, obs_vy):
+    """Oracle gradient with the energy preconditioner set in p; returns
+    (gradient dict, We [NY][NX])."""
+    vp, vs, rho = (_f32(model[k]) for k in ("vp", "vs", "rho"))
+    taus = _f32(model["taus"]) if p.L else None
+    taup = _f32(model["taup"]) if p.L else None
+    srcpos = _f32(srcpos).reshape(-1, 8)
+    recpos = np.ascontiguousarray(recpos, dtype=np.int32).reshape(-1, 2)
+    ntr = recpos.shape[0]
+    svx = np.zeros((ntr, p.NT), np.float32)
+    svy = np.zeros((ntr, p.NT), np.float32)
+    g = np.zeros((3, p.NY, p.NX), np.float32)
+    we = np.zeros((p.NY, p.NX), np.float32)
+    mis = np.zeros(5, np.float64)
+    rc = oracle().orc_gradient_we(C.byref(p), _ptr(vp), _ptr(vs), _ptr(rho), _ptr(taus), _ptr(taup),
+                                  C.c_int(srcpos.shape[0]), _ptr(srcpos), C.c_int(ntr), _ptr(recpos, C.c_int),
+                                  _ptr(_f32(obs_vx)), _ptr(_f32(obs_vy)), None, _ptr(svx), _ptr(svy), _ptr(g[0]),
+                                  _ptr(g[1]), _ptr(g[2]), _ptr(mis, C.c_double), _ptr(we))
+    assert rc == 0, rc
+    return dict(seis_vx=svx, seis_vy=svy, grad_vp=g[0], grad_vs=g[1], grad_rho=g[2], misfit=mis), we
+
+
+def ref_gradient_we(p, model, srcpos, recpos, diff_vx, diff_vy):
+    vp, vs, rho = (_f32(model[k]) for k in ("vp", "vs", "rho"))
+    taus = _f32(model["taus"]) if p.L else None
+    taup = _f32(model["taup"]) if p.L else None
+    srcpos = _f32(srcpos).reshape(-1, 8)
+    recpos = np.ascontiguousarray(recpos, dtype=np.int32).reshape(-1, 2)
+    ntr = recpos.shape[0]
+    svx = np.zeros((ntr, p.NT), np.float32)
+    svy = np.zeros((ntr, p.NT), np.float32)
+    g = np.zeros((3, p.NY, p.NX), np.float32)
+    we = np.zeros((p.NY, p.NX), np.float32)
+    ip, fp, fl = _ip_fp(p)
+    rc = ref().ref_gradient_we(ip, fp, fl, _ptr(vp), _ptr(vs), _ptr(rho), _ptr(taus), _ptr(taup),
+                               C.c_int(srcpos.shape[0]), _ptr(srcpos), C.c_int(ntr), _ptr(recpos, C.c_int),
+                               _ptr(_f32(diff_vx)), _ptr(_f32(diff_vy)), _ptr(svx), _ptr(svy), _ptr(g[0]), _ptr(g[1]),
+                               _ptr(g[2]), C.c_int(p.EPRECOND), C.c_float(p.EPSILON_WE),
+                               C.c_int(p.EPRECOND_PER_SHOT), _ptr(we))
+    assert rc == 0, rc
+    return dict(grad_vp=g[0], grad_vs=g[1], grad_rho=g[2]), we

@@ -24,10 +24,11 @@ needs_ref = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not bu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def params_of(w):
+def params_of(w, **over):
     cfg = w.cfg
-    return O.make_params(FL=[cfg.FL[i] for i in range(1, 8)],
-                         **{k: getattr(cfg, k) for k, _ in cfg._fields_ if k != "FL"})
+    kw = {k: getattr(cfg, k) for k, _ in cfg._fields_ if k != "FL"}
+    kw.update(over)
+    return O.make_params(FL=[cfg.FL[i] for i in range(1, 8)], **kw)
 
 
 # ---------------------------------------------------------------- holberg --
@@ -166,6 +167,33 @@ def test_gradient_matches_reference(fd, L, vel):
     r = O.ref_gradient(p, w.start, s.srcpos, s.recpos, dvx, dvy)
     for k in ("seis_vx", "seis_vy", "grad_vp", "grad_vs", "grad_rho"):
         np.testing.assert_array_equal(g[k], r[k], err_msg=k)
+
+
+@needs_ref
+@pytest.mark.parametrize("mode,per_shot,L", [(1, 0, 0), (1, 1, 2), (3, 0, 0), (3, 1, 2)])
+def test_energy_preconditioner_matches_reference(mode, per_shot, L):
+    """eprecond / eprecond1 and the per-shot division (src/eprecond.c,
+    src/eprecond1.c, src/IFOS2D.c:1872-1874, 2420-2425, 2615-2638): the
+    reference's own functions in oracle/_ref against the oracle."""
+    w = workloads.small_case(seed=17 + mode + L, NX=52, NY=38, NT=110, FDORDER=4, L=L, VELOCITY=1, DTINV=2, FW=8)
+    p = params_of(w, EPRECOND=mode, EPRECOND_PER_SHOT=per_shot, EPSILON_WE=0.005)
+    s = w.shots[0]
+    ovx, ovy, _ = O.forward(p, w.true, s.srcpos, s.recpos)
+    ovx[:, 0] = 0
+    ovy[:, 0] = 0
+    g, we = O.gradient_we(p, w.start, s.srcpos, s.recpos, ovx, ovy)
+    dvx, _ = O.calc_res(p, g["seis_vx"], ovx)
+    dvy, _ = O.calc_res(p, g["seis_vy"], ovy)
+    r, we_ref = O.ref_gradient_we(p, w.start, s.srcpos, s.recpos, dvx, dvy)
+    np.testing.assert_array_equal(we, we_ref)
+    for k in ("grad_vp", "grad_vs", "grad_rho"):
+        np.testing.assert_array_equal(g[k], r[k], err_msg=k)
+    assert we.min() > 0
+    # closed form of the weight for mode 1: sqrt(Ws*Wr) + eps*max, Ws and Wr sums of squared velocities
+    if per_shot:
+        g0 = O.gradient(params_of(w), w.start, s.srcpos, s.recpos, ovx, ovy)
+        scale = we / np.abs(we).max()
+        np.testing.assert_allclose(g["grad_vp"], g0["grad_vp"] / scale, rtol=2e-6, atol=0)
 
 
 # ------------------------------------------------------------ golden data --
