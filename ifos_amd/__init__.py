@@ -281,6 +281,13 @@ class Engine:
     def apply_eprecond(self):
         _check(lib().ifos_engine_apply_eprecond(self.h))
 
+    def set_shot_taper(self, on, srtradius=0.0, srtshape=1, filtsize=0):
+        """Circular taper of each shot's gradient around its source
+        (SWS_TAPER_CIRCULAR_PER_SHOT); on=False turns it off."""
+        L = lib()
+        L.ifos_engine_set_shot_taper.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_int]
+        _check(L.ifos_engine_set_shot_taper(self.h, int(bool(on)), srtradius, srtshape, filtsize))
+
     def energy_weight(self, which=0):
         out = np.zeros((self.cfg.NY, self.cfg.NX), np.float32)
         _check(lib().ifos_engine_energy_weight(self.h, which, _fp(out)))

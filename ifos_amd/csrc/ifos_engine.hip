@@ -20,6 +20,7 @@
 
 #include "host/ifos_setup.h"
 #include "ifos2d_hip.h"
+#include "ifos2d_opt.h"
 #include "ifos_internal.h"
 #include "ifos_step.h"
 
@@ -136,6 +137,13 @@ struct ifos_engine {
     double *ep_geom = nullptr;  // [plane], mode 3
     float *we_max = nullptr;    // [slots]
     float *red = nullptr;       // reduction partials
+    // circular source taper of each shot's gradient (taper_grad_shot sws 1)
+    int tap_on = 0, tap_n = 0, tap_fs = 0;
+    float *tap_m = nullptr;         // [tap_n][tap_n]
+    int *tap_ij = nullptr;          // [slots][2] source cell of each slot
+    float *tap_lohi = nullptr;      // [slots][2] normalisation
+    std::vector<float> tap_host;    // the table on the host
+    std::vector<int> first_src;     // [slots][2] first source of each shot (0, 0: none)
     StepArgs *dargs = nullptr;  // device copies of the argument blocks: forward, forward with
                                 // snapshots, adjoint, eager/timing
 
@@ -322,6 +330,7 @@ extern "C" void ifos_engine_destroy(ifos_engine *E)
     hipFree(E->slab); hipFree(E->mslab); hipFree(E->interior); hipFree(E->wcbuf); hipFree(E->pml);
     hipFree(E->psi); hipFree(E->ctl); hipFree(E->explosive); hipFree(E->dargs);
     hipFree(E->wbuf); hipFree(E->we_sum); hipFree(E->ep_geom); hipFree(E->we_max); hipFree(E->red);
+    hipFree(E->tap_m); hipFree(E->tap_ij); hipFree(E->tap_lohi);
     hipFree(E->src_i); hipFree(E->src_d); hipFree(E->sig); hipFree(E->rec_i); hipFree(E->seis);
     hipFree(E->trace_sums); hipFree(E->snap); hipFree(E->tiles);
     if (E->stream) hipStreamDestroy(E->stream);
@@ -436,6 +445,12 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
     E->src_off = so;
     E->tr_off = to;
     E->shot_misfit.assign(2 * (size_t)nshots, 0.0);
+    E->first_src.assign(2 * (size_t)nshots, 0);
+    for (int k = 0; k < nshots; k++)
+        if (nsrc_of[k]) {
+            E->first_src[2 * k] = (int)srcpos[(size_t)so[k] * IFOS_SRC_STRIDE];
+            E->first_src[2 * k + 1] = (int)srcpos[(size_t)so[k] * IFOS_SRC_STRIDE + 1];
+        }
     // SOURCE_TYPE is rewritten by update_v_PML from the last source before
     // psource consults it (src/update_v_PML.c:558, src/IFOS2D.c:1758): per
     // shot, the last source's type decides whether explosive sources act
@@ -974,6 +989,57 @@ extern "C" int ifos_engine_energy_weight(ifos_engine *E, int which, float *out)
     return 0;
 }
 
+// per-slot source cells and normalisations of the circular taper
+static int shot_taper_args(ifos_engine *E, GradArgs &ga)
+{
+    const ifos_config &c = E->c;
+    std::vector<float> lohi(2 * (size_t)E->nslots);
+    for (int k = 0; k < E->nslots; k++) {
+        const int is = E->first_src[2 * k], js = E->first_src[2 * k + 1];
+        if (!is) return ifos_set_error("the source taper needs a source in every shot (shot %d has none)", k + 1), -2;
+        ifos_srctaper_norm(c.NX, c.NY, E->tap_n, E->tap_host.data(), is, js, &lohi[2 * k], &lohi[2 * k + 1]);
+    }
+    HIP_TRY(hipMemcpy(E->tap_ij, E->first_src.data(), sizeof(int) * 2 * E->nslots, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(E->tap_lohi, lohi.data(), sizeof(float) * 2 * E->nslots, hipMemcpyHostToDevice));
+    ga.tap_m = E->tap_m;
+    ga.tap_ij = E->tap_ij;
+    ga.tap_lohi = E->tap_lohi;
+    ga.tap_n = E->tap_n;
+    ga.tap_c = ifos_srctaper_centre(E->tap_n);
+    ga.tap_fs = E->tap_fs;
+    ga.NX = c.NX;
+    ga.NY = c.NY;
+    return 0;
+}
+
+extern "C" int ifos_engine_set_shot_taper(ifos_engine *E, int on, float SRTRADIUS, int SRTSHAPE, int FILTSIZE)
+{
+    HIP_TRY(hipSetDevice(E->device));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    E->tap_on = 0;
+    if (!on) return 0;
+    if (!(SRTRADIUS > 0.0f)) return ifos_set_error("SRTRADIUS must be positive"), -2;
+    const int n = ifos_srctaper_table(E->c.DH, SRTRADIUS, SRTSHAPE, nullptr, 0);
+    if (n > 4096) return ifos_set_error("SRTRADIUS %g gives a taper of %d x %d cells", SRTRADIUS, n, n), -2;
+    E->tap_host.assign((size_t)n * n, 0.0f);
+    ifos_srctaper_table(E->c.DH, SRTRADIUS, SRTSHAPE, E->tap_host.data(), n * n);
+    if (n > E->tap_n) {
+        hipFree(E->tap_m);
+        E->tap_m = nullptr;
+        E->tap_n = 0;
+        HIP_TRY(hipMalloc(&E->tap_m, sizeof(float) * n * n));
+    }
+    if (!E->tap_ij) {
+        HIP_TRY(hipMalloc(&E->tap_ij, sizeof(int) * 2 * E->slots));
+        HIP_TRY(hipMalloc(&E->tap_lohi, sizeof(float) * 2 * E->slots));
+    }
+    HIP_TRY(hipMemcpy(E->tap_m, E->tap_host.data(), sizeof(float) * n * n, hipMemcpyHostToDevice));
+    E->tap_n = n;
+    E->tap_fs = FILTSIZE;
+    E->tap_on = 1;
+    return 0;
+}
+
 extern "C" int ifos_engine_adjoint(ifos_engine *E)
 {
     HIP_TRY(hipSetDevice(E->device));
@@ -986,6 +1052,7 @@ extern "C" int ifos_engine_adjoint(ifos_engine *E)
     GradArgs ga;
     memset(&ga, 0, sizeof ga);
     if (E->ep_mode) ENG_TRY(energy_weights(E, ga));
+    if (E->tap_on) ENG_TRY(shot_taper_args(E, ga));
     ga.n = (int)E->plane; ga.DT = E->c.DT;
     ga.vp = E->vp_i; ga.vs = E->vs_i; ga.rho = E->rho_i;
     ga.wc = E->wcbuf; ga.wc_u = E->wcbuf + E->plane; ga.wc_rho = E->wcbuf + 2 * E->plane;

@@ -173,6 +173,15 @@ struct GradArgs {
     long wes;
     int per_shot;
     float *we_sum;
+    // circular source taper of each shot (taper_grad_shot sws 1): table
+    // tap_m [tap_n][tap_n] whose cell (tap_c, tap_c) sits on the source;
+    // per slot the source cell tap_ij[2s..2s+1] and the normalisation
+    // tap_lohi[2s..2s+1]; cells within tap_fs of the source become zero
+    // (null tap_m: no taper)
+    const float *tap_m;
+    const int *tap_ij;
+    const float *tap_lohi;
+    int tap_n, tap_c, tap_fs, NX, NY;
 };
 
 // adjoint sources from seismograms (src/calc_res.c, LNORM 2)

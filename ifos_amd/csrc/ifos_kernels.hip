@@ -524,6 +524,19 @@ IFOS_KERNEL void k_correlate(CorrArgs a)
 // chain rule from lambda/mu/rho' to Vp/Vs/rho (src/IFOS2D.c:2471-2611,
 // PARAMETERIZATION 1) and the sum over shots (src/IFOS2D.c:2703-2713)
 // ---------------------------------------------------------------------------
+// Is interior cell (i, j) zeroed by the (2F+1)^2 box around source (is, js)?
+// The reference writes the box into a scratch matrix(0,NY+1,0,NX+1) with
+// contiguous rows, for columns i' with (i'-1)/NX == 0 and rows 1..NY
+// (src/taper_grad_shot.c:168-181): columns 1..NX hit their own cell, columns
+// i' <= -2 hit column NX+2+i' of the row above.
+__device__ __forceinline__ bool in_box(int i, int j, int is, int js, int F, int NX, int NY)
+{
+    const int j0 = max(js - F, 1), j1 = min(js + F, NY);
+    if (j >= j0 && j <= j1 && i >= is - F && i <= is + F) return true;
+    const int iw = i - NX - 2;  // the box column that wraps onto (i, j)
+    return j + 1 >= j0 && j + 1 <= j1 && iw >= is - F && iw <= is + F && iw >= 2 - NX;
+}
+
 IFOS_KERNEL void k_gradient(GradArgs a)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -534,6 +547,7 @@ IFOS_KERNEL void k_gradient(GradArgs a)
     const float lamss = (float)((double)(rh * pv * pv) - 2.0 * (double)muss);
     float gvp = 0.0f, gvs = 0.0f, grho = 0.0f;
     float tvp = 0.0f, tvs = 0.0f, trho = 0.0f;
+    const int ci = a.tap_m ? (int)(e % a.NX) + 1 : 0, cj = a.tap_m ? (int)(e / a.NX) + 1 : 0;
     if (a.s_vp) {
         tvp = a.s_vp[e];
         tvs = a.s_vs[e];
@@ -563,6 +577,19 @@ IFOS_KERNEL void k_gradient(GradArgs a)
             } else {
                 a.we_sum[e] = (float)((double)a.we_sum[e] + 1.0 / (double)We);
             }
+        }
+        // circular taper around the shot's source, then the zeroed box
+        // (src/IFOS2D.c:2679-2689, src/taper_grad_shot.c:146-187)
+        if (a.tap_m) {
+            const int is = a.tap_ij[2 * s], js = a.tap_ij[2 * s + 1];
+            const int tx = ci - is + a.tap_c, ty = cj - js + a.tap_c;
+            const float v = (tx >= 1 && tx <= a.tap_n && ty >= 1 && ty <= a.tap_n)
+                                ? a.tap_m[(ty - 1) * a.tap_n + (tx - 1)] : 1.0f;
+            const float t = (v - a.tap_lohi[2 * s]) / a.tap_lohi[2 * s + 1];
+            svp *= t;
+            svs *= t;
+            srho *= t;
+            if (in_box(ci, cj, is, js, a.tap_fs, a.NX, a.NY)) svp = svs = srho = 0.0f;
         }
         gvp = s ? gvp + svp : svp;
         gvs = s ? gvs + svs : svs;

@@ -225,6 +225,9 @@ struct ifos_opt {
     hipStream_t stream = nullptr;
     double *partial = nullptr;  // [kRedBlocks] then one result slot
     float *fpart = nullptr;     // [kRedBlocks] then scratch for the step length
+    float *work = nullptr;      // gradient post-processing: taper coefficients / smoothing input
+    unsigned char *mask = nullptr;
+    size_t work_cap = 0, mask_cap = 0;
 };
 
 extern "C" int ifos_opt_create(int device, ifos_opt **out)
@@ -250,6 +253,8 @@ extern "C" void ifos_opt_destroy(ifos_opt *o)
     if (o->stream) hipStreamSynchronize(o->stream);
     hipFree(o->partial);
     hipFree(o->fpart);
+    hipFree(o->work);
+    hipFree(o->mask);
     if (o->stream) hipStreamDestroy(o->stream);
     delete o;
 }
@@ -446,6 +451,130 @@ extern "C" int ifos_opt_lbfgs(ifos_opt *o, long ncell, int npar, float *const *g
                            -avg[k], ncell);
         hipLaunchKernelGGL(k_scale_into, dim3(nb), dim3(256), 0, o->stream, grad[k], r + k * ncell, avg[k], ncell);
     }
+    OPT_TRY(hipGetLastError());
+    OPT_TRY(hipStreamSynchronize(o->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Gradient post-processing: tapers (coefficients built on the host,
+// ifos_taper_coeff) and the smoothing of smooth() (src/smooth.c:24).
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ __launch_bounds__(256) void k_taper(float *g, const float *coef, const unsigned char *zero, long n)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float v = g[e] * coef[e];
+    g[e] = (zero && zero[e]) ? 0.0f : v;
+}
+
+// order-preserving map of a float's bits to an unsigned key
+__device__ __forceinline__ unsigned int fkey(float v)
+{
+    const unsigned int u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float funkey(unsigned int k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+constexpr int kSmTile = 16;        // outputs per workgroup side
+constexpr int kSmMaxHalf = 56;     // (16 + 2*56)^2 keys = 64 KiB of LDS
+
+// One workgroup: a 16 x 16 block of outputs.  The (16+2h)^2 window union
+// is loaded once into LDS as keys, with the grid's edge values repeated
+// outside (src/smooth.c:87-104).  Each thread then selects element
+// rank = k/2 + 1 of its window with the (+h,+h) corner taken as +0, the
+// set median2d sorts (src/median2D.c:9-16, quicksort over t[0..k-1]), by a
+// 32-step bitwise search: the largest key x with #{key < x} <= rank.
+__global__ __launch_bounds__(256) void k_smooth(const float *in, float *out, int NX, int NY, int h)
+{
+    extern __shared__ unsigned int keys[];
+    const int W = kSmTile + 2 * h;
+    const int x0 = blockIdx.x * kSmTile, y0 = blockIdx.y * kSmTile;
+    for (int q = threadIdx.x; q < W * W; q += blockDim.x) {
+        const int ty = q / W, tx = q - ty * W;
+        const int x = min(max(x0 + tx - h, 0), NX - 1), y = min(max(y0 + ty - h, 0), NY - 1);
+        keys[q] = fkey(in[(long)y * NX + x]);
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % kSmTile, ly = threadIdx.x / kSmTile;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= NX || y >= NY) return;
+    const int side = 2 * h + 1, rank = side * side / 2 + 1;
+    const unsigned int corner = keys[(ly + 2 * h) * W + lx + 2 * h], zero = fkey(0.0f);
+    unsigned int prefix = 0;
+    for (int bit = 31; bit >= 0; bit--) {
+        const unsigned int cand = prefix | (1u << bit);
+        int cnt = (int)(zero < cand) - (int)(corner < cand);
+        for (int r = 0; r < side; r++) {
+            const unsigned int *row = keys + (ly + r) * W + lx;
+            for (int c = 0; c < side; c++) cnt += row[c] < cand;
+        }
+        if (cnt <= rank) prefix = cand;
+    }
+    out[(long)y * NX + x] = funkey(prefix);
+}
+
+int ensure_work(ifos_opt *o, size_t floats, size_t bytes)
+{
+    if (floats > o->work_cap) {
+        hipFree(o->work);
+        o->work = nullptr;
+        o->work_cap = 0;
+        OPT_TRY(hipMalloc(&o->work, sizeof(float) * floats));
+        o->work_cap = floats;
+    }
+    if (bytes > o->mask_cap) {
+        hipFree(o->mask);
+        o->mask = nullptr;
+        o->mask_cap = 0;
+        OPT_TRY(hipMalloc(&o->mask, bytes));
+        o->mask_cap = bytes;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int ifos_opt_taper(ifos_opt *o, long n, const float *coef, const unsigned char *zero, float *grad)
+{
+    OPT_TRY(hipSetDevice(o->device));
+    if (n < 0 || !coef || !grad) return ifos_set_error("ifos_opt_taper: bad arguments"), -2;
+    if (n == 0) return 0;
+    if (ensure_work(o, (size_t)n, zero ? (size_t)n : 0)) return -1;
+    OPT_TRY(hipMemcpyAsync(o->work, coef, sizeof(float) * n, hipMemcpyHostToDevice, o->stream));
+    if (zero) OPT_TRY(hipMemcpyAsync(o->mask, zero, (size_t)n, hipMemcpyHostToDevice, o->stream));
+    hipLaunchKernelGGL(k_taper, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, o->stream, grad, o->work,
+                       zero ? o->mask : (const unsigned char *)nullptr, n);
+    OPT_TRY(hipGetLastError());
+    OPT_TRY(hipStreamSynchronize(o->stream));
+    return 0;
+}
+
+extern "C" int ifos_opt_smooth(ifos_opt *o, int NX, int NY, int filtsize, float *grad, int *side)
+{
+    OPT_TRY(hipSetDevice(o->device));
+    if (NX < 1 || NY < 1 || !grad) return ifos_set_error("ifos_opt_smooth: bad arguments"), -2;
+    if (side) *side = 0;
+    if (filtsize == 0) return 0;
+    if (!(filtsize % 2)) filtsize += filtsize > 0 ? 1 : -1;
+    const int s = filtsize < 0 ? -filtsize : filtsize, h = s / 2;
+    if (h > kSmMaxHalf)
+        return ifos_set_error("ifos_opt_smooth: window side %d exceeds %d", s, 2 * kSmMaxHalf + 1), -2;
+    if (side) *side = s;
+    if (h == 0) return 0;  // a 1 x 1 window returns each value itself
+    const size_t n = (size_t)NX * NY;
+    if (ensure_work(o, n, 0)) return -1;
+    OPT_TRY(hipMemcpyAsync(o->work, grad, sizeof(float) * n, hipMemcpyDeviceToDevice, o->stream));
+    const int W = kSmTile + 2 * h;
+    dim3 grid((NX + kSmTile - 1) / kSmTile, (NY + kSmTile - 1) / kSmTile);
+    hipLaunchKernelGGL(k_smooth, grid, dim3(kSmTile * kSmTile), sizeof(unsigned int) * W * W, o->stream, o->work,
+                       grad, NX, NY, h);
     OPT_TRY(hipGetLastError());
     OPT_TRY(hipStreamSynchronize(o->stream));
     return 0;

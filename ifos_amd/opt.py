@@ -1,6 +1,7 @@
 """ctypes mirror of include/ifos2d_opt.h: the model-update side of an FWI
 iteration on the device (reductions, PCG direction, step-length parabola,
-bounded model update, L-BFGS).  Arrays are DeviceBuffer objects holding
+bounded model update, L-BFGS) and the gradient post-processing (tapers,
+smoothing).  Arrays are DeviceBuffer objects holding
 interior [NY][NX] float32 grids; the compute runs in the gfx950 library."""
 import ctypes as C
 
@@ -37,6 +38,35 @@ def _p(b):
     return _VOIDP(b.ptr if isinstance(b, DeviceBuffer) else b)
 
 
+TAPER_VERTICAL, TAPER_HORIZONTAL, TAPER_SOURCES = 1, 2, 3
+
+
+class Taper(C.Structure):
+    """ifos_taper: a whole-gradient taper of taper_grad (src/taper_grad.c:28)."""
+    _fields_ = [("kind", C.c_int), ("GRADT1", C.c_int), ("GRADT2", C.c_int), ("GRADT3", C.c_int),
+                ("GRADT4", C.c_int), ("DH", C.c_float), ("SRTRADIUS", C.c_float), ("SRTSHAPE", C.c_int),
+                ("FILTSIZE", C.c_int), ("nsrc", C.c_int), ("srcpos", C.POINTER(C.c_int)), ("ntr", C.c_int),
+                ("recpos", C.POINTER(C.c_int))]
+
+
+def taper_coeff(kind, NX, NY, gradt=(0, 0, 0, 0), DH=1.0, SRTRADIUS=0.0, SRTSHAPE=1, FILTSIZE=0, srcpos=None,
+                recpos=None):
+    """Host: coefficients [NY][NX] and zero mask of a taper (ifos_taper_coeff).
+    srcpos / recpos: [n][2] grid indices."""
+    L = lib()
+    L.ifos_taper_coeff.argtypes = [C.POINTER(Taper), C.c_int, C.c_int, C.POINTER(C.c_float),
+                                   C.POINTER(C.c_ubyte)]
+    sp = np.ascontiguousarray(srcpos if srcpos is not None else np.zeros((0, 2)), np.int32).reshape(-1, 2)
+    rp = np.ascontiguousarray(recpos if recpos is not None else np.zeros((0, 2)), np.int32).reshape(-1, 2)
+    t = Taper(kind, *[int(v) for v in gradt], DH, SRTRADIUS, SRTSHAPE, FILTSIZE, len(sp),
+              sp.ctypes.data_as(C.POINTER(C.c_int)), len(rp), rp.ctypes.data_as(C.POINTER(C.c_int)))
+    coef = np.empty((NY, NX), np.float32)
+    zero = np.empty((NY, NX), np.uint8)
+    _check(L.ifos_taper_coeff(C.byref(t), NX, NY, coef.ctypes.data_as(C.POINTER(C.c_float)),
+                              zero.ctypes.data_as(C.POINTER(C.c_ubyte))))
+    return coef, zero
+
+
 class Optimizer:
     def __init__(self, device=0):
         L = lib()
@@ -54,6 +84,8 @@ class Optimizer:
                                              C.POINTER(C.c_float), _VOIDP, C.c_int, C.c_int]
         L.ifos_opt_lbfgs.argtypes = [_VOIDP, C.c_long, C.c_int, C.POINTER(_VOIDP), C.POINTER(C.c_float), _VOIDP,
                                      _VOIDP, C.POINTER(C.c_double), _VOIDP, C.c_int, C.c_int, C.c_int]
+        L.ifos_opt_taper.argtypes = [_VOIDP, C.c_long, C.POINTER(C.c_float), C.POINTER(C.c_ubyte), _VOIDP]
+        L.ifos_opt_smooth.argtypes = [_VOIDP, C.c_int, C.c_int, C.c_int, _VOIDP, C.POINTER(C.c_int)]
         h = _VOIDP()
         _check(L.ifos_opt_create(device, C.byref(h)))
         self.h = h
@@ -112,3 +144,17 @@ class Optimizer:
         av = (C.c_float * npar)(*[float(v) for v in avg])
         r = rho_hist.ctypes.data_as(C.POINTER(C.c_double))
         _check(lib().ifos_opt_lbfgs(self.h, ncell, npar, g, av, _p(s_hist), _p(y_hist), r, _p(work), N, it, start))
+
+    def taper(self, grad, coef, zero=None):
+        """grad (device) = zero ? 0 : grad * coef (coef, zero host arrays)."""
+        coef = np.ascontiguousarray(coef, np.float32)
+        z = None if zero is None else np.ascontiguousarray(zero, np.uint8)
+        _check(lib().ifos_opt_taper(self.h, coef.size, coef.ctypes.data_as(C.POINTER(C.c_float)),
+                                    None if z is None else z.ctypes.data_as(C.POINTER(C.c_ubyte)), _p(grad)))
+
+    def smooth(self, NX, NY, filtsize, grad):
+        """smooth() of src/smooth.c on a device gradient, in place; returns
+        the window side used."""
+        side = C.c_int()
+        _check(lib().ifos_opt_smooth(self.h, NX, NY, filtsize, _p(grad), C.byref(side)))
+        return side.value

@@ -156,6 +156,15 @@ int  ifos_engine_apply_eprecond(ifos_engine *e);
 /* which = 0: We of the first slot's shot of the last adjoint run; 1: the
  * sum of 1/We since the last apply.  [NY][NX]. */
 int  ifos_engine_energy_weight(ifos_engine *e, int which, float *out);
+/* Circular taper of each shot's gradient around its source
+ * (SWS_TAPER_CIRCULAR_PER_SHOT, src/IFOS2D.c:2679-2689 calling
+ * taper_grad_shot sws 1, src/taper_grad_shot.c:57-196): erf (SRTSHAPE 1) or
+ * log (2) profile of radius SRTRADIUS metres, normalised over the grid, and
+ * zero in the (2 FILTSIZE+1)^2 box around the source.  Applied after the
+ * energy preconditioning and before the shot is added to the sum, to all
+ * three gradients.  The taper centre is the shot's first source (the
+ * reference's srcpos[.][ishot]).  on = 0 turns it off. */
+int  ifos_engine_set_shot_taper(ifos_engine *e, int on, float SRTRADIUS, int SRTSHAPE, int FILTSIZE);
 /* Storage of the forward snapshots (src/IFOS2D.c:1805-1871 keeps them FP32):
  * 0 = FP32 (default; gradients bit-identical to the reference), 1 = BF16,
  * 2 = FP16.  16-bit snapshots hold value / 2^e rounded to nearest-even, with

@@ -70,6 +70,54 @@ int ifos_opt_lbfgs_store_s(ifos_opt *o, long ncell, int npar, const float *const
 int ifos_opt_lbfgs(ifos_opt *o, long ncell, int npar, float *const *grad, const float *avg, float *s_hist,
                    float *y_hist, double *rho_hist, float *work, int N_LBFGS, int iteration, int lbfgs_iter_start);
 
+/* ---- gradient post-processing ------------------------------------------ */
+
+/* Whole-gradient tapers of taper_grad (src/taper_grad.c:28), applied by
+ * the reference inside PCG before the scaling (src/PCG.c:64-74):
+ *   IFOS_TAPER_VERTICAL   sws 1: Gaussian flanks between columns GRADT1 and
+ *                         GRADT4, zero at the sources and receivers;
+ *   IFOS_TAPER_HORIZONTAL sws 2: zero above row GRADT1, Gaussian flank to
+ *                         row GRADT2;
+ *   IFOS_TAPER_SOURCES    sws 3: erf (SRTSHAPE 1) or log (2) taper of radius
+ *                         SRTRADIUS around every source, zero in the
+ *                         (2 FILTSIZE+1)^2 box around it.
+ * The per-shot form of the source taper (taper_grad_shot sws 1) is
+ * ifos_engine_set_shot_taper.  A taper read from a file (sws 4-6) is a
+ * coefficient grid as ifos_model_read returns it. */
+enum { IFOS_TAPER_VERTICAL = 1, IFOS_TAPER_HORIZONTAL = 2, IFOS_TAPER_SOURCES = 3 };
+typedef struct ifos_taper {
+    int kind;
+    int GRADT1, GRADT2, GRADT3, GRADT4;
+    float DH, SRTRADIUS;
+    int SRTSHAPE, FILTSIZE;
+    int nsrc;
+    const int *srcpos;   /* [nsrc][2] grid indices iround(x/DH), iround(y/DH) */
+    int ntr;
+    const int *recpos;   /* [ntr][2] grid indices */
+} ifos_taper;
+/* Host: the taper's coefficients coef [NY][NX] and (zero != NULL) the
+ * cells it sets to zero afterwards (zero[e] = 1), including the cells the
+ * reference's box reaches through its scratch matrix's row wrap when a box
+ * extends two or more columns left of the grid. */
+int ifos_taper_coeff(const ifos_taper *t, int NX, int NY, float *coef, unsigned char *zero);
+/* Device: grad[e] = zero[e] ? 0 : grad[e] * coef[e] for n cells; coef and
+ * zero (may be NULL) are host arrays. */
+int ifos_opt_taper(ifos_opt *o, long n, const float *coef, const unsigned char *zero, float *grad);
+/* Device, in place: smooth() (src/smooth.c:24) with filter size filtsize
+ * (FILT_SIZE_GRAD or FILT_SIZE; an even size grows by one away from zero;
+ * 0 does nothing), edges extended by replication.  The value is what
+ * median2d (src/median2D.c:3) returns: its sort covers the zero at t[0] and
+ * the window without its last element, and it picks element k/2+1 of k.
+ * *side receives the window side used.  Window sides up to 113. */
+int ifos_opt_smooth(ifos_opt *o, int NX, int NY, int filtsize, float *grad, int *side);
+
+/* Source taper table helpers (host): table side n and entries m[n][n]
+ * (returns n; fills m when cap >= n*n), the table cell that sits on the
+ * source (1-based, ceil(n/2)), and one shot's normalisation lo, hi. */
+int  ifos_srctaper_table(float DH, float SRTRADIUS, int SRTSHAPE, float *m, int cap);
+int  ifos_srctaper_centre(int n);
+void ifos_srctaper_norm(int NX, int NY, int n, const float *m, int is, int js, float *lo, float *hi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -540,3 +540,95 @@ void ref_mat_change(int nx, int ny, const int *ip, const float *fp, const float 
     eps[2] = EPSILON_rho;
     for (int k = 0; k < 12; k++) free_matrix(a[k], 1, ny, 1, nx);
 }
+
+/* ---------------------------------------------------------------------------
+ * Gradient post-processing of the reference, called unmodified:
+ * taper_grad_shot (src/taper_grad_shot.c:29), taper_grad
+ * (src/taper_grad.c:28) and smooth (src/smooth.c:24).  Gradients are
+ * interior [NY][NX]; source positions in metres (srcpos[1..2][n]),
+ * receivers as grid indices [ntr][2].  taper_grad writes
+ * taper_coeff_*.bin into the working directory, as the reference does.
+ * ------------------------------------------------------------------------- */
+static float **harness_grid_in(int nx, int ny, const float *g)
+{
+    const int nd = 3;
+    float **a = matrix(-nd + 1, ny + nd, -nd + 1, nx + nd);
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) a[j][i] = g[(size_t)(j - 1) * nx + i - 1];
+    return a;
+}
+
+static void harness_grid_out(int nx, int ny, float **a, float *g)
+{
+    const int nd = 3;
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) g[(size_t)(j - 1) * nx + i - 1] = a[j][i];
+    free_matrix(a, -nd + 1, ny + nd, -nd + 1, nx + nd);
+}
+
+static float **harness_srcpos(int nshots, const float *xs, const float *ys)
+{
+    float **s = matrix(1, 8, 1, nshots);
+    for (int n = 1; n <= nshots; n++) {
+        s[1][n] = xs[n - 1];
+        s[2][n] = ys[n - 1];
+    }
+    return s;
+}
+
+/* taper_grad_shot(..., ishot, sws) for one gradient; sws 1 (circular) */
+void ref_taper_grad_shot(int nx, int ny, float dh, float srtradius, int srtshape, int filtsize, int nshots,
+                         const float *xs, const float *ys, int ishot, float *g)
+{
+    harness_setup_opt(nx, ny);
+    DH = dh; SRTRADIUS = srtradius; SRTSHAPE = srtshape; FILTSIZE = filtsize;
+    USE_WORKFLOW = 0;
+    float **w = harness_grid_in(nx, ny, g);
+    /* taper_coeff is matrix(1,NY,1,NX) in src/IFOS2D.c:645; a wider one
+     * here keeps the box writes left of column 1 inside the allocation */
+    float **tc = matrix(-2, ny + 3, -2 * nx, nx + 3);
+    float **sp = harness_srcpos(nshots, xs, ys);
+    int **rp = imatrix(1, 3, 1, 1);
+    taper_grad_shot(w, tc, sp, nshots, rp, 1, ishot, 1);
+    harness_grid_out(nx, ny, w, g);
+    free_matrix(tc, -2, ny + 3, -2 * nx, nx + 3);
+    free_matrix(sp, 1, 8, 1, nshots);
+    free_imatrix(rp, 1, 3, 1, 1);
+}
+
+/* taper_grad(..., sws) for sws 1 (vertical), 2 (horizontal), 3 (sources) */
+void ref_taper_grad(int nx, int ny, float dh, const int *gradt, float srtradius, int srtshape, int filtsize,
+                    int nshots, const float *xs, const float *ys, int ntr, const int *recpos, int sws, float *g)
+{
+    harness_setup_opt(nx, ny);
+    DH = dh; SRTRADIUS = srtradius; SRTSHAPE = srtshape; FILTSIZE = filtsize;
+    GRADT1 = gradt[0]; GRADT2 = gradt[1]; GRADT3 = gradt[2]; GRADT4 = gradt[3];
+    USE_WORKFLOW = 0;
+    float **w = harness_grid_in(nx, ny, g);
+    float **tc = matrix(-2, ny + 3, -2 * nx, nx + 3);
+    float **sp = harness_srcpos(nshots, xs, ys);
+    int **rp = imatrix(1, 3, 1, ntr > 0 ? ntr : 1);
+    for (int n = 1; n <= ntr; n++) {
+        rp[1][n] = recpos[2 * (n - 1)];
+        rp[2][n] = recpos[2 * (n - 1) + 1];
+    }
+    taper_grad(w, tc, sp, nshots, rp, ntr, sws);
+    harness_grid_out(nx, ny, w, g);
+    free_matrix(tc, -2, ny + 3, -2 * nx, nx + 3);
+    free_matrix(sp, 1, 8, 1, nshots);
+    free_imatrix(rp, 1, 3, 1, ntr > 0 ? ntr : 1);
+}
+
+/* smooth(mat, 1, 1, ...) with FILT_SIZE_GRAD = filtsize; returns the
+ * FILT_SIZE_GRAD the call leaves behind (an even size grows by one) */
+int ref_smooth(int nx, int ny, int filtsize, float *g)
+{
+    harness_setup_opt(nx, ny);
+    FILT_SIZE_GRAD = filtsize;
+    GRAD_FILT_WAVELENGTH = 0;
+    TIME_FILT = 0;
+    float **w = harness_grid_in(nx, ny, g);
+    smooth(w, 1, 1, 1.0f, 1.0f);
+    harness_grid_out(nx, ny, w, g);
+    return FILT_SIZE_GRAD;
+}

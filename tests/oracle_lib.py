@@ -325,3 +325,75 @@ def ref_gradient_we(p, model, srcpos, recpos, diff_vx, diff_vy):
                                C.c_int(p.EPRECOND_PER_SHOT), _ptr(we))
     assert rc == 0, rc
     return dict(grad_vp=g[0], grad_vs=g[1], grad_rho=g[2]), we
+
+
+# ---------------------------------------------------------------------------
+# gradient post-processing (oracle/post_oracle.c; reference taper_grad_shot,
+# taper_grad, smooth through oracle/_ref)
+def taper_sources(g, DH, SRTRADIUS, SRTSHAPE, FILTSIZE, isrc, jsrc, which="oracle", ishot=1):
+    """Circular source taper + box zeroing.  isrc/jsrc: grid indices of all
+    shots.  which="ref" with ishot > 0 calls taper_grad_shot(..., ishot, 1);
+    ishot = 0 calls taper_grad(..., 3) over all shots."""
+    ny, nx = g.shape
+    out = _f32(g).copy()
+    isrc = np.ascontiguousarray(isrc, np.int32)
+    jsrc = np.ascontiguousarray(jsrc, np.int32)
+    if which == "ref":
+        # positions in metres that iround(x/DH) maps back to the indices
+        xs = _f32(isrc * np.float32(DH))
+        ys = _f32(jsrc * np.float32(DH))
+        if ishot > 0:
+            ref().ref_taper_grad_shot(nx, ny, C.c_float(DH), C.c_float(SRTRADIUS), SRTSHAPE, FILTSIZE, len(isrc),
+                                      _ptr(xs), _ptr(ys), ishot, _ptr(out))
+        else:
+            gt = np.zeros(4, np.int32)
+            ref().ref_taper_grad(nx, ny, C.c_float(DH), _ptr(gt, C.c_int), C.c_float(SRTRADIUS), SRTSHAPE,
+                                 FILTSIZE, len(isrc), _ptr(xs), _ptr(ys), 0, None, 3, _ptr(out))
+        return out
+    if ishot > 0:
+        isrc, jsrc = isrc[ishot - 1:ishot].copy(), jsrc[ishot - 1:ishot].copy()
+    oracle().orc_taper_sources(nx, ny, C.c_float(DH), C.c_float(SRTRADIUS), SRTSHAPE, FILTSIZE, len(isrc),
+                               _ptr(isrc, C.c_int), _ptr(jsrc, C.c_int), _ptr(out), None)
+    return out
+
+
+def taper_vertical(g, gradt, isrc, jsrc, recpos, DH=1.0, which="oracle"):
+    ny, nx = g.shape
+    out = _f32(g).copy()
+    isrc = np.ascontiguousarray(isrc, np.int32)
+    jsrc = np.ascontiguousarray(jsrc, np.int32)
+    rp = np.ascontiguousarray(recpos, np.int32).reshape(-1, 2)
+    if which == "ref":
+        gt = np.asarray(gradt, np.int32)
+        xs = _f32(isrc * np.float32(DH))
+        ys = _f32(jsrc * np.float32(DH))
+        ref().ref_taper_grad(nx, ny, C.c_float(DH), _ptr(gt, C.c_int), C.c_float(0), 1, 0, len(isrc), _ptr(xs),
+                             _ptr(ys), len(rp), _ptr(rp, C.c_int), 1, _ptr(out))
+        return out
+    oracle().orc_taper_vertical(nx, ny, *[int(v) for v in gradt], len(isrc), _ptr(isrc, C.c_int),
+                                _ptr(jsrc, C.c_int), len(rp), _ptr(rp, C.c_int), _ptr(out), None)
+    return out
+
+
+def taper_horizontal(g, gradt, which="oracle"):
+    ny, nx = g.shape
+    out = _f32(g).copy()
+    if which == "ref":
+        gt = np.asarray(gradt, np.int32)
+        one = np.ones(1, np.float32)
+        ref().ref_taper_grad(nx, ny, C.c_float(1.0), _ptr(gt, C.c_int), C.c_float(0), 1, 0, 1, _ptr(one),
+                             _ptr(one), 0, None, 2, _ptr(out))
+        return out
+    oracle().orc_taper_horizontal(nx, ny, int(gradt[0]), int(gradt[1]), _ptr(out), None)
+    return out
+
+
+def median_smooth(g, filtsize, which="oracle"):
+    ny, nx = g.shape
+    if which == "ref":
+        out = _f32(g).copy()
+        ref().ref_smooth(nx, ny, filtsize, _ptr(out))
+        return out
+    out = np.empty((ny, nx), np.float32)
+    oracle().orc_median_smooth(nx, ny, filtsize, _ptr(_f32(g)), _ptr(out))
+    return out
