@@ -156,7 +156,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--batch", type=int, default=1,
                     help="shots advanced together per kernel launch (1: the one-shot-at-a-time path)")
-    ap.add_argument("--no-autotune", action="store_true", help="batched runs: keep the default launch shape")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape (no timing sweep)")
     ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"],
                     help="snapshot storage (fp32 keeps the gradient bit-identical to the reference)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -204,6 +204,10 @@ def main():
     if args.kernels_only:
         eng.set_model(**w.start)
         eng.set_shots([(s.srcpos, s.recpos) for s in batches[0]])
+        if S > 1 and not args.no_autotune:
+            spb, grid, ms = eng.autotune(reps=5)
+            log(f"[bench] autotune: {spb} shots per workgroup, grid {grid or 'one per item'}, "
+                f"{'slot' if eng.get_work_order() else 'tile'}-major order, {ms:.4f} ms per step")
         for kern in (0, 1):
             ms = eng.time_kernel(kern, reps=20)
             gbs = eng.kernel_bytes_per_cell(kern) * cfg.NX * cfg.NY * S / (ms * 1e-3) / 1e9
@@ -225,11 +229,11 @@ def main():
     eng.reset_gradient()
     eng.set_snapshot_precision(args.snap)
     shape = None
-    if S > 1 and not args.no_autotune:
+    if not args.no_autotune:
         eng.set_shots([(s.srcpos, s.recpos) for s in batches[0]])
-        shape = eng.autotune(reps=5)
+        shape = eng.autotune(reps=5) + (eng.get_work_order(),)
         log(f"[bench] autotune: {shape[0]} shots per workgroup, grid {shape[1] or 'one per item'}, "
-            f"{shape[2]:.4f} ms per step")
+            f"{'slot' if shape[3] else 'tile'}-major order, {shape[2]:.4f} ms per step")
 
     def step(k):
         batch = batches[k]
@@ -289,7 +293,8 @@ def main():
         "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "dh": cfg.DH, "FDORDER": cfg.FDORDER, "L": cfg.L,
                    "NT": cfg.NT, "DTINV": cfg.DTINV, "snapshots": args.snap, "shots_per_step": S, "receivers": int(len(w.shots[0].recpos)),
                    "parallelism": "one GPU, one shot at a time" if S == 1 else
-                   f"one GPU, {S} shots per launch" + (f" ({shape[0]} per workgroup, grid {shape[1]})" if shape else "")},
+                   f"one GPU, {S} shots per launch" + (f" ({shape[0]} per workgroup, grid {shape[1]}, "
+                                                       f"{'slot' if shape[3] else 'tile'}-major)" if shape else "")},
         "gcell_updates_per_s": round(gcells, 3),
         "gradient_parity": ("bit-identical to the reference FP32 kernels (tests/test_gpu_parity.py)" if args.snap == "fp32"
                             else f"{args.snap} snapshots: bit-identical to the oracle with the same quantisation, "

@@ -215,6 +215,14 @@ class Engine:
         _check(lib().ifos_engine_get_launch(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def set_work_order(self, slot_major):
+        _check(lib().ifos_engine_set_work_order(self.h, int(bool(slot_major))))
+
+    def get_work_order(self):
+        v = C.c_int()
+        _check(lib().ifos_engine_get_work_order(self.h, C.byref(v)))
+        return v.value
+
     def autotune(self, reps=5):
         """Pick shots per workgroup and the persistent grid by timing; returns (spb, grid, ms per step)."""
         a, b, ms = C.c_int(), C.c_int(), C.c_double()

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -69,6 +70,29 @@ extern "C" int ifos_device_count(void)
 // ---------------------------------------------------------------------------
 // engine state
 // ---------------------------------------------------------------------------
+// Engines of one process may run in several threads (the request server).
+// Device allocation, release and graph capture of all engines are ordered
+// by one lock, so that no capture on one engine's stream overlaps another
+// thread's hipMalloc/hipFree.  Copies and fills go to each engine's own
+// non-blocking stream, never to the legacy stream, for the same reason.
+static std::mutex &device_lock()
+{
+    static std::mutex m;
+    return m;
+}
+template <typename T>
+static hipError_t dmalloc(T **p, size_t bytes)
+{
+    std::lock_guard<std::mutex> g(device_lock());
+    return hipMalloc(p, bytes);
+}
+static hipError_t dfree(void *p)
+{
+    if (!p) return hipSuccess;
+    std::lock_guard<std::mutex> g(device_lock());
+    return hipFree(p);
+}
+
 struct ifos_engine {
     ifos_config c;
     int device = 0;
@@ -231,11 +255,11 @@ static int engine_create(const ifos_config *cfg, int device, int slots, ifos_eng
     const int nwave = 10 + 3 * L;
     E->ws = (long)nwave * E->field;
     const size_t wbytes = sizeof(float) * (size_t)E->ws * slots;
-    if (hipMalloc(&E->slab, wbytes) != hipSuccess) {
+    if (dmalloc(&E->slab, wbytes) != hipSuccess) {
         ifos_set_error("device allocation of %d x %d wavefield grids failed", slots, nwave);
         return fail(-3);
     }
-    hipMemset(E->slab, 0, wbytes);
+    hipMemsetAsync(E->slab, 0, wbytes, E->stream);
     float *gp = E->slab + E->off;
     auto next = [&]() {
         float *r = gp;
@@ -251,11 +275,11 @@ static int engine_create(const ifos_config *cfg, int device, int slots, ifos_eng
     }
     // shared: 15 material grids + 3L coefficient planes
     const int nmat = 15 + 3 * L;
-    if (hipMalloc(&E->mslab, sizeof(float) * E->field * nmat) != hipSuccess) {
+    if (dmalloc(&E->mslab, sizeof(float) * E->field * nmat) != hipSuccess) {
         ifos_set_error("device allocation of %d material grids failed", nmat);
         return fail(-3);
     }
-    hipMemset(E->mslab, 0, sizeof(float) * E->field * nmat);
+    hipMemsetAsync(E->mslab, 0, sizeof(float) * E->field * nmat, E->stream);
     gp = E->mslab + E->off;
     E->rip = next(); E->rjp = next(); E->uipjp = next(); E->f = next(); E->g = next(); E->fipjp = next();
     E->rho = next(); E->vs = next(); E->vp = next(); E->taus = next(); E->taup = next();
@@ -268,12 +292,12 @@ static int engine_create(const ifos_config *cfg, int device, int slots, ifos_eng
 
     // interior arrays: 11 shared + 3 correlation planes per slot
     const int nint = 11;
-    if (hipMalloc(&E->interior, sizeof(float) * E->plane * nint) != hipSuccess ||
-        hipMalloc(&E->wcbuf, sizeof(float) * E->plane * 3 * slots) != hipSuccess) {
+    if (dmalloc(&E->interior, sizeof(float) * E->plane * nint) != hipSuccess ||
+        dmalloc(&E->wcbuf, sizeof(float) * E->plane * 3 * slots) != hipSuccess) {
         ifos_set_error("device allocation of interior arrays failed");
         return fail(-3);
     }
-    hipMemset(E->interior, 0, sizeof(float) * E->plane * nint);
+    hipMemsetAsync(E->interior, 0, sizeof(float) * E->plane * nint, E->stream);
     float **ia[nint] = {&E->muss, &E->lamss, &E->vs_i, &E->vp_i, &E->rho_i, &E->g_vp, &E->g_vs, &E->g_rho,
                         &E->s_vp, &E->s_vs, &E->s_rho};
     for (int k = 0; k < nint; k++) *ia[k] = E->interior + (long)k * E->plane;
@@ -282,22 +306,26 @@ static int engine_create(const ifos_config *cfg, int device, int slots, ifos_eng
     const int fw2 = c.FW > 0 ? 2 * c.FW : 1;
     std::vector<float> prof((size_t)12 * fw2, 0.0f);
     if (c.FW > 0) ifos_pml_profiles(&c, prof.data());
-    if (hipMalloc(&E->pml, sizeof(float) * prof.size()) != hipSuccess) return fail(-3);
-    hipMemcpy(E->pml, prof.data(), sizeof(float) * prof.size(), hipMemcpyHostToDevice);
+    if (dmalloc(&E->pml, sizeof(float) * prof.size()) != hipSuccess) return fail(-3);
+    hipMemcpyAsync(E->pml, prof.data(), sizeof(float) * prof.size(), hipMemcpyHostToDevice, E->stream);
     const size_t xs = (size_t)c.NY * fw2, ys = (size_t)fw2 * c.NX;
     E->pss = (long)(5 * xs + 4 * ys);
-    if (hipMalloc(&E->psi, sizeof(float) * E->pss * slots) != hipSuccess) return fail(-3);
-    hipMemset(E->psi, 0, sizeof(float) * E->pss * slots);
+    if (dmalloc(&E->psi, sizeof(float) * E->pss * slots) != hipSuccess) return fail(-3);
+    hipMemsetAsync(E->psi, 0, sizeof(float) * E->pss * slots, E->stream);
     E->psi_sxx_x = E->psi; E->psi_sxy_x = E->psi + xs; E->psi_vxx = E->psi + 2 * xs;
     E->psi_vyx = E->psi + 3 * xs; E->psi_vxxs = E->psi + 4 * xs;
     float *yb = E->psi + 5 * xs;
     E->psi_syy_y = yb; E->psi_sxy_y = yb + ys; E->psi_vyy = yb + 2 * ys; E->psi_vxy = yb + 3 * ys;
 
-    if (hipMalloc(&E->ctl, sizeof(int) * 4) != hipSuccess) return fail(-3);
-    hipMemset(E->ctl, 0, sizeof(int) * 4);
-    if (hipMalloc(&E->explosive, sizeof(int) * slots) != hipSuccess) return fail(-3);
-    hipMemset(E->explosive, 0, sizeof(int) * slots);
-    if (hipMalloc(&E->dargs, sizeof(StepArgs) * 4) != hipSuccess) return fail(-3);
+    if (dmalloc(&E->ctl, sizeof(int) * 4) != hipSuccess) return fail(-3);
+    hipMemsetAsync(E->ctl, 0, sizeof(int) * 4, E->stream);
+    if (dmalloc(&E->explosive, sizeof(int) * slots) != hipSuccess) return fail(-3);
+    hipMemsetAsync(E->explosive, 0, sizeof(int) * slots, E->stream);
+    if (dmalloc(&E->dargs, sizeof(StepArgs) * 4) != hipSuccess) return fail(-3);
+    if (hipStreamSynchronize(E->stream) != hipSuccess) {
+        ifos_set_error("engine set-up failed on the device");
+        return fail(-1);
+    }
     *out = E;
     return 0;
 }
@@ -327,12 +355,12 @@ extern "C" void ifos_engine_destroy(ifos_engine *E)
     hipSetDevice(E->device);
     if (E->stream) hipStreamSynchronize(E->stream);
     drop_graphs(E);
-    hipFree(E->slab); hipFree(E->mslab); hipFree(E->interior); hipFree(E->wcbuf); hipFree(E->pml);
-    hipFree(E->psi); hipFree(E->ctl); hipFree(E->explosive); hipFree(E->dargs);
-    hipFree(E->wbuf); hipFree(E->we_sum); hipFree(E->ep_geom); hipFree(E->we_max); hipFree(E->red);
-    hipFree(E->tap_m); hipFree(E->tap_ij); hipFree(E->tap_lohi);
-    hipFree(E->src_i); hipFree(E->src_d); hipFree(E->sig); hipFree(E->rec_i); hipFree(E->seis);
-    hipFree(E->trace_sums); hipFree(E->snap); hipFree(E->tiles);
+    dfree(E->slab); dfree(E->mslab); dfree(E->interior); dfree(E->wcbuf); dfree(E->pml);
+    dfree(E->psi); dfree(E->ctl); dfree(E->explosive); dfree(E->dargs);
+    dfree(E->wbuf); dfree(E->we_sum); dfree(E->ep_geom); dfree(E->we_max); dfree(E->red);
+    dfree(E->tap_m); dfree(E->tap_ij); dfree(E->tap_lohi);
+    dfree(E->src_i); dfree(E->src_d); dfree(E->sig); dfree(E->rec_i); dfree(E->seis);
+    dfree(E->trace_sums); dfree(E->snap); dfree(E->tiles);
     if (E->stream) hipStreamDestroy(E->stream);
     ifos_free_materials(&E->mat);
     delete E;
@@ -420,22 +448,22 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
     HIP_TRY(hipStreamSynchronize(E->stream));
     E->snap_valid = false;
     if (nsrc > E->cap_src) {
-        hipFree(E->src_i); hipFree(E->src_d); hipFree(E->sig);
+        dfree(E->src_i); dfree(E->src_d); dfree(E->sig);
         E->src_i = nullptr; E->src_d = nullptr; E->sig = nullptr;
         E->cap_src = nsrc;
-        HIP_TRY(hipMalloc(&E->src_i, sizeof(int) * 3 * nsrc));
-        HIP_TRY(hipMalloc(&E->src_d, sizeof(double) * 2 * nsrc));
-        HIP_TRY(hipMalloc(&E->sig, sizeof(float) * (size_t)nsrc * c.NT));
+        HIP_TRY(dmalloc(&E->src_i, sizeof(int) * 3 * nsrc));
+        HIP_TRY(dmalloc(&E->src_d, sizeof(double) * 2 * nsrc));
+        HIP_TRY(dmalloc(&E->sig, sizeof(float) * (size_t)nsrc * c.NT));
         drop_graphs(E);
     }
     if (ntr > E->cap_tr) {
-        hipFree(E->rec_i); hipFree(E->seis); hipFree(E->trace_sums);
+        dfree(E->rec_i); dfree(E->seis); dfree(E->trace_sums);
         E->rec_i = nullptr; E->seis = nullptr; E->trace_sums = nullptr;
         E->cap_tr = ntr;
-        HIP_TRY(hipMalloc(&E->rec_i, sizeof(int) * 3 * ntr));
-        HIP_TRY(hipMalloc(&E->seis, sizeof(float) * 6 * (size_t)ntr * c.NT));
-        HIP_TRY(hipMalloc(&E->trace_sums, sizeof(double) * 2 * ntr));
-        HIP_TRY(hipMemset(E->seis, 0, sizeof(float) * 6 * (size_t)ntr * c.NT));
+        HIP_TRY(dmalloc(&E->rec_i, sizeof(int) * 3 * ntr));
+        HIP_TRY(dmalloc(&E->seis, sizeof(float) * 6 * (size_t)ntr * c.NT));
+        HIP_TRY(dmalloc(&E->trace_sums, sizeof(double) * 2 * ntr));
+        HIP_TRY(hipMemsetAsync(E->seis, 0, sizeof(float) * 6 * (size_t)ntr * c.NT, E->stream));
         drop_graphs(E);
     }
     if (nsrc != E->nsrc || ntr != E->ntr || nshots != E->nslots) drop_graphs(E);
@@ -457,7 +485,8 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
     std::vector<int> expl(nshots, 0);
     for (int k = 0; k < nshots; k++)
         expl[k] = nsrc_of[k] ? ((int)srcpos[(size_t)(so[k + 1] - 1) * IFOS_SRC_STRIDE + 7]) == 1 : c.SOURCE_TYPE == 1;
-    HIP_TRY(hipMemcpy(E->explosive, expl.data(), sizeof(int) * nshots, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(E->explosive, expl.data(), sizeof(int) * nshots, hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
     if (nsrc) {
         std::vector<int> si(3 * (size_t)E->cap_src, 0);
         std::vector<double> sd(2 * (size_t)E->cap_src, 0.0);
@@ -472,9 +501,10 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
         }
         std::vector<float> w((size_t)nsrc * c.NT);
         ifos_wavelet(&c, nsrc, srcpos, w.data());
-        HIP_TRY(hipMemcpy(E->src_i, si.data(), sizeof(int) * si.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(E->src_d, sd.data(), sizeof(double) * sd.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(E->sig, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpyAsync(E->src_i, si.data(), sizeof(int) * si.size(), hipMemcpyHostToDevice, E->stream));
+        HIP_TRY(hipMemcpyAsync(E->src_d, sd.data(), sizeof(double) * sd.size(), hipMemcpyHostToDevice, E->stream));
+        HIP_TRY(hipMemcpyAsync(E->sig, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice, E->stream));
+        HIP_TRY(hipStreamSynchronize(E->stream));
     }
     if (ntr) {
         std::vector<int> ri(3 * (size_t)E->cap_tr, 0);
@@ -482,7 +512,8 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
             ri[t] = recpos[2 * t];
             ri[E->cap_tr + t] = recpos[2 * t + 1];
         }
-        HIP_TRY(hipMemcpy(E->rec_i, ri.data(), sizeof(int) * ri.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpyAsync(E->rec_i, ri.data(), sizeof(int) * ri.size(), hipMemcpyHostToDevice, E->stream));
+        HIP_TRY(hipStreamSynchronize(E->stream));
     }
     // bucket each slot's sources and receivers by 64x8 tile, keeping their order
     const int nblk = E->nbx * E->nby;
@@ -520,13 +551,14 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
     E->off_rec_idx = (int)blob.size();
     blob.insert(blob.end(), r_idx.begin(), r_idx.end());
     if (blob.size() > E->tiles_cap) {
-        hipFree(E->tiles);
+        dfree(E->tiles);
         E->tiles = nullptr;
         E->tiles_cap = blob.size() * 2;
-        HIP_TRY(hipMalloc(&E->tiles, sizeof(int) * E->tiles_cap));
+        HIP_TRY(dmalloc(&E->tiles, sizeof(int) * E->tiles_cap));
         drop_graphs(E);
     }
-    HIP_TRY(hipMemcpy(E->tiles, blob.data(), sizeof(int) * blob.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(E->tiles, blob.data(), sizeof(int) * blob.size(), hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
     return 0;
 }
 
@@ -659,7 +691,8 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
     int unit = c.DTINV;
     while (unit < 32 && NT % (unit * 2) == 0) unit *= 2;
     const int kind = sw;
-    const int key[6] = {E->nsrc, E->ntr, snapshots ? 1 : 0, E->nslots, E->shape.spb * 65536 + E->shape.grid, unit};
+    const int key[6] = {E->nsrc, E->ntr, snapshots ? 1 : 0, E->nslots,
+                        (E->shape.order * 8 + E->shape.spb) * 65536 + E->shape.grid, unit};
     const LaunchShape ls = E->shape;
     StepArgs a = shaped_args(step_args(E, sw, snapshots), ls);
     const StepArgs *da = nullptr;
@@ -669,6 +702,7 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
             if (E->gexec[kind]) hipGraphExecDestroy(E->gexec[kind]);
             E->gexec[kind] = nullptr;
             hipGraph_t graph;
+            std::lock_guard<std::mutex> capture_guard(device_lock());
             HIP_TRY(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
             for (int k = 0; k < unit; k++) {
                 launch_vel_step(a, da, ls, k, E->stream);
@@ -719,10 +753,10 @@ static int ensure_snapshots(ifos_engine *E)
 {
     const size_t need = snapshot_bytes(E->c, E->snap_prec) * E->slots + 256;
     if (E->snap && E->snap_bytes >= need) return 0;
-    hipFree(E->snap);
+    dfree(E->snap);
     E->snap = nullptr;
     drop_graphs(E);
-    if (hipMalloc(&E->snap, need) != hipSuccess) {
+    if (dmalloc(&E->snap, need) != hipSuccess) {
         ifos_set_error("cannot allocate %.2f GB of snapshot storage", need / 1e9);
         return -3;
     }
@@ -945,15 +979,15 @@ extern "C" int ifos_engine_set_eprecond(ifos_engine *E, int mode, float epsilon_
     E->ep_mode = mode;
     if (!mode || E->wbuf) return 0;
     const size_t n = (size_t)E->plane;
-    if (hipMalloc(&E->wbuf, sizeof(float) * 2 * n * E->slots) != hipSuccess ||
-        hipMalloc(&E->we_sum, sizeof(float) * n) != hipSuccess ||
-        hipMalloc(&E->ep_geom, sizeof(double) * n) != hipSuccess ||
-        hipMalloc(&E->we_max, sizeof(float) * E->slots) != hipSuccess ||
-        hipMalloc(&E->red, sizeof(float) * kMaxParts) != hipSuccess) {
+    if (dmalloc(&E->wbuf, sizeof(float) * 2 * n * E->slots) != hipSuccess ||
+        dmalloc(&E->we_sum, sizeof(float) * n) != hipSuccess ||
+        dmalloc(&E->ep_geom, sizeof(double) * n) != hipSuccess ||
+        dmalloc(&E->we_max, sizeof(float) * E->slots) != hipSuccess ||
+        dmalloc(&E->red, sizeof(float) * kMaxParts) != hipSuccess) {
         E->ep_mode = 0;
         return ifos_set_error("device allocation for the energy preconditioner failed"), -3;
     }
-    HIP_TRY(hipMemset(E->we_sum, 0, sizeof(float) * n));
+    HIP_TRY(hipMemsetAsync(E->we_sum, 0, sizeof(float) * n, E->stream));
     // geometric receiver term of mode 3 (src/eprecond1.c:46-48), FP64 on the host
     const ifos_config &c = E->c;
     std::vector<double> g(n);
@@ -963,7 +997,8 @@ extern "C" int ifos_engine_set_eprecond(ifos_engine *E, int mode, float epsilon_
             const float x = i * c.DH, y = j * c.DH;
             g[(size_t)(j - 1) * c.NX + (i - 1)] = asinh((xmax - x) / y) - asinh((xmin - x) / y);
         }
-    HIP_TRY(hipMemcpy(E->ep_geom, g.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(E->ep_geom, g.data(), sizeof(double) * n, hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
     return 0;
 }
 
@@ -985,7 +1020,9 @@ extern "C" int ifos_engine_energy_weight(ifos_engine *E, int which, float *out)
 {
     HIP_TRY(hipSetDevice(E->device));
     if (!E->ep_mode) return ifos_set_error("energy preconditioning is off"), -2;
-    HIP_TRY(hipMemcpy(out, which ? E->we_sum : E->wbuf, sizeof(float) * E->plane, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(out, which ? E->we_sum : E->wbuf, sizeof(float) * E->plane, hipMemcpyDeviceToHost,
+                           E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
     return 0;
 }
 
@@ -999,8 +1036,11 @@ static int shot_taper_args(ifos_engine *E, GradArgs &ga)
         if (!is) return ifos_set_error("the source taper needs a source in every shot (shot %d has none)", k + 1), -2;
         ifos_srctaper_norm(c.NX, c.NY, E->tap_n, E->tap_host.data(), is, js, &lohi[2 * k], &lohi[2 * k + 1]);
     }
-    HIP_TRY(hipMemcpy(E->tap_ij, E->first_src.data(), sizeof(int) * 2 * E->nslots, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(E->tap_lohi, lohi.data(), sizeof(float) * 2 * E->nslots, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(E->tap_ij, E->first_src.data(), sizeof(int) * 2 * E->nslots, hipMemcpyHostToDevice,
+                           E->stream));
+    HIP_TRY(hipMemcpyAsync(E->tap_lohi, lohi.data(), sizeof(float) * 2 * E->nslots, hipMemcpyHostToDevice,
+                           E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
     ga.tap_m = E->tap_m;
     ga.tap_ij = E->tap_ij;
     ga.tap_lohi = E->tap_lohi;
@@ -1024,16 +1064,17 @@ extern "C" int ifos_engine_set_shot_taper(ifos_engine *E, int on, float SRTRADIU
     E->tap_host.assign((size_t)n * n, 0.0f);
     ifos_srctaper_table(E->c.DH, SRTRADIUS, SRTSHAPE, E->tap_host.data(), n * n);
     if (n > E->tap_n) {
-        hipFree(E->tap_m);
+        dfree(E->tap_m);
         E->tap_m = nullptr;
         E->tap_n = 0;
-        HIP_TRY(hipMalloc(&E->tap_m, sizeof(float) * n * n));
+        HIP_TRY(dmalloc(&E->tap_m, sizeof(float) * n * n));
     }
     if (!E->tap_ij) {
-        HIP_TRY(hipMalloc(&E->tap_ij, sizeof(int) * 2 * E->slots));
-        HIP_TRY(hipMalloc(&E->tap_lohi, sizeof(float) * 2 * E->slots));
+        HIP_TRY(dmalloc(&E->tap_ij, sizeof(int) * 2 * E->slots));
+        HIP_TRY(dmalloc(&E->tap_lohi, sizeof(float) * 2 * E->slots));
     }
-    HIP_TRY(hipMemcpy(E->tap_m, E->tap_host.data(), sizeof(float) * n * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(E->tap_m, E->tap_host.data(), sizeof(float) * n * n, hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
     E->tap_n = n;
     E->tap_fs = FILTSIZE;
     E->tap_on = 1;
@@ -1099,7 +1140,7 @@ extern "C" int ifos_engine_set_snapshot_precision(ifos_engine *E, int prec)
     if (prec == E->snap_prec) return 0;
     HIP_TRY(hipStreamSynchronize(E->stream));
     drop_graphs(E);
-    hipFree(E->snap);
+    dfree(E->snap);
     E->snap = nullptr;
     E->snap_bytes = 0;
     E->snap_valid = false;
@@ -1160,11 +1201,24 @@ extern "C" int ifos_engine_get_launch(ifos_engine *E, int *spb, int *grid)
     return 0;
 }
 
+extern "C" int ifos_engine_set_work_order(ifos_engine *E, int slot_major)
+{
+    E->shape.order = slot_major ? 1 : 0;
+    return 0;
+}
+
+extern "C" int ifos_engine_get_work_order(ifos_engine *E, int *slot_major)
+{
+    *slot_major = E->shape.order;
+    return 0;
+}
+
 extern "C" int ifos_engine_nan_step(ifos_engine *E)
 {
     int h[2] = {0, 0};
     hipSetDevice(E->device);
-    hipMemcpy(h, E->ctl, sizeof h, hipMemcpyDeviceToHost);
+    hipMemcpyAsync(h, E->ctl, sizeof h, hipMemcpyDeviceToHost, E->stream);
+    hipStreamSynchronize(E->stream);
     return h[1];
 }
 
@@ -1174,32 +1228,38 @@ extern "C" int ifos_engine_nan_step(ifos_engine *E)
 extern "C" void *ifos_dev_alloc(size_t bytes)
 {
     void *p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-        ifos_set_error("hipMalloc(%zu) failed", bytes);
+    if (dmalloc(&p, bytes) != hipSuccess) {
+        ifos_set_error("dmalloc(%zu) failed", bytes);
         return nullptr;
     }
     return p;
 }
 extern "C" int ifos_dev_free(void *p)
 {
-    HIP_TRY(hipFree(p));
+    HIP_TRY(dfree(p));
+    return 0;
+}
+// copies of the helpers run on a non-blocking stream of the calling thread
+// (not the legacy stream, which would wait on a capturing engine stream)
+static int copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    thread_local hipStream_t st = nullptr;
+    if (!st) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, kind, st));
+    HIP_TRY(hipStreamSynchronize(st));
     return 0;
 }
 extern "C" int ifos_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
-    return 0;
+    return copy_sync(dst, src, bytes, hipMemcpyHostToDevice);
 }
 extern "C" int ifos_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-    return 0;
+    return copy_sync(dst, src, bytes, hipMemcpyDeviceToHost);
 }
 extern "C" int ifos_memcpy_d2d(void *dst, const void *src, size_t bytes)
 {
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
-    HIP_TRY(hipDeviceSynchronize());
-    return 0;
+    return copy_sync(dst, src, bytes, hipMemcpyDeviceToDevice);
 }
 
 // ---------------------------------------------------------------------------
@@ -1270,11 +1330,13 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
     HIP_TRY(hipEventCreate(&t1));
     double best = 1e30;
     LaunchShape pick = E->shape;
+    for (int order = 0; order < (E->nslots > 1 ? 2 : 1); order++)
     for (int spb : {1, 2, 4}) {
         if (spb > 1 && spb > E->nslots) break;
         for (int per_cu : {0, 2, 3, 4}) {
             LaunchShape ls;
             ls.spb = spb;
+            ls.order = order;
             ls.grid = per_cu ? (per_cu * ncu + 7) / 8 * 8 : 0;
             const StepArgs a = shaped_args(a0, ls);
             ENG_TRY(upload_args(E, a, 3, &da));

@@ -68,18 +68,23 @@ struct StepArgs {
     int nslots;                 // active shot slots
     int swizzle;                // 1: XCD-contiguous work order (see work_range)
     int ngroups;                // slot groups per tile (set by the launcher)
-    // division by nbx and by ngroups as multiply-high and shift (set by the
-    // launcher): the quotients of every work item without integer division
-    uint32_t nbx_mul, nbx_sh, ng_mul, ng_sh;
+    int slot_major;             // work order: 0 all slot groups of a tile, then the next tile;
+                                // 1 all tiles of a slot group, then the next group
+    // division by nbx, by ngroups and by the tile count as multiply-high and
+    // shift (set by the launcher): the quotients of every work item without
+    // integer division
+    uint32_t nbx_mul, nbx_sh, ng_mul, ng_sh, nt_mul, nt_sh;
 };
 
 // launch shape of the step kernels: spb slots per workgroup (1, 2 or 4; the
-// materials of a cell are loaded once for all of them) and, when grid > 0, a
+// materials of a cell are loaded once for all of them), when grid > 0 a
 // persistent grid of that many workgroups (a multiple of 8) looping over the
-// work items; grid == 0 launches one workgroup per work item
+// work items (grid == 0 launches one workgroup per work item), and the order
+// of the work items
 struct LaunchShape {
     int spb = 1;
     int grid = 0;
+    int order = 0;  // StepArgs::slot_major
 };
 
 // a: host copy of the argument block (grid sizes), da: the same block in

@@ -47,6 +47,19 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t mul, uint32_t sh)
     return (((n - t) >> 1) + t) >> sh;
 }
 
+// tile and first slot of work item it
+__device__ __forceinline__ void item_of(const StepArgs &a, int it, int spb, int &blk, int &s0)
+{
+    if (a.slot_major) {
+        const int grp = (int)fdiv(it, a.nt_mul, a.nt_sh);
+        blk = it - grp * (a.nbx * a.nby);
+        s0 = grp * spb;
+    } else {
+        blk = (int)fdiv(it, a.ng_mul, a.ng_sh);
+        s0 = (it - blk * a.ngroups) * spb;
+    }
+}
+
 // items [first, end) with the given stride for this workgroup: one item per
 // workgroup, or a persistent grid (gridDim.x a multiple of 8) looping over
 // the items of its XCD's run
@@ -233,7 +246,8 @@ __global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ p
     int first, end, stride;
     work_range(a, a.nbx * a.nby * ngroups, persistent != 0, first, end, stride);
     for (int it = first; it < end; it += stride) {
-        const int blk = (int)fdiv(it, a.ng_mul, a.ng_sh), s0 = (it - blk * ngroups) * SPB;
+        int blk, s0;
+        item_of(a, it, SPB, blk, s0);
         const int by = (int)fdiv(blk, a.nbx_mul, a.nbx_sh), bx = blk - by * a.nbx;
         const int i = bx * kTileX + threadIdx.x + 1;
         const int j = by * kTileY + threadIdx.y + 1;
@@ -265,7 +279,7 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int P = g.pitch;
     const long wo = (long)s * a.ws, po = (long)s * a.pss;
-    const const auto vxs = gmem(a.vx) + wo, vys = gmem(a.vy) + wo;
+    const auto vxs = gmem(a.vx) + wo, vys = gmem(a.vy) + wo;
     const bool snap_step = a.sw == 0 && has_snap(a) && (nt - 1) % a.DTINV == 0;
     const bool corr_step = a.sw == 1 && (a.NT - nt) % a.DTINV == 0;
     const bool rates = snap_step && a.VELOCITY == 1;
@@ -389,7 +403,8 @@ __global__ __launch_bounds__(512) void k_str_step(const StepArgs *__restrict__ p
     int first, end, stride;
     work_range(a, a.nbx * a.nby * ngroups, persistent != 0, first, end, stride);
     for (int it = first; it < end; it += stride) {
-        const int blk = (int)fdiv(it, a.ng_mul, a.ng_sh), s0 = (it - blk * ngroups) * SPB;
+        int blk, s0;
+        item_of(a, it, SPB, blk, s0);
         const int by = (int)fdiv(blk, a.nbx_mul, a.nbx_sh), bx = blk - by * a.nbx;
         const int i = bx * kTileX + threadIdx.x + 1;
         const int j = by * kTileY + threadIdx.y + 1;
@@ -442,8 +457,10 @@ StepArgs shaped_args(const StepArgs &a0, const LaunchShape &ls)
 {
     StepArgs a = a0;
     a.ngroups = (a.nslots + ls.spb - 1) / ls.spb;
+    a.slot_major = ls.order ? 1 : 0;
     make_div((uint32_t)a.nbx, a.nbx_mul, a.nbx_sh);
     make_div((uint32_t)a.ngroups, a.ng_mul, a.ng_sh);
+    make_div((uint32_t)(a.nbx * a.nby), a.nt_mul, a.nt_sh);
     return a;
 }
 

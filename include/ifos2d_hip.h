@@ -114,9 +114,16 @@ int  ifos_engine_shot_misfit(ifos_engine *e, double *per_shot);
  * of that many workgroups. */
 int  ifos_engine_set_launch(ifos_engine *e, int spb, int grid);
 int  ifos_engine_get_launch(ifos_engine *e, int *spb, int *grid);
-/* Time every candidate launch shape on one forward step of the current
- * model and shots (reps launches each), keep the fastest, zero the stepped
- * wavefields; reports the pick and its time per step. */
+/* Order of the work items of a several-shot engine: 0 = every slot group of
+ * a tile, then the next tile (a tile's materials are reused across shots);
+ * 1 = every tile of a slot group, then the next group (a shot's
+ * neighbouring tiles, whose halo rows they share, run close together). */
+int  ifos_engine_set_work_order(ifos_engine *e, int slot_major);
+int  ifos_engine_get_work_order(ifos_engine *e, int *slot_major);
+/* Time every candidate launch shape (and, with several slots, both work
+ * orders) on one forward step of the current model and shots (reps launches
+ * each), keep the fastest, zero the stepped wavefields; reports the pick and
+ * its time per step. */
 int  ifos_engine_autotune(ifos_engine *e, int reps, int *spb, int *grid, double *ms_per_step);
 /* Forward run (src/IFOS2D.c:1651-1894); snapshots every DTINV steps if asked. */
 int  ifos_engine_forward(ifos_engine *e, int store_snapshots);

@@ -71,9 +71,11 @@ def test_batch_equals_shots_alone(gpu_lib, case):
     obs_vy = np.concatenate([o[1] for o in obs])
     eng = ifos_amd.Engine(w.cfg, slots=6)
     eng.set_model(**w.start)
-    for spb, grid in [(1, 0), (2, 0), (4, 0), (1, 512), (2, 256), (4, 1024)]:
-        print(f"[batch] case {case['seed']} spb {spb} grid {grid}", flush=True)
+    for spb, grid, order in [(1, 0, 0), (2, 0, 0), (4, 0, 0), (1, 512, 0), (2, 256, 0), (4, 1024, 0),
+                             (1, 0, 1), (2, 0, 1), (4, 512, 1)]:
+        print(f"[batch] case {case['seed']} spb {spb} grid {grid} order {order}", flush=True)
         eng.set_launch(spb, grid)
+        eng.set_work_order(order)
         eng.reset_gradient()
         eng.set_shots(shots)
         eng.shot_gradient(obs_vx, obs_vy)
