@@ -157,6 +157,8 @@ def main():
     ap.add_argument("--batch", type=int, default=1,
                     help="shots advanced together per kernel launch (1: the one-shot-at-a-time path)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape (no timing sweep)")
+    ap.add_argument("--order", default="auto", choices=["auto", "tile", "slot"],
+                    help="work order of several-shot launches (auto: the autotuner's pick)")
     ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"],
                     help="snapshot storage (fp32 keeps the gradient bit-identical to the reference)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -208,6 +210,9 @@ def main():
             spb, grid, ms = eng.autotune(reps=5)
             log(f"[bench] autotune: {spb} shots per workgroup, grid {grid or 'one per item'}, "
                 f"{'slot' if eng.get_work_order() else 'tile'}-major order, {ms:.4f} ms per step")
+        if args.order != "auto":
+            eng.set_work_order(args.order == "slot")
+        log(f"[bench] launch {eng.get_launch()}, {'slot' if eng.get_work_order() else 'tile'}-major")
         for kern in (0, 1):
             ms = eng.time_kernel(kern, reps=20)
             gbs = eng.kernel_bytes_per_cell(kern) * cfg.NX * cfg.NY * S / (ms * 1e-3) / 1e9
@@ -232,6 +237,9 @@ def main():
     if not args.no_autotune:
         eng.set_shots([(s.srcpos, s.recpos) for s in batches[0]])
         shape = eng.autotune(reps=5) + (eng.get_work_order(),)
+        if args.order != "auto":
+            eng.set_work_order(args.order == "slot")
+            shape = shape[:3] + (eng.get_work_order(),)
         log(f"[bench] autotune: {shape[0]} shots per workgroup, grid {shape[1] or 'one per item'}, "
             f"{'slot' if shape[3] else 'tile'}-major order, {shape[2]:.4f} ms per step")
 

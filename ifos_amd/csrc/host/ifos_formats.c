@@ -308,3 +308,41 @@ int ifos_su_read(const char *path, int max_tr, int ns, float *data, int *ntr_out
     *ntr_out = t;
     return 0;
 }
+
+/* ---- trace kill tables (src/calc_res.c:66-126, src/create_trkill_table.c) ---- */
+
+int ifos_trkill_read(const char *path, int ntr_glob, int nsrc_glob, int *table)
+{
+    if (ntr_glob < 0 || nsrc_glob < 0 || !table) return ifos_set_error("ifos_trkill_read: bad arguments"), -2;
+    FILE *f = fopen(path, "r");
+    if (!f) return ifos_set_error(" Trace kill file could not be opened!"), -2;
+    /* receiver-major, one integer per (receiver, shot); running into the
+     * end of the file during a read fails as in the reference */
+    for (int r = 0; r < ntr_glob; r++)
+        for (int s = 0; s < nsrc_glob; s++) {
+            int v = 0;
+            const int got = fscanf(f, "%d", &v);
+            if (feof(f) || got != 1) {
+                fclose(f);
+                return ifos_set_error(" Error while reading TraceKill file. Check dimensions!"), -2;
+            }
+            table[(long)r * nsrc_glob + s] = v;
+        }
+    fclose(f);
+    return 0;
+}
+
+int ifos_trkill_offsets(int ntr_glob, const int *recpos, int nsrc_glob, const float *srcpos, float DH, float lower,
+                        float upper, int *table)
+{
+    if (ntr_glob < 0 || nsrc_glob < 0 || !table) return ifos_set_error("ifos_trkill_offsets: bad arguments"), -2;
+    for (int s = 0; s < nsrc_glob; s++) {
+        const float xs = srcpos[2 * s], ys = srcpos[2 * s + 1];
+        for (int r = 0; r < ntr_glob; r++) {
+            const float dx = xs - recpos[2 * r] * DH, dy = ys - recpos[2 * r + 1] * DH;
+            const float offset = (float)fabs(sqrt((double)(dx * dx + dy * dy)));
+            if (offset > lower && offset < upper) table[(long)r * nsrc_glob + s] = 1;
+        }
+    }
+    return 0;
+}

@@ -74,3 +74,28 @@ def read_su(path, ns, max_tr=100000):
     n = C.c_int()
     _check(_L().ifos_su_read(path.encode(), max_tr, ns, buf.ctypes.data_as(C.POINTER(C.c_float)), C.byref(n)))
     return buf[:n.value].copy()
+
+
+def read_trkill(path, ntr_glob, nsrc_glob):
+    """Trace kill table [ntr_glob][nsrc_glob] from a TRKILL_FILE.dat."""
+    L = _L()
+    L.ifos_trkill_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    t = np.zeros((ntr_glob, nsrc_glob), np.int32)
+    _check(L.ifos_trkill_read(path.encode(), ntr_glob, nsrc_glob, t.ctypes.data_as(C.POINTER(C.c_int))))
+    return t
+
+
+def trkill_offsets(recpos, srcpos, DH, lower, upper, table=None):
+    """Offset-based trace kill (create_trkill_table): recpos [ntr][2] grid
+    indices, srcpos [nsrc][2] metres; marks 1 where lower < offset < upper,
+    added to `table` when given."""
+    L = _L()
+    L.ifos_trkill_offsets.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_float), C.c_float,
+                                      C.c_float, C.c_float, C.POINTER(C.c_int)]
+    rp = np.ascontiguousarray(recpos, np.int32).reshape(-1, 2)
+    sp = np.ascontiguousarray(srcpos, np.float32).reshape(-1, 2)
+    t = np.zeros((len(rp), len(sp)), np.int32) if table is None else np.ascontiguousarray(table, np.int32).copy()
+    _check(L.ifos_trkill_offsets(len(rp), rp.ctypes.data_as(C.POINTER(C.c_int)), len(sp),
+                                 sp.ctypes.data_as(C.POINTER(C.c_float)), DH, lower, upper,
+                                 t.ctypes.data_as(C.POINTER(C.c_int))))
+    return t

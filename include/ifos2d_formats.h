@@ -61,6 +61,20 @@ int ifos_su_write(const char *path, int ntr, int ns, const float *data, const in
  * sample count differs from ns. */
 int ifos_su_read(const char *path, int max_tr, int ns, float *data, int *ntr_out);
 
+/* Trace kill tables [ntr_glob][nsrc_glob] (1 = trace not used), as
+ * calc_res builds kill_tmp (src/calc_res.c:66-126).  A shot's kill mask for
+ * ifos_engine_residual is its column: kill[r] = table[r][shot].
+ * ifos_trkill_read: the integers of TRKILL_FILE.dat, receiver-major (fails,
+ * as the reference does, if the file ends while a value is read, so the
+ * last value must be followed by white space).  ifos_trkill_offsets:
+ * create_trkill_table (src/create_trkill_table.c:21-43) -- sets 1 where the
+ * source-receiver offset lies strictly between lower and upper; srcpos
+ * [nsrc_glob][2] in metres, recpos [ntr_glob][2] grid indices; entries
+ * outside the band are left as they are (TRKILL_OFFSET 2 adds to a file). */
+int ifos_trkill_read(const char *path, int ntr_glob, int nsrc_glob, int *table);
+int ifos_trkill_offsets(int ntr_glob, const int *recpos, int nsrc_glob, const float *srcpos, float DH, float lower,
+                        float upper, int *table);
+
 #ifdef __cplusplus
 }
 #endif

@@ -632,3 +632,30 @@ int ref_smooth(int nx, int ny, int filtsize, float *g)
     harness_grid_out(nx, ny, w, g);
     return FILT_SIZE_GRAD;
 }
+
+/* create_trkill_table (src/create_trkill_table.c:21) on a cleared table:
+ * recpos [ntr][2] grid indices, srcpos [nsrc][2] metres, out [ntr][nsrc] */
+void ref_trkill_offsets(int ntr, const int *recpos, int nsrc, const float *srcpos, float dh, float lower, float upper,
+                        int *out)
+{
+    harness_setup_opt(1, 1);
+    DH = dh;
+    int **kt = imatrix(1, ntr, 1, nsrc);
+    int **rp = imatrix(1, 3, 1, ntr);
+    float **sp = matrix(1, 8, 1, nsrc);
+    for (int r = 1; r <= ntr; r++) {
+        rp[1][r] = recpos[2 * (r - 1)];
+        rp[2][r] = recpos[2 * (r - 1) + 1];
+        for (int s = 1; s <= nsrc; s++) kt[r][s] = 0;
+    }
+    for (int s = 1; s <= nsrc; s++) {
+        sp[1][s] = srcpos[2 * (s - 1)];
+        sp[2][s] = srcpos[2 * (s - 1) + 1];
+    }
+    create_trkill_table(kt, ntr, rp, nsrc, sp, 1, lower, upper);
+    for (int r = 1; r <= ntr; r++)
+        for (int s = 1; s <= nsrc; s++) out[(r - 1) * nsrc + (s - 1)] = kt[r][s];
+    free_imatrix(kt, 1, ntr, 1, nsrc);
+    free_imatrix(rp, 1, 3, 1, ntr);
+    free_matrix(sp, 1, 8, 1, nsrc);
+}

@@ -8,7 +8,9 @@ import numpy as np
 import pytest
 
 import ifos_amd
-from ifos_amd import formats
+from ifos_amd import IfosError, formats
+from ifos_amd import formats as F
+import oracle_lib as O
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -83,3 +85,42 @@ def test_su_roundtrip_and_headers(tmp_path):
     np.testing.assert_array_equal(formats.read_su(p, ns), data)
     with pytest.raises(ifos_amd.IfosError):
         formats.read_su(p, ns + 1)
+
+
+def test_trkill_file(tmp_path):
+    t = np.array([[0, 1, 0], [1, 1, 0], [0, 0, 1], [1, 0, 0]], np.int32)
+    p = tmp_path / "kill.dat"
+    p.write_text("\n".join(" ".join(str(v) for v in row) for row in t) + "\n")
+    np.testing.assert_array_equal(F.read_trkill(str(p), 4, 3), t)
+    # a file that ends inside the last value, or is short, is rejected as in calc_res
+    p.write_text("\n".join(" ".join(str(v) for v in row) for row in t))
+    with pytest.raises(IfosError):
+        F.read_trkill(str(p), 4, 3)
+    p.write_text("0 1 0\n1 1 0\n")
+    with pytest.raises(IfosError):
+        F.read_trkill(str(p), 4, 3)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference tree not built into oracle/_ref")
+def test_trkill_offsets_match_reference():
+    import ctypes as C
+    rng = np.random.default_rng(3)
+    rec = np.stack([np.arange(5, 200, 7), np.full(28, 4)], 1).astype(np.int32)
+    src = np.stack([rng.uniform(20, 400, 6), rng.uniform(4, 30, 6)], 1).astype(np.float32)
+    DH, lo, hi = 2.0, 50.0, 250.0
+    got = F.trkill_offsets(rec, src, DH, lo, hi)
+    want = np.zeros((len(rec), len(src)), np.int32)
+    O.ref().ref_trkill_offsets(len(rec), rec.ctypes.data_as(C.POINTER(C.c_int)), len(src),
+                               src.ctypes.data_as(C.POINTER(C.c_float)), C.c_float(DH), C.c_float(lo), C.c_float(hi),
+                               want.ctypes.data_as(C.POINTER(C.c_int)))
+    np.testing.assert_array_equal(got, want)
+    assert 0 < got.sum() < got.size
+    # closed form: offsets by numpy in float64 away from the band edges
+    off = np.hypot(src[None, :, 0] - rec[:, None, 0] * DH, src[None, :, 1] - rec[:, None, 1] * DH)
+    far = (np.abs(off - lo) > 1e-3) & (np.abs(off - hi) > 1e-3)
+    np.testing.assert_array_equal(got[far], ((off > lo) & (off < hi)).astype(np.int32)[far])
+    # added to an existing table (TRKILL_OFFSET 2)
+    base = np.zeros_like(got)
+    base[0, :] = 1
+    both = F.trkill_offsets(rec, src, DH, lo, hi, table=base)
+    np.testing.assert_array_equal(both, np.maximum(base, got))
