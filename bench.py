@@ -206,9 +206,9 @@ def main():
     if args.kernels_only:
         eng.set_model(**w.start)
         eng.set_shots([(s.srcpos, s.recpos) for s in batches[0]])
-        if S > 1 and not args.no_autotune:
+        if not args.no_autotune:
             spb, grid, ms = eng.autotune(reps=5)
-            log(f"[bench] autotune: {spb} shots per workgroup, grid {grid or 'one per item'}, "
+            log(f"[bench] autotune: {spb} shots per workgroup, grids {eng.get_launch_grids()}, "
                 f"{'slot' if eng.get_work_order() else 'tile'}-major order, {ms:.4f} ms per step")
         if args.order != "auto":
             eng.set_work_order(args.order == "slot")
@@ -240,8 +240,10 @@ def main():
         if args.order != "auto":
             eng.set_work_order(args.order == "slot")
             shape = shape[:3] + (eng.get_work_order(),)
-        log(f"[bench] autotune: {shape[0]} shots per workgroup, grid {shape[1] or 'one per item'}, "
-            f"{'slot' if shape[3] else 'tile'}-major order, {shape[2]:.4f} ms per step")
+        gv, gs = eng.get_launch_grids()
+        log(f"[bench] autotune: {shape[0]} shots per workgroup, grids {gv or 'one per item'} (velocity) / "
+            f"{gs or 'one per item'} (stress), {'slot' if shape[3] else 'tile'}-major order, "
+            f"{shape[2]:.4f} ms per step")
 
     def step(k):
         batch = batches[k]
@@ -301,8 +303,8 @@ def main():
         "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "dh": cfg.DH, "FDORDER": cfg.FDORDER, "L": cfg.L,
                    "NT": cfg.NT, "DTINV": cfg.DTINV, "snapshots": args.snap, "shots_per_step": S, "receivers": int(len(w.shots[0].recpos)),
                    "parallelism": "one GPU, one shot at a time" if S == 1 else
-                   f"one GPU, {S} shots per launch" + (f" ({shape[0]} per workgroup, grid {shape[1]}, "
-                                                       f"{'slot' if shape[3] else 'tile'}-major)" if shape else "")},
+                   f"one GPU, {S} shots per launch" + (f" ({shape[0]} per workgroup, {'slot' if shape[3] else 'tile'}-major)"
+                                                       if shape else "")},
         "gcell_updates_per_s": round(gcells, 3),
         "gradient_parity": ("bit-identical to the reference FP32 kernels (tests/test_gpu_parity.py)" if args.snap == "fp32"
                             else f"{args.snap} snapshots: bit-identical to the oracle with the same quantisation, "

@@ -106,6 +106,13 @@ def _f32(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
 
 
+def time_filter(buf, ntr, ns, DT, fc, order, method=1, f_high_pass=0.0):
+    """timedomain_filt on device traces buf [ntr][ns] (a DeviceBuffer), in place."""
+    L = lib()
+    L.ifos_time_filter.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float]
+    _check(L.ifos_time_filter(C.c_void_p(buf.ptr), ntr, ns, DT, fc, order, method, f_high_pass))
+
+
 class DeviceBuffer:
     """Device allocation owned by the library's allocator (plain bytes)."""
 
@@ -215,6 +222,11 @@ class Engine:
         _check(lib().ifos_engine_get_launch(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def get_launch_grids(self):
+        a, b = C.c_int(), C.c_int()
+        _check(lib().ifos_engine_get_launch_grids(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def set_work_order(self, slot_major):
         _check(lib().ifos_engine_set_work_order(self.h, int(bool(slot_major))))
 
@@ -288,6 +300,12 @@ class Engine:
 
     def apply_eprecond(self):
         _check(lib().ifos_engine_apply_eprecond(self.h))
+
+    def set_time_filter(self, f_low_pass, order=4, f_high_pass=0.0):
+        """Butterworth filter of wavelets and observed traces (TIME_FILT); 0 turns it off."""
+        L = lib()
+        L.ifos_engine_set_time_filter.argtypes = [C.c_void_p, C.c_float, C.c_int, C.c_float]
+        _check(L.ifos_engine_set_time_filter(self.h, f_low_pass, order, f_high_pass))
 
     def set_shot_taper(self, on, srtradius=0.0, srtshape=1, filtsize=0):
         """Circular taper of each shot's gradient around its source

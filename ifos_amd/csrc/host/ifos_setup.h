@@ -30,6 +30,12 @@ int ifos_setup_materials(const ifos_config *c, const float *vp, const float *vs,
                          const float *taus, const float *taup, ifos_materials *m);
 void ifos_free_materials(ifos_materials *m);
 
+/* Butterworth low- or highpass of the given order and corner fc (Hz) for
+ * sample interval dt as cascaded sections {f0, f1, f2, g1, g2} (coef
+ * [max_sections][5]); returns the section count, 0 for no filter, -1 if
+ * max_sections is too small. */
+int ifos_butterworth(int highpass, double dt, double fc, int order, double *coef, int max_sections);
+
 #ifdef __cplusplus
 }
 #endif

@@ -168,6 +168,10 @@ struct ifos_engine {
     float *tap_lohi = nullptr;      // [slots][2] normalisation
     std::vector<float> tap_host;    // the table on the host
     std::vector<int> first_src;     // [slots][2] first source of each shot (0, 0: none)
+    // time-domain Butterworth filter of wavelets and observed traces (TIME_FILT)
+    int filt_nlow = 0, filt_nhigh = 0;
+    double filt_c[2 * kMaxFiltSections][5];
+    std::vector<float> sig_host;    // unfiltered wavelets of the current shots
     StepArgs *dargs = nullptr;  // device copies of the argument blocks: forward, forward with
                                 // snapshots, adjoint, eager/timing
 
@@ -419,6 +423,21 @@ extern "C" int ifos_engine_set_model(ifos_engine *E, const float *vp, const floa
     return 0;
 }
 
+// the time-domain filter over traces [ntr][NT] in place, on the engine stream
+static void filter_traces(ifos_engine *E, float *d, int ntr, bool apply)
+{
+    if (!apply || (!E->filt_nlow && !E->filt_nhigh) || ntr <= 0) return;
+    FiltArgs f;
+    f.data = d;
+    f.ntr = ntr;
+    f.ns = E->c.NT;
+    f.stride = E->c.NT;
+    f.nlow = E->filt_nlow;
+    f.nhigh = E->filt_nhigh;
+    memcpy(f.c, E->filt_c, sizeof f.c);
+    launch_time_filter(f, E->stream);
+}
+
 // shots of the slots: slot s has nsrc[s] sources (rows of srcpos, all
 // slots' rows concatenated) and ntr[s] receivers (rows of recpos, likewise)
 extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc_of, const float *srcpos,
@@ -504,7 +523,9 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
         HIP_TRY(hipMemcpyAsync(E->src_i, si.data(), sizeof(int) * si.size(), hipMemcpyHostToDevice, E->stream));
         HIP_TRY(hipMemcpyAsync(E->src_d, sd.data(), sizeof(double) * sd.size(), hipMemcpyHostToDevice, E->stream));
         HIP_TRY(hipMemcpyAsync(E->sig, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice, E->stream));
+        filter_traces(E, E->sig, nsrc, c.SOURCE_SHAPE != 6);
         HIP_TRY(hipStreamSynchronize(E->stream));
+        E->sig_host.swap(w);
     }
     if (ntr) {
         std::vector<int> ri(3 * (size_t)E->cap_tr, 0);
@@ -692,7 +713,7 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
     while (unit < 32 && NT % (unit * 2) == 0) unit *= 2;
     const int kind = sw;
     const int key[6] = {E->nsrc, E->ntr, snapshots ? 1 : 0, E->nslots,
-                        (E->shape.order * 8 + E->shape.spb) * 65536 + E->shape.grid, unit};
+                        ((E->shape.order * 8 + E->shape.spb) * 4096 + E->shape.grid_v) * 4096 + E->shape.grid, unit};
     const LaunchShape ls = E->shape;
     StepArgs a = shaped_args(step_args(E, sw, snapshots), ls);
     const StepArgs *da = nullptr;
@@ -828,6 +849,19 @@ extern "C" int ifos_engine_residual(ifos_engine *E, const float *obs_vx, const f
         if (obs_vy) HIP_TRY(hipMemcpyAsync(seis_buf(E, 5), obs_vy, n * sizeof(float), hipMemcpyHostToDevice, E->stream));
         ovx = seis_buf(E, 4);
         ovy = seis_buf(E, 5);
+    }
+    if (E->filt_nlow || E->filt_nhigh) {
+        // observed traces are filtered like the reference's sectionread
+        // (src/IFOS2D.c:1985-2071), in the engine's copy
+        if (obs_on_device) {
+            if (obs_vx) HIP_TRY(hipMemcpyAsync(seis_buf(E, 4), obs_vx, n * sizeof(float), hipMemcpyDeviceToDevice, E->stream));
+            if (obs_vy) HIP_TRY(hipMemcpyAsync(seis_buf(E, 5), obs_vy, n * sizeof(float), hipMemcpyDeviceToDevice, E->stream));
+            ovx = seis_buf(E, 4);
+            ovy = seis_buf(E, 5);
+        }
+        if (obs_vx) filter_traces(E, seis_buf(E, 4), ntr, true);
+        if (obs_vy) filter_traces(E, seis_buf(E, 5), ntr, true);
+        HIP_TRY(hipGetLastError());
     }
     const int *dkill = nullptr;
     if (c.TRKILL && kill) {
@@ -1052,6 +1086,62 @@ static int shot_taper_args(ifos_engine *E, GradArgs &ga)
     return 0;
 }
 
+extern "C" int ifos_time_filter(float *data, int ntr, int ns, float DT, float fc, int order, int method,
+                                float F_HIGH_PASS)
+{
+    if (ntr < 0 || ns < 0 || (ntr && !data)) return ifos_set_error("ifos_time_filter: bad arguments"), -2;
+    if (method != 1 && method != 2) return ifos_set_error("ifos_time_filter: method must be 1 or 2"), -2;
+    if (order < 1 || order > 2 * kMaxFiltSections)
+        return ifos_set_error("filter order %d outside 1..%d", order, 2 * kMaxFiltSections), -2;
+    FiltArgs f;
+    memset(&f, 0, sizeof f);
+    f.data = data;
+    f.ntr = ntr;
+    f.ns = ns;
+    f.stride = ns;
+    // timedomain_filt: method 1 lowpass at fc (+ highpass at F_HIGH_PASS when
+    // set), method 2 highpass at fc
+    if (method == 1) f.nlow = ifos_butterworth(0, DT, (double)fc, order, &f.c[0][0], kMaxFiltSections);
+    const float fh = method == 2 ? fc : F_HIGH_PASS;
+    if (method == 2 || F_HIGH_PASS != 0.0f)
+        f.nhigh = ifos_butterworth(1, DT, (double)fh, order, &f.c[f.nlow][0], kMaxFiltSections);
+    if (f.nlow < 0 || f.nhigh < 0) return ifos_set_error("filter design failed"), -2;
+    thread_local hipStream_t st = nullptr;
+    if (!st) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    launch_time_filter(f, st);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+extern "C" int ifos_engine_set_time_filter(ifos_engine *E, float F_LOW_PASS, int ORDER, float F_HIGH_PASS)
+{
+    HIP_TRY(hipSetDevice(E->device));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    if (F_LOW_PASS != 0.0f && (ORDER < 1 || ORDER > 2 * kMaxFiltSections))
+        return ifos_set_error("filter order %d outside 1..%d", ORDER, 2 * kMaxFiltSections), -2;
+    E->filt_nlow = E->filt_nhigh = 0;
+    if (F_LOW_PASS != 0.0f) {
+        // timedomain_filt method 1: T0 = 1/fc, lowpass, then highpass at F_HIGH_PASS when set
+        const int nl = ifos_butterworth(0, E->c.DT, (double)F_LOW_PASS, ORDER, &E->filt_c[0][0], kMaxFiltSections);
+        const int nh = F_HIGH_PASS != 0.0f ? ifos_butterworth(1, E->c.DT, (double)F_HIGH_PASS, ORDER,
+                                                              &E->filt_c[nl][0], kMaxFiltSections)
+                                           : 0;
+        if (nl < 0 || nh < 0) return ifos_set_error("filter design failed"), -2;
+        E->filt_nlow = nl;
+        E->filt_nhigh = nh;
+    }
+    // wavelets of shots already set: filter afresh from the unfiltered ones
+    if (E->nsrc && !E->sig_host.empty()) {
+        HIP_TRY(hipMemcpyAsync(E->sig, E->sig_host.data(), sizeof(float) * E->sig_host.size(), hipMemcpyHostToDevice,
+                               E->stream));
+        filter_traces(E, E->sig, E->nsrc, E->c.SOURCE_SHAPE != 6);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(E->stream));
+    }
+    return 0;
+}
+
 extern "C" int ifos_engine_set_shot_taper(ifos_engine *E, int on, float SRTRADIUS, int SRTSHAPE, int FILTSIZE)
 {
     HIP_TRY(hipSetDevice(E->device));
@@ -1191,6 +1281,14 @@ extern "C" int ifos_engine_set_launch(ifos_engine *E, int spb, int grid)
     if (grid < 0 || grid % 8) return ifos_set_error("persistent grid must be 0 or a positive multiple of 8"), -2;
     E->shape.spb = spb;
     E->shape.grid = grid;
+    E->shape.grid_v = grid;
+    return 0;
+}
+
+extern "C" int ifos_engine_get_launch_grids(ifos_engine *E, int *grid_vel, int *grid_str)
+{
+    *grid_vel = E->shape.grid_v;
+    *grid_str = E->shape.grid;
     return 0;
 }
 
@@ -1330,33 +1428,47 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
     HIP_TRY(hipEventCreate(&t1));
     double best = 1e30;
     LaunchShape pick = E->shape;
+    // each kernel's persistent grid is chosen on its own; shots per
+    // workgroup and the work order are common to both (one argument block)
+    auto timed = [&](bool vel, const StepArgs &a, const LaunchShape &ls, float *ms) -> int {
+        if (vel) launch_vel_step(a, da, ls, 2, E->stream);
+        else launch_str_step(a, da, ls, 2, E->stream);
+        HIP_TRY(hipEventRecord(t0, E->stream));
+        for (int k = 0; k < reps; k++) {
+            if (vel) launch_vel_step(a, da, ls, 2, E->stream);
+            else launch_str_step(a, da, ls, 2, E->stream);
+        }
+        HIP_TRY(hipEventRecord(t1, E->stream));
+        HIP_TRY(hipEventSynchronize(t1));
+        HIP_TRY(hipEventElapsedTime(ms, t0, t1));
+        *ms /= reps;
+        return 0;
+    };
     for (int order = 0; order < (E->nslots > 1 ? 2 : 1); order++)
-    for (int spb : {1, 2, 4}) {
-        if (spb > 1 && spb > E->nslots) break;
-        for (int per_cu : {0, 2, 3, 4}) {
+        for (int spb : {1, 2, 4}) {
+            if (spb > 1 && spb > E->nslots) break;
             LaunchShape ls;
             ls.spb = spb;
             ls.order = order;
-            ls.grid = per_cu ? (per_cu * ncu + 7) / 8 * 8 : 0;
             const StepArgs a = shaped_args(a0, ls);
             ENG_TRY(upload_args(E, a, 3, &da));
-            launch_vel_step(a, da, ls, 2, E->stream);
-            launch_str_step(a, da, ls, 2, E->stream);
-            HIP_TRY(hipEventRecord(t0, E->stream));
-            for (int k = 0; k < reps; k++) {
-                launch_vel_step(a, da, ls, 2, E->stream);
-                launch_str_step(a, da, ls, 2, E->stream);
+            float best_v = 1e30f, best_s = 1e30f;
+            int grid_v = 0, grid_s = 0;
+            for (int per_cu : {0, 2, 3, 4}) {
+                ls.grid = ls.grid_v = per_cu ? (per_cu * ncu + 7) / 8 * 8 : 0;
+                float mv = 0.0f, ms = 0.0f;
+                ENG_TRY(timed(true, a, ls, &mv));
+                ENG_TRY(timed(false, a, ls, &ms));
+                if (mv < best_v) best_v = mv, grid_v = ls.grid_v;
+                if (ms < best_s) best_s = ms, grid_s = ls.grid;
             }
-            HIP_TRY(hipEventRecord(t1, E->stream));
-            HIP_TRY(hipEventSynchronize(t1));
-            float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
-            if (ms / reps < best) {
-                best = ms / reps;
+            if (best_v + best_s < best) {
+                best = best_v + best_s;
                 pick = ls;
+                pick.grid_v = grid_v;
+                pick.grid = grid_s;
             }
         }
-    }
     hipEventDestroy(t0);
     hipEventDestroy(t1);
     E->shape = pick;

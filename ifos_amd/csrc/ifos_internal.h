@@ -195,7 +195,20 @@ struct ResArgs {
     float *l2f_trace, *enf_trace;  // per-trace FP32 running sums
 };
 
+// time-domain Butterworth filter of traces [ntr][ns] (stride between
+// traces): a lowpass cascade, its result rounded to float, then a highpass
+// cascade (either may be empty), per trace behind one leading zero sample
+constexpr int kMaxFiltSections = 6;  // orders up to 12
+struct FiltArgs {
+    float *data;
+    int ntr, ns;
+    long stride;
+    int nlow, nhigh;
+    double c[2 * kMaxFiltSections][5];  // the lowpass sections, then the highpass ones
+};
+
 // launchers (ifos_kernels.hip)
+void launch_time_filter(const FiltArgs &a, hipStream_t s);
 void launch_update_v(const VArgs &a, hipStream_t s);
 void launch_inject(const PointArgs &a, const StepGeom &g, float *vx, float *vy, const float *rip,
                    const float *rjp, int sw, int ADJOINT_TYPE, const int *nt_dev, int nt, int *nt_tick,

@@ -756,6 +756,56 @@ void launch_gradient(const GradArgs &a, hipStream_t s)
     hipLaunchKernelGGL(k_gradient, gr, b, 0, s, a);
 }
 
+// Time-domain Butterworth filter (timedomain_filt, src/timedomain_filt.c:27,
+// libcseife seife_butterworth / seife_rekfl), one thread per trace.  The
+// reference runs each section over the whole trace in double; a cascade of
+// causal sections gives every section the same inputs when it is run sample
+// by sample instead, so the state of all sections stays in registers.  The
+// reference filters ns+1 samples whose first is 0 (its vector's element 0),
+// and stores the lowpass result as float before the highpass pass.
+template <int NS>
+__device__ __forceinline__ double cascade(const double (*c)[5], double (*st)[4], int n, double x)
+{
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        if (k >= n) break;
+        const double y = c[k][0] * x + c[k][1] * st[k][0] + c[k][2] * st[k][1] + c[k][3] * st[k][2] +
+                         c[k][4] * st[k][3];
+        st[k][1] = st[k][0];
+        st[k][0] = x;
+        st[k][3] = st[k][2];
+        st[k][2] = y;
+        x = y;
+    }
+    return x;
+}
+
+IFOS_KERNEL void k_time_filter(FiltArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntr) return;
+    float *tr = a.data + (long)t * a.stride;
+    double lo[kMaxFiltSections][4], hi[kMaxFiltSections][4];
+    for (int k = 0; k < kMaxFiltSections; k++)
+        for (int q = 0; q < 4; q++) lo[k][q] = hi[k][q] = 0.0;
+    // the leading zero sample of both passes
+    if (a.nlow) cascade<kMaxFiltSections>(a.c, lo, a.nlow, 0.0);
+    if (a.nhigh) cascade<kMaxFiltSections>(a.c + a.nlow, hi, a.nhigh, 0.0);
+    for (int j = 0; j < a.ns; j++) {
+        float v = tr[j];
+        if (a.nlow) v = (float)cascade<kMaxFiltSections>(a.c, lo, a.nlow, (double)v);
+        if (a.nhigh) v = (float)cascade<kMaxFiltSections>(a.c + a.nlow, hi, a.nhigh, (double)v);
+        tr[j] = v;
+    }
+}
+
+void launch_time_filter(const FiltArgs &a, hipStream_t s)
+{
+    if (a.ntr <= 0 || (!a.nlow && !a.nhigh)) return;
+    dim3 b(64), gr((a.ntr + 63) / 64);
+    hipLaunchKernelGGL(k_time_filter, gr, b, 0, s, a);
+}
+
 void launch_residual(const ResArgs &a, hipStream_t s)
 {
     dim3 b(64), gr((a.ntr + 63) / 64);

@@ -83,8 +83,9 @@ struct StepArgs {
 // of the work items
 struct LaunchShape {
     int spb = 1;
-    int grid = 0;
-    int order = 0;  // StepArgs::slot_major
+    int grid = 0;    // stress step
+    int grid_v = 0;  // velocity step
+    int order = 0;   // StepArgs::slot_major
 };
 
 // a: host copy of the argument block (grid sizes), da: the same block in

@@ -464,17 +464,17 @@ StepArgs shaped_args(const StepArgs &a0, const LaunchShape &ls)
     return a;
 }
 
-static int grid_of(const StepArgs &a, const LaunchShape &ls)
+static int grid_of(const StepArgs &a, const LaunchShape &ls, int grid)
 {
     const int nitems = a.nbx * a.nby * ((a.nslots + ls.spb - 1) / ls.spb);
-    return ls.grid > 0 ? std::min(ls.grid, (nitems + 7) / 8 * 8) : nitems;
+    return grid > 0 ? std::min(grid, (nitems + 7) / 8 * 8) : nitems;
 }
 
 template <int N, int SPB>
 static void vel_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
 {
-    const int gr = grid_of(a, ls);
-    hipLaunchKernelGGL((k_vel_step<N, SPB>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid > 0 ? 1 : 0);
+    const int gr = grid_of(a, ls, ls.grid_v);
+    hipLaunchKernelGGL((k_vel_step<N, SPB>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid_v > 0 ? 1 : 0);
 }
 
 template <int N>
@@ -504,7 +504,7 @@ void launch_vel_step(const StepArgs &a, const StepArgs *da, const LaunchShape &l
 template <int N, int L, int SPB>
 static void str_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
 {
-    const int gr = grid_of(a, ls);
+    const int gr = grid_of(a, ls, ls.grid);
     hipLaunchKernelGGL((k_str_step<N, L, SPB>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid > 0 ? 1 : 0);
 }
 

@@ -114,6 +114,9 @@ int  ifos_engine_shot_misfit(ifos_engine *e, double *per_shot);
  * of that many workgroups. */
 int  ifos_engine_set_launch(ifos_engine *e, int spb, int grid);
 int  ifos_engine_get_launch(ifos_engine *e, int *spb, int *grid);
+/* The persistent grids of the velocity and the stress step (set_launch sets
+ * both; the autotuner picks each on its own). */
+int  ifos_engine_get_launch_grids(ifos_engine *e, int *grid_vel, int *grid_str);
 /* Order of the work items of a several-shot engine: 0 = every slot group of
  * a tile, then the next tile (a tile's materials are reused across shots);
  * 1 = every tile of a slot group, then the next group (a shot's
@@ -122,8 +125,8 @@ int  ifos_engine_set_work_order(ifos_engine *e, int slot_major);
 int  ifos_engine_get_work_order(ifos_engine *e, int *slot_major);
 /* Time every candidate launch shape (and, with several slots, both work
  * orders) on one forward step of the current model and shots (reps launches
- * each), keep the fastest, zero the stepped wavefields; reports the pick and
- * its time per step. */
+ * of each kernel), keep the fastest, zero the stepped wavefields; reports
+ * the pick (grid: the stress step's) and its time per step. */
 int  ifos_engine_autotune(ifos_engine *e, int reps, int *spb, int *grid, double *ms_per_step);
 /* Forward run (src/IFOS2D.c:1651-1894); snapshots every DTINV steps if asked. */
 int  ifos_engine_forward(ifos_engine *e, int store_snapshots);
@@ -172,6 +175,21 @@ int  ifos_engine_energy_weight(ifos_engine *e, int which, float *out);
  * three gradients.  The taper centre is the shot's first source (the
  * reference's srcpos[.][ishot]).  on = 0 turns it off. */
 int  ifos_engine_set_shot_taper(ifos_engine *e, int on, float SRTRADIUS, int SRTSHAPE, int FILTSIZE);
+/* Time-domain Butterworth filter of the multi-scale inversion (TIME_FILT,
+ * timedomain_filt src/timedomain_filt.c:27 with the vendored libcseife):
+ * lowpass of order ORDER at F_LOW_PASS Hz, then, when F_HIGH_PASS != 0, a
+ * highpass of the same order at F_HIGH_PASS.  Applied, as the reference
+ * does, to every shot's source wavelets (unless SOURCE_SHAPE is 6,
+ * src/IFOS2D.c:1556-1561) and to the observed traces given to
+ * ifos_engine_residual (src/IFOS2D.c:1985-2071; a device copy is filtered,
+ * the caller's traces are left alone).  Double precision per trace as
+ * there; bit-identical traces.  F_LOW_PASS = 0 turns it off; ORDER <= 12. */
+int  ifos_engine_set_time_filter(ifos_engine *e, float F_LOW_PASS, int ORDER, float F_HIGH_PASS);
+/* replaces timedomain_filt() src/timedomain_filt.c:27 on device traces
+ * data [ntr][ns]: method 1 lowpass at fc (then a highpass at F_HIGH_PASS
+ * when it is not 0, the global of that name), method 2 highpass at fc;
+ * Butterworth of the given order (<= 12) in double per trace. */
+int  ifos_time_filter(float *data, int ntr, int ns, float DT, float fc, int order, int method, float F_HIGH_PASS);
 /* Storage of the forward snapshots (src/IFOS2D.c:1805-1871 keeps them FP32):
  * 0 = FP32 (default; gradients bit-identical to the reference), 1 = BF16,
  * 2 = FP16.  16-bit snapshots hold value / 2^e rounded to nearest-even, with

@@ -659,3 +659,20 @@ void ref_trkill_offsets(int ntr, const int *recpos, int nsrc, const float *srcpo
     free_imatrix(rp, 1, 3, 1, ntr);
     free_matrix(sp, 1, 8, 1, nsrc);
 }
+
+/* timedomain_filt (src/timedomain_filt.c:27) on data [ntr][ns]: method 1
+ * lowpass at fc (plus a highpass at fhp when fhp != 0), 2 highpass at fc */
+void ref_time_filter(float *data, int ntr, int ns, float dt, float fc, int order, int method, float fhp)
+{
+    harness_setup_opt(1, 1);
+    DT = dt;
+    F_HIGH_PASS = fhp;
+    NT = ns;
+    float **d = matrix(1, ntr, 1, ns);
+    for (int t = 1; t <= ntr; t++)
+        for (int j = 1; j <= ns; j++) d[t][j] = data[(size_t)(t - 1) * ns + j - 1];
+    timedomain_filt(d, fc, order, ntr, ns, method);
+    for (int t = 1; t <= ntr; t++)
+        for (int j = 1; j <= ns; j++) data[(size_t)(t - 1) * ns + j - 1] = d[t][j];
+    free_matrix(d, 1, ntr, 1, ns);
+}

@@ -397,3 +397,14 @@ def median_smooth(g, filtsize, which="oracle"):
     out = np.empty((ny, nx), np.float32)
     oracle().orc_median_smooth(nx, ny, filtsize, _ptr(_f32(g)), _ptr(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# time-domain Butterworth filter (oracle/filt_oracle.c; reference
+# timedomain_filt + contrib/libcseife through oracle/_ref)
+def time_filter(data, DT, fc, order, method=1, fhp=0.0, which="oracle"):
+    out = _f32(data).copy()
+    ntr, ns = out.shape
+    f = ref().ref_time_filter if which == "ref" else oracle().orc_time_filter
+    f(_ptr(out), ntr, ns, C.c_float(DT), C.c_float(fc), order, method, C.c_float(fhp))
+    return out
