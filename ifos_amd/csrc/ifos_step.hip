@@ -180,6 +180,10 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     const int P = g.pitch;
     const long wo = (long)s * a.ws, po = (long)s * a.pss;
     const auto vxs = gmem(a.vx) + wo, vys = gmem(a.vy) + wo;
+    // bounds of this tile's point-force / residual list, loaded before the
+    // stencil work so that their latency is hidden behind it
+    const long so = (long)s * (a.nbx * a.nby + 1) + blk;
+    const int q0 = gmem(a.inj.start)[so], q1 = gmem(a.inj.start)[so + 1];
     float sxx_x = 0.0f, sxy_x = 0.0f, sxy_y = 0.0f, syy_y = 0.0f, vx0 = 0.0f, vy0 = 0.0f;
     if (live) {
         stress_derivs<N>(g.hc, a.sxx + wo + c, a.syy + wo + c, a.sxy + wo + c, P, sxx_x, sxy_x, sxy_y, syy_y);
@@ -206,8 +210,7 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     float vy = vel_new(vy0, g.DT, rjp, sxy_x + syy_y, g.DH);
 
     // point forces (forward) / residual injection (adjoint) of this tile, in order
-    const auto st = gmem(a.inj.start) + (long)s * (a.nbx * a.nby + 1);
-    for (int q = st[blk]; q < st[blk + 1]; q++) {
+    for (int q = q0; q < q1; q++) {
         const int l = gmem(a.inj.idx)[q];
         if (gmem(a.inj.ix)[l] != i || gmem(a.inj.iy)[l] != j) continue;
         const float s0 = gmem(a.inj.sig)[(long)l * a.NT + nt - 1];
@@ -283,6 +286,17 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
     const bool snap_step = a.sw == 0 && has_snap(a) && (nt - 1) % a.DTINV == 0;
     const bool corr_step = a.sw == 1 && (a.NT - nt) % a.DTINV == 0;
     const bool rates = snap_step && a.VELOCITY == 1;
+    // bounds of this tile's explosive-source and receiver lists, loaded
+    // before the stencil work so that their latency is hidden behind it
+    int expl = 0, ex0 = 0, ex1 = 0, rc0 = 0, rc1 = 0;
+    if (a.sw == 0) {
+        const long so = (long)s * (a.nbx * a.nby + 1) + blk;
+        expl = gmem(a.explosive)[s];
+        ex0 = gmem(a.exp.start)[so];
+        ex1 = gmem(a.exp.start)[so + 1];
+        rc0 = gmem(a.rec.start)[so];
+        rc1 = gmem(a.rec.start)[so + 1];
+    }
 
     float sxx = 0.0f, syy = 0.0f, sxy = 0.0f, ux = 0.0f, uy = 0.0f, uxy = 0.0f;
     float rr[L > 0 ? L : 1], pp[L > 0 ? L : 1], qq[L > 0 ? L : 1];
@@ -309,9 +323,8 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
         }
         // explosive sources of this tile, applied after the stress update
         // and before the free surface (src/IFOS2D.c:1758-1773)
-        if (a.sw == 0 && gmem(a.explosive)[s]) {
-            const auto st = gmem(a.exp.start) + (long)s * (a.nbx * a.nby + 1);
-            for (int q = st[blk]; q < st[blk + 1]; q++) {
+        if (expl) {
+            for (int q = ex0; q < ex1; q++) {
                 const int l = gmem(a.exp.idx)[q];
                 if (gmem(a.exp.ix)[l] != i || gmem(a.exp.iy)[l] != j) continue;
                 const float amp = explosive_amp(a.exp.sig + (long)l * a.NT - 1, nt, a.NT, g.DH);
@@ -376,8 +389,7 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
         gmem(a.wc_u)[wco + e] = wu;
     }
     if (a.sw == 0) {
-        const auto st = gmem(a.rec.start) + (long)s * (a.nbx * a.nby + 1);
-        for (int q = st[blk]; q < st[blk + 1]; q++) {
+        for (int q = rc0; q < rc1; q++) {
             const int t = gmem(a.rec.idx)[q];
             if (gmem(a.rec.ix)[t] != i || gmem(a.rec.iy)[t] != j) continue;
             gmem(a.seis_vx)[(long)t * a.NT + nt - 1] = vxs[c];
