@@ -180,10 +180,6 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     const int P = g.pitch;
     const long wo = (long)s * a.ws, po = (long)s * a.pss;
     const auto vxs = gmem(a.vx) + wo, vys = gmem(a.vy) + wo;
-    // bounds of this tile's point-force / residual list, loaded before the
-    // stencil work so that their latency is hidden behind it
-    const long so = (long)s * (a.nbx * a.nby + 1) + blk;
-    const int q0 = gmem(a.inj.start)[so], q1 = gmem(a.inj.start)[so + 1];
     float sxx_x = 0.0f, sxy_x = 0.0f, sxy_y = 0.0f, syy_y = 0.0f, vx0 = 0.0f, vy0 = 0.0f;
     if (live) {
         stress_derivs<N>(g.hc, a.sxx + wo + c, a.syy + wo + c, a.sxy + wo + c, P, sxx_x, sxy_x, sxy_y, syy_y);
@@ -192,6 +188,10 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
         vx0 = vxs[c];
         vy0 = vys[c];
     }
+    // bounds of this tile's point-force / residual list, loaded behind the
+    // stencil loads so that their latency overlaps them
+    const long so = (long)s * (a.nbx * a.nby + 1) + blk;
+    const int q0 = gmem(a.inj.start)[so], q1 = gmem(a.inj.start)[so + 1];
     if (a.sw == 0) {
         if (has_snap(a) && (nt - 1) % a.DTINV == 0) {
             const float val[2] = {vel_rate(rip, sxx_x + sxy_y, g.DH), vel_rate(rjp, sxy_x + syy_y, g.DH)};
