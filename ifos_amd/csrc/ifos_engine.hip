@@ -1428,22 +1428,14 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
     HIP_TRY(hipEventCreate(&t1));
     double best = 1e30;
     LaunchShape pick = E->shape;
-    // each kernel's persistent grid is chosen on its own; shots per
-    // workgroup and the work order are common to both (one argument block)
-    auto timed = [&](bool vel, const StepArgs &a, const LaunchShape &ls, float *ms) -> int {
-        if (vel) launch_vel_step(a, da, ls, 2, E->stream);
-        else launch_str_step(a, da, ls, 2, E->stream);
-        HIP_TRY(hipEventRecord(t0, E->stream));
-        for (int k = 0; k < reps; k++) {
-            if (vel) launch_vel_step(a, da, ls, 2, E->stream);
-            else launch_str_step(a, da, ls, 2, E->stream);
-        }
-        HIP_TRY(hipEventRecord(t1, E->stream));
-        HIP_TRY(hipEventSynchronize(t1));
-        HIP_TRY(hipEventElapsedTime(ms, t0, t1));
-        *ms /= reps;
-        return 0;
-    };
+    // each kernel's persistent grid is chosen on its own, timed in the
+    // alternation the time loop runs (velocity, stress, velocity, ...) with
+    // events between the kernels; shots per workgroup and the work order are
+    // common to both (one argument block)
+    constexpr int kMaxReps = 16;
+    reps = std::min(reps, kMaxReps);
+    hipEvent_t ev[2 * kMaxReps + 1];
+    for (auto &x : ev) HIP_TRY(hipEventCreate(&x));
     for (int order = 0; order < (E->nslots > 1 ? 2 : 1); order++)
         for (int spb : {1, 2, 4}) {
             if (spb > 1 && spb > E->nslots) break;
@@ -1456,9 +1448,26 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
             int grid_v = 0, grid_s = 0;
             for (int per_cu : {0, 2, 3, 4}) {
                 ls.grid = ls.grid_v = per_cu ? (per_cu * ncu + 7) / 8 * 8 : 0;
+                launch_vel_step(a, da, ls, 2, E->stream);
+                launch_str_step(a, da, ls, 2, E->stream);
+                HIP_TRY(hipEventRecord(ev[0], E->stream));
+                for (int k = 0; k < reps; k++) {
+                    launch_vel_step(a, da, ls, 2, E->stream);
+                    HIP_TRY(hipEventRecord(ev[2 * k + 1], E->stream));
+                    launch_str_step(a, da, ls, 2, E->stream);
+                    HIP_TRY(hipEventRecord(ev[2 * k + 2], E->stream));
+                }
+                HIP_TRY(hipEventSynchronize(ev[2 * reps]));
                 float mv = 0.0f, ms = 0.0f;
-                ENG_TRY(timed(true, a, ls, &mv));
-                ENG_TRY(timed(false, a, ls, &ms));
+                for (int k = 0; k < reps; k++) {
+                    float x = 0.0f, y = 0.0f;
+                    HIP_TRY(hipEventElapsedTime(&x, ev[2 * k], ev[2 * k + 1]));
+                    HIP_TRY(hipEventElapsedTime(&y, ev[2 * k + 1], ev[2 * k + 2]));
+                    mv += x;
+                    ms += y;
+                }
+                mv /= reps;
+                ms /= reps;
                 if (mv < best_v) best_v = mv, grid_v = ls.grid_v;
                 if (ms < best_s) best_s = ms, grid_s = ls.grid;
             }
@@ -1469,6 +1478,7 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
                 pick.grid = grid_s;
             }
         }
+    for (auto &x : ev) hipEventDestroy(x);
     hipEventDestroy(t0);
     hipEventDestroy(t1);
     E->shape = pick;

@@ -26,6 +26,7 @@ extern "C" void ifos_set_error(const char *fmt, ...);
 namespace ifos {
 
 constexpr int kMaxL = 8;
+constexpr int kMaxN = 6;  // FDORDER/2 up to 12/2
 
 // Every buffer the kernels touch is device global memory.  Pointers that
 // reach a kernel through memory (an argument block in device memory) are

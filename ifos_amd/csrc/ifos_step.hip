@@ -26,6 +26,23 @@ namespace ifos {
 
 __device__ __forceinline__ int step_number(const StepArgs &a, int k) { return a.base ? *gmem(a.base) + k + 1 : k; }
 
+// Per-launch step flags, decided on the host: kSnapStep for a forward step
+// that stores a snapshot, (nt-1) % DTINV == 0 (src/IFOS2D.c:1805); kCorrStep
+// for an adjoint step that correlates, (NT-nt) % DTINV == 0
+// (src/IFOS2D.c:2381).  Inside a graph unit (a.base set) step nt is
+// base + k + 1 with base a multiple of the unit, and the unit a multiple of
+// DTINV dividing NT, so both tests depend on k alone.
+constexpr int kSnapStep = 1, kCorrStep = 2;
+
+static int step_flags(const StepArgs &a, int k)
+{
+    const int D = a.DTINV > 0 ? a.DTINV : 1;
+    int f = 0;
+    if (a.sw == 0 && (a.snap[0] || a.snap16[0]) && (a.base ? k % D == 0 : (k - 1) % D == 0)) f |= kSnapStep;
+    if (a.sw == 1 && (a.base ? (k + 1) % D == 0 : (a.NT - k) % D == 0)) f |= kCorrStep;
+    return f;
+}
+
 // Workgroups are dealt round-robin to the 8 XCDs (workgroup h runs on XCD
 // h % 8), and each XCD has its own L2.  With swizzle on, XCD x works through
 // one contiguous run [lo, hi) of the work items in row-major tile order, so
@@ -47,33 +64,59 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t mul, uint32_t sh)
     return (((n - t) >> 1) + t) >> sh;
 }
 
-// tile and first slot of work item it
-__device__ __forceinline__ void item_of(const StepArgs &a, int it, int spb, int &blk, int &s0)
+// The work-item mapping of a launch, read from the argument block into
+// locals once per workgroup, so that these loads issue together instead of
+// one by one where they are first used.
+struct ItemMap {
+    int nbx, ntiles, nslots, ngroups, slot_major, swizzle;
+    uint32_t nbx_mul, nbx_sh, ng_mul, ng_sh, nt_mul, nt_sh;
+};
+
+__device__ __forceinline__ ItemMap item_map(const StepArgs &a)
 {
-    if (a.slot_major) {
-        const int grp = (int)fdiv(it, a.nt_mul, a.nt_sh);
-        blk = it - grp * (a.nbx * a.nby);
+    ItemMap m;
+    m.nbx = a.nbx;
+    m.ntiles = a.nbx * a.nby;
+    m.nslots = a.nslots;
+    m.ngroups = a.ngroups;
+    m.slot_major = a.slot_major;
+    m.swizzle = a.swizzle;
+    m.nbx_mul = a.nbx_mul;
+    m.nbx_sh = a.nbx_sh;
+    m.ng_mul = a.ng_mul;
+    m.ng_sh = a.ng_sh;
+    m.nt_mul = a.nt_mul;
+    m.nt_sh = a.nt_sh;
+    return m;
+}
+
+// tile and first slot of work item it
+__device__ __forceinline__ void item_of(const ItemMap &m, int it, int spb, int &blk, int &s0)
+{
+    if (m.slot_major) {
+        const int grp = (int)fdiv(it, m.nt_mul, m.nt_sh);
+        blk = it - grp * m.ntiles;
         s0 = grp * spb;
     } else {
-        blk = (int)fdiv(it, a.ng_mul, a.ng_sh);
-        s0 = (it - blk * a.ngroups) * spb;
+        blk = (int)fdiv(it, m.ng_mul, m.ng_sh);
+        s0 = (it - blk * m.ngroups) * spb;
     }
 }
 
 // items [first, end) with the given stride for this workgroup: one item per
 // workgroup, or a persistent grid (gridDim.x a multiple of 8) looping over
 // the items of its XCD's run
-__device__ __forceinline__ void work_range(const StepArgs &a, int nitems, bool persistent, int &first, int &end,
+__device__ __forceinline__ void work_range(const ItemMap &m, int nitems, bool persistent, int &first, int &end,
                                            int &stride)
 {
     const int h = blockIdx.x;
     int lo = 0, hi = nitems;
-    if (a.swizzle) xcd_chunk(nitems, h % 8, lo, hi);
+    if (m.swizzle) xcd_chunk(nitems, h % 8, lo, hi);
     if (!persistent) {
-        first = a.swizzle ? lo + h / 8 : h;
+        first = m.swizzle ? lo + h / 8 : h;
         end = first + 1;
         stride = 1;
-    } else if (a.swizzle) {
+    } else if (m.swizzle) {
         first = lo + h / 8;
         end = hi;
         stride = gridDim.x / 8;
@@ -174,7 +217,7 @@ __device__ __forceinline__ float snap_load(const StepArgs &a, int s, int f, long
 // velocity step of one cell of slot s
 template <int N>
 __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int i, int j, bool live, long c, long e,
-                                         int nt, float rip, float rjp, float (*s_red)[kTileY])
+                                         int nt, int flags, float rip, float rjp, float (*s_red)[kTileY])
 {
     const StepGeom &g = a.g;
     const int P = g.pitch;
@@ -193,7 +236,7 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     const long so = (long)s * (a.nbx * a.nby + 1) + blk;
     const int q0 = gmem(a.inj.start)[so], q1 = gmem(a.inj.start)[so + 1];
     if (a.sw == 0) {
-        if (has_snap(a) && (nt - 1) % a.DTINV == 0) {
+        if (flags & kSnapStep) {
             const float val[2] = {vel_rate(rip, sxx_x + sxy_y, g.DH), vel_rate(rjp, sxy_x + syy_y, g.DH)};
             snap_store<2>(a, s, 0, (nt - 1) / a.DTINV, e, blk, live, val, s_red);
         }
@@ -201,7 +244,7 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     } else if (a.VELOCITY == 0) {
         gmem(a.vxp1)[wo + c] += vx0 * g.DT;
         gmem(a.vyp1)[wo + c] += vy0 * g.DT;
-    } else if ((a.NT - nt) % a.DTINV == 0) {
+    } else if (flags & kCorrStep) {
         gmem(a.vxp1)[wo + c] = vx0;
         gmem(a.vyp1)[wo + c] = vy0;
     }
@@ -239,19 +282,19 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
 }
 
 template <int N, int SPB>
-__global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ pa, int k, int persistent)
+__global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ pa, int k, int persistent, int flags)
 {
     const StepArgs &a = *pa;
     __shared__ float s_red[2][kTileY];
     const StepGeom &g = a.g;
-    const int ngroups = a.ngroups;
+    const ItemMap m = item_map(a);
     const int nt = step_number(a, k);
     int first, end, stride;
-    work_range(a, a.nbx * a.nby * ngroups, persistent != 0, first, end, stride);
+    work_range(m, m.ntiles * m.ngroups, persistent != 0, first, end, stride);
     for (int it = first; it < end; it += stride) {
         int blk, s0;
-        item_of(a, it, SPB, blk, s0);
-        const int by = (int)fdiv(blk, a.nbx_mul, a.nbx_sh), bx = blk - by * a.nbx;
+        item_of(m, it, SPB, blk, s0);
+        const int by = (int)fdiv(blk, m.nbx_mul, m.nbx_sh), bx = blk - by * m.nbx;
         const int i = bx * kTileX + threadIdx.x + 1;
         const int j = by * kTileY + threadIdx.y + 1;
         const bool live = i <= g.NX && j <= g.NY;
@@ -260,7 +303,7 @@ __global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ p
         const float rip = live ? gmem(a.rip)[c] : 0.0f, rjp = live ? gmem(a.rjp)[c] : 0.0f;
 #pragma unroll
         for (int u = 0; u < SPB; u++)
-            if (s0 + u < a.nslots) vel_cell<N>(a, s0 + u, blk, i, j, live, c, e, nt, rip, rjp, s_red);
+            if (s0 + u < m.nslots) vel_cell<N>(a, s0 + u, blk, i, j, live, c, e, nt, flags, rip, rjp, s_red);
     }
 }
 
@@ -275,7 +318,7 @@ struct StressMat {
 // stress step of one cell of slot s
 template <int N, int L>
 __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int i, int j, bool live, bool top, long c,
-                                         long e, int nt, const StressMat<L> &M, float (*s_syy)[kTileX],
+                                         long e, int nt, int flags, const StressMat<L> &M, float (*s_syy)[kTileX],
                                          float (*s_sxy)[kTileX], float (*s_red)[kTileY])
 {
     const StepGeom &g = a.g;
@@ -283,8 +326,8 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
     const int P = g.pitch;
     const long wo = (long)s * a.ws, po = (long)s * a.pss;
     const auto vxs = gmem(a.vx) + wo, vys = gmem(a.vy) + wo;
-    const bool snap_step = a.sw == 0 && has_snap(a) && (nt - 1) % a.DTINV == 0;
-    const bool corr_step = a.sw == 1 && (a.NT - nt) % a.DTINV == 0;
+    const bool snap_step = (flags & kSnapStep) != 0;
+    const bool corr_step = (flags & kCorrStep) != 0;
     const bool rates = snap_step && a.VELOCITY == 1;
     // bounds of this tile's explosive-source and receiver lists, loaded
     // before the stencil work so that their latency is hidden behind it
@@ -403,21 +446,21 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
 }
 
 template <int N, int L, int SPB>
-__global__ __launch_bounds__(512) void k_str_step(const StepArgs *__restrict__ pa, int k, int persistent)
+__global__ __launch_bounds__(512) void k_str_step(const StepArgs *__restrict__ pa, int k, int persistent, int flags)
 {
     const StepArgs &a = *pa;
     __shared__ float s_syy[kTileY][kTileX];
     __shared__ float s_sxy[kTileY][kTileX];
     __shared__ float s_red[3][kTileY];
     const StepGeom &g = a.g;
-    const int ngroups = a.ngroups;
+    const ItemMap m = item_map(a);
     const int nt = step_number(a, k);
     int first, end, stride;
-    work_range(a, a.nbx * a.nby * ngroups, persistent != 0, first, end, stride);
+    work_range(m, m.ntiles * m.ngroups, persistent != 0, first, end, stride);
     for (int it = first; it < end; it += stride) {
         int blk, s0;
-        item_of(a, it, SPB, blk, s0);
-        const int by = (int)fdiv(blk, a.nbx_mul, a.nbx_sh), bx = blk - by * a.nbx;
+        item_of(m, it, SPB, blk, s0);
+        const int by = (int)fdiv(blk, m.nbx_mul, m.nbx_sh), bx = blk - by * m.nbx;
         const int i = bx * kTileX + threadIdx.x + 1;
         const int j = by * kTileY + threadIdx.y + 1;
         const bool live = i <= g.NX && j <= g.NY;
@@ -438,7 +481,8 @@ __global__ __launch_bounds__(512) void k_str_step(const StepArgs *__restrict__ p
         }
 #pragma unroll
         for (int u = 0; u < SPB; u++)
-            if (s0 + u < a.nslots) str_cell<N, L>(a, s0 + u, blk, i, j, live, top, c, e, nt, M, s_syy, s_sxy, s_red);
+            if (s0 + u < m.nslots)
+                str_cell<N, L>(a, s0 + u, blk, i, j, live, top, c, e, nt, flags, M, s_syy, s_sxy, s_red);
     }
 }
 
@@ -486,7 +530,8 @@ template <int N, int SPB>
 static void vel_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
 {
     const int gr = grid_of(a, ls, ls.grid_v);
-    hipLaunchKernelGGL((k_vel_step<N, SPB>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid_v > 0 ? 1 : 0);
+    hipLaunchKernelGGL((k_vel_step<N, SPB>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid_v > 0 ? 1 : 0,
+                       step_flags(a, k));
 }
 
 template <int N>
@@ -517,7 +562,8 @@ template <int N, int L, int SPB>
 static void str_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
 {
     const int gr = grid_of(a, ls, ls.grid);
-    hipLaunchKernelGGL((k_str_step<N, L, SPB>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid > 0 ? 1 : 0);
+    hipLaunchKernelGGL((k_str_step<N, L, SPB>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid > 0 ? 1 : 0,
+                       step_flags(a, k));
 }
 
 template <int N, int L>
