@@ -39,37 +39,37 @@ __device__ __forceinline__ void cpml_velocity(const StepGeom &g, int j, int i, f
         const int h = i - 1;
         auto p1 = gmem(psi_sxx_x) + (long)(j - 1) * g.px + h;
         auto p2 = gmem(psi_sxy_x) + (long)(j - 1) * g.px + h;
-        *p1 = gmem(pm.b_xh)[h] * *p1 + gmem(pm.a_xh)[h] * sxx_x;
-        sxx_x = sxx_x / gmem(pm.K_xh)[h] + *p1;
-        *p2 = gmem(pm.b_x)[h] * *p2 + gmem(pm.a_x)[h] * sxy_x;
-        sxy_x = sxy_x / gmem(pm.K_x)[h] + *p2;
+        *p1 = cmem(pm.b_xh)[h] * *p1 + cmem(pm.a_xh)[h] * sxx_x;
+        sxx_x = sxx_x / cmem(pm.K_xh)[h] + *p1;
+        *p2 = cmem(pm.b_x)[h] * *p2 + cmem(pm.a_x)[h] * sxy_x;
+        sxy_x = sxy_x / cmem(pm.K_x)[h] + *p2;
     }
     if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
         const int h = i - g.NX + 2 * g.FW - 1;
         auto p1 = gmem(psi_sxx_x) + (long)(j - 1) * g.px + h;
         auto p2 = gmem(psi_sxy_x) + (long)(j - 1) * g.px + h;
-        *p1 = gmem(pm.b_xh)[h] * *p1 + gmem(pm.a_xh)[h] * sxx_x;
-        sxx_x = sxx_x / gmem(pm.K_xh)[h] + *p1;
-        *p2 = gmem(pm.b_x)[h] * *p2 + gmem(pm.a_x)[h] * sxy_x;
-        sxy_x = sxy_x / gmem(pm.K_x)[h] + *p2;
+        *p1 = cmem(pm.b_xh)[h] * *p1 + cmem(pm.a_xh)[h] * sxx_x;
+        sxx_x = sxx_x / cmem(pm.K_xh)[h] + *p1;
+        *p2 = cmem(pm.b_x)[h] * *p2 + cmem(pm.a_x)[h] * sxy_x;
+        sxy_x = sxy_x / cmem(pm.K_x)[h] + *p2;
     }
     if (!g.FREE_SURF && j <= g.FW) {
         const int h = j - 1;
         auto p1 = gmem(psi_syy_y) + (long)h * g.py + (i - 1);
         auto p2 = gmem(psi_sxy_y) + (long)h * g.py + (i - 1);
-        *p1 = gmem(pm.b_yh)[h] * *p1 + gmem(pm.a_yh)[h] * syy_y;
-        syy_y = syy_y / gmem(pm.K_yh)[h] + *p1;
-        *p2 = gmem(pm.b_y)[h] * *p2 + gmem(pm.a_y)[h] * sxy_y;
-        sxy_y = sxy_y / gmem(pm.K_y)[h] + *p2;
+        *p1 = cmem(pm.b_yh)[h] * *p1 + cmem(pm.a_yh)[h] * syy_y;
+        syy_y = syy_y / cmem(pm.K_yh)[h] + *p1;
+        *p2 = cmem(pm.b_y)[h] * *p2 + cmem(pm.a_y)[h] * sxy_y;
+        sxy_y = sxy_y / cmem(pm.K_y)[h] + *p2;
     }
     if (j >= g.NY - g.FW + 1) {
         const int h = j - g.NY + 2 * g.FW - 1;
         auto p1 = gmem(psi_syy_y) + (long)h * g.py + (i - 1);
         auto p2 = gmem(psi_sxy_y) + (long)h * g.py + (i - 1);
-        *p1 = gmem(pm.b_yh)[h] * *p1 + gmem(pm.a_yh)[h] * syy_y;
-        syy_y = syy_y / gmem(pm.K_yh)[h] + *p1;
-        *p2 = gmem(pm.b_y)[h] * *p2 + gmem(pm.a_y)[h] * sxy_y;
-        sxy_y = sxy_y / gmem(pm.K_y)[h] + *p2;
+        *p1 = cmem(pm.b_yh)[h] * *p1 + cmem(pm.a_yh)[h] * syy_y;
+        syy_y = syy_y / cmem(pm.K_yh)[h] + *p1;
+        *p2 = cmem(pm.b_y)[h] * *p2 + cmem(pm.a_y)[h] * sxy_y;
+        sxy_y = sxy_y / cmem(pm.K_y)[h] + *p2;
     }
 }
 
@@ -107,37 +107,37 @@ __device__ __forceinline__ void cpml_stress(const StepGeom &g, int j, int i, flo
         const int h = i - 1;
         auto p1 = gmem(psi_vxx) + (long)(j - 1) * g.px + h;
         auto p2 = gmem(psi_vyx) + (long)(j - 1) * g.px + h;
-        *p1 = gmem(pm.b_x)[h] * *p1 + gmem(pm.a_x)[h] * vxx;
-        vxx = vxx / gmem(pm.K_x)[h] + *p1;
-        *p2 = gmem(pm.b_xh)[h] * *p2 + gmem(pm.a_xh)[h] * vyx;
-        vyx = vyx / gmem(pm.K_xh)[h] + *p2;
+        *p1 = cmem(pm.b_x)[h] * *p1 + cmem(pm.a_x)[h] * vxx;
+        vxx = vxx / cmem(pm.K_x)[h] + *p1;
+        *p2 = cmem(pm.b_xh)[h] * *p2 + cmem(pm.a_xh)[h] * vyx;
+        vyx = vyx / cmem(pm.K_xh)[h] + *p2;
     }
     if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
         const int h = i - g.NX + 2 * g.FW - 1;
         auto p1 = gmem(psi_vxx) + (long)(j - 1) * g.px + h;
         auto p2 = gmem(psi_vyx) + (long)(j - 1) * g.px + h;
-        *p1 = gmem(pm.b_x)[h] * *p1 + gmem(pm.a_x)[h] * vxx;
-        vxx = vxx / gmem(pm.K_x)[h] + *p1;
-        *p2 = gmem(pm.b_xh)[h] * *p2 + gmem(pm.a_xh)[h] * vyx;
-        vyx = vyx / gmem(pm.K_xh)[h] + *p2;
+        *p1 = cmem(pm.b_x)[h] * *p1 + cmem(pm.a_x)[h] * vxx;
+        vxx = vxx / cmem(pm.K_x)[h] + *p1;
+        *p2 = cmem(pm.b_xh)[h] * *p2 + cmem(pm.a_xh)[h] * vyx;
+        vyx = vyx / cmem(pm.K_xh)[h] + *p2;
     }
     if (!g.FREE_SURF && j <= g.FW) {
         const int h = j - 1;
         auto p1 = gmem(psi_vyy) + (long)h * g.py + (i - 1);
         auto p2 = gmem(psi_vxy) + (long)h * g.py + (i - 1);
-        *p1 = gmem(pm.b_y)[h] * *p1 + gmem(pm.a_y)[h] * vyy;
-        *p2 = gmem(pm.b_yh)[h] * *p2 + gmem(pm.a_yh)[h] * vxy;
-        vyy = vyy / gmem(pm.K_y)[h] + *p1;
-        vxy = vxy / gmem(pm.K_yh)[h] + *p2;
+        *p1 = cmem(pm.b_y)[h] * *p1 + cmem(pm.a_y)[h] * vyy;
+        *p2 = cmem(pm.b_yh)[h] * *p2 + cmem(pm.a_yh)[h] * vxy;
+        vyy = vyy / cmem(pm.K_y)[h] + *p1;
+        vxy = vxy / cmem(pm.K_yh)[h] + *p2;
     }
     if (j >= g.NY - g.FW + 1) {
         const int h = j - g.NY + 2 * g.FW - 1;
         auto p1 = gmem(psi_vyy) + (long)h * g.py + (i - 1);
         auto p2 = gmem(psi_vxy) + (long)h * g.py + (i - 1);
-        *p1 = gmem(pm.b_y)[h] * *p1 + gmem(pm.a_y)[h] * vyy;
-        vyy = vyy / gmem(pm.K_y)[h] + *p1;
-        *p2 = gmem(pm.b_yh)[h] * *p2 + gmem(pm.a_yh)[h] * vxy;
-        vxy = vxy / gmem(pm.K_yh)[h] + *p2;
+        *p1 = cmem(pm.b_y)[h] * *p1 + cmem(pm.a_y)[h] * vyy;
+        vyy = vyy / cmem(pm.K_y)[h] + *p1;
+        *p2 = cmem(pm.b_yh)[h] * *p2 + cmem(pm.a_yh)[h] * vxy;
+        vxy = vxy / cmem(pm.K_yh)[h] + *p2;
     }
 }
 
@@ -276,14 +276,14 @@ __device__ __forceinline__ void surface_derivs(const StepGeom &g, int i, const f
     if (!g.BOUNDARY && i <= g.FW) {
         const int h = i - 1;
         auto ps = gmem(psi_vxxs) + h;
-        *ps = gmem(pm.b_x)[h] * *ps + gmem(pm.a_x)[h] * vxx;
-        vxx = vxx / gmem(pm.K_x)[h] + *ps;
+        *ps = cmem(pm.b_x)[h] * *ps + cmem(pm.a_x)[h] * vxx;
+        vxx = vxx / cmem(pm.K_x)[h] + *ps;
     }
     if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
         const int h = i - g.NX + 2 * g.FW - 1;
         auto ps = gmem(psi_vxxs) + h;
-        *ps = gmem(pm.b_x)[h] * *ps + gmem(pm.a_x)[h] * vxx;
-        vxx = vxx / gmem(pm.K_x)[h] + *ps;
+        *ps = cmem(pm.b_x)[h] * *ps + cmem(pm.a_x)[h] * vxx;
+        vxx = vxx / cmem(pm.K_x)[h] + *ps;
     }
 }
 

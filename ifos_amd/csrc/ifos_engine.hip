@@ -712,7 +712,7 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
     int unit = c.DTINV;
     while (unit < 32 && NT % (unit * 2) == 0) unit *= 2;
     const int kind = sw;
-    const int key[6] = {E->nsrc, E->ntr, snapshots ? 1 : 0, E->nslots,
+    const int key[6] = {E->nsrc, E->ntr, (snapshots ? 1 : 0) + 2 * (int)(E->shape.cpml_weight * 4.0f), E->nslots,
                         ((E->shape.order * 8 + E->shape.spb) * 4096 + E->shape.grid_v) * 4096 + E->shape.grid, unit};
     const LaunchShape ls = E->shape;
     StepArgs a = shaped_args(step_args(E, sw, snapshots), ls);
@@ -1436,12 +1436,14 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
     reps = std::min(reps, kMaxReps);
     hipEvent_t ev[2 * kMaxReps + 1];
     for (auto &x : ev) HIP_TRY(hipEventCreate(&x));
+    for (float cw : {0.0f, 1.0f, 2.0f})
     for (int order = 0; order < (E->nslots > 1 ? 2 : 1); order++)
         for (int spb : {1, 2, 4}) {
             if (spb > 1 && spb > E->nslots) break;
             LaunchShape ls;
             ls.spb = spb;
             ls.order = order;
+            ls.cpml_weight = cw;
             const StepArgs a = shaped_args(a0, ls);
             ENG_TRY(upload_args(E, a, 3, &da));
             float best_v = 1e30f, best_s = 1e30f;

@@ -44,6 +44,15 @@ __device__ __forceinline__ const IFOS_GAS T *gmem(const T *p)
 {
     return (const IFOS_GAS T *)p;
 }
+// read-only data of a launch (tile lists, CPML profiles, flags) through the
+// constant address space: the compiler may keep such loads scalar and
+// schedule them freely, since nothing writes them while the kernel runs
+#define IFOS_CAS __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const IFOS_CAS T *cmem(const T *p)
+{
+    return (const IFOS_CAS T *)p;
+}
 
 struct Cpml {
     const float *K_x, *a_x, *b_x, *K_xh, *a_xh, *b_xh;

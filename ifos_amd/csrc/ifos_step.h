@@ -74,6 +74,9 @@ struct StepArgs {
     // shift (set by the launcher): the quotients of every work item without
     // integer division
     uint32_t nbx_mul, nbx_sh, ng_mul, ng_sh, nt_mul, nt_sh;
+    // with swizzle: the work items of XCD x are [xlo[x], xlo[x+1]), runs of
+    // equal estimated cost (set by the launcher)
+    int xlo[9];
 };
 
 // launch shape of the step kernels: spb slots per workgroup (1, 2 or 4; the
@@ -86,6 +89,7 @@ struct LaunchShape {
     int grid = 0;    // stress step
     int grid_v = 0;  // velocity step
     int order = 0;   // StepArgs::slot_major
+    float cpml_weight = 1.0f;  // extra cost of a tile in an absorbing strip, for the XCD split
 };
 
 // a: host copy of the argument block (grid sizes), da: the same block in

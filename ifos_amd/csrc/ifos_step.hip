@@ -19,6 +19,9 @@
 // the top tile row holds them; they are exchanged through LDS after a barrier.
 // The arithmetic is the per-cell code of ifos_cellmath.h, bit-identical to
 // the reference.
+#include <algorithm>
+#include <vector>
+
 #include "ifos_cellmath.h"
 #include "ifos_step.h"
 
@@ -45,15 +48,10 @@ static int step_flags(const StepArgs &a, int k)
 
 // Workgroups are dealt round-robin to the 8 XCDs (workgroup h runs on XCD
 // h % 8), and each XCD has its own L2.  With swizzle on, XCD x works through
-// one contiguous run [lo, hi) of the work items in row-major tile order, so
-// the halos neighbouring tiles share, and the materials the slot groups of a
-// tile share, are mostly fetched into one L2 once.
-__device__ __forceinline__ void xcd_chunk(int nitems, int x, int &lo, int &hi)
-{
-    const int per = nitems / 8, rem = nitems % 8;
-    lo = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
-    hi = lo + per + (x < rem ? 1 : 0);
-}
+// one contiguous run [xlo[x], xlo[x+1]) of the work items in row-major tile
+// order, so the halos neighbouring tiles share, and the materials the slot
+// groups of a tile share, are mostly fetched into one L2 once.  The runs have
+// equal estimated cost rather than equal length (shaped_args).
 
 // n / d for 0 <= n < 2^31 from (mul, sh) of make_div: t = mulhi(n, mul),
 // q = (((n - t) >> 1) + t) >> sh; sh = 0 with mul = 0 marks d = 1
@@ -70,6 +68,7 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t mul, uint32_t sh)
 struct ItemMap {
     int nbx, ntiles, nslots, ngroups, slot_major, swizzle;
     uint32_t nbx_mul, nbx_sh, ng_mul, ng_sh, nt_mul, nt_sh;
+    int lo, hi;  // this workgroup's XCD run of items
 };
 
 __device__ __forceinline__ ItemMap item_map(const StepArgs &a)
@@ -87,6 +86,9 @@ __device__ __forceinline__ ItemMap item_map(const StepArgs &a)
     m.ng_sh = a.ng_sh;
     m.nt_mul = a.nt_mul;
     m.nt_sh = a.nt_sh;
+    const int x = blockIdx.x % 8;
+    m.lo = a.xlo[x];
+    m.hi = a.xlo[x + 1];
     return m;
 }
 
@@ -111,10 +113,13 @@ __device__ __forceinline__ void work_range(const ItemMap &m, int nitems, bool pe
 {
     const int h = blockIdx.x;
     int lo = 0, hi = nitems;
-    if (m.swizzle) xcd_chunk(nitems, h % 8, lo, hi);
+    if (m.swizzle) {
+        lo = m.lo;
+        hi = m.hi;
+    }
     if (!persistent) {
         first = m.swizzle ? lo + h / 8 : h;
-        end = first + 1;
+        end = m.swizzle ? (first < hi ? first + 1 : first) : first + 1;
         stride = 1;
     } else if (m.swizzle) {
         first = lo + h / 8;
@@ -233,8 +238,8 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     }
     // bounds of this tile's point-force / residual list, loaded behind the
     // stencil loads so that their latency overlaps them
-    const long so = (long)s * (a.nbx * a.nby + 1) + blk;
-    const int q0 = gmem(a.inj.start)[so], q1 = gmem(a.inj.start)[so + 1];
+    const int so = __builtin_amdgcn_readfirstlane(s * (a.nbx * a.nby + 1) + blk);  // uniform
+    const int q0 = cmem(a.inj.start)[so], q1 = cmem(a.inj.start)[so + 1];
     if (a.sw == 0) {
         if (flags & kSnapStep) {
             const float val[2] = {vel_rate(rip, sxx_x + sxy_y, g.DH), vel_rate(rjp, sxy_x + syy_y, g.DH)};
@@ -294,6 +299,9 @@ __global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ p
     for (int it = first; it < end; it += stride) {
         int blk, s0;
         item_of(m, it, SPB, blk, s0);
+        // uniform across the workgroup: keep them in scalar registers
+        blk = __builtin_amdgcn_readfirstlane(blk);
+        s0 = __builtin_amdgcn_readfirstlane(s0);
         const int by = (int)fdiv(blk, m.nbx_mul, m.nbx_sh), bx = blk - by * m.nbx;
         const int i = bx * kTileX + threadIdx.x + 1;
         const int j = by * kTileY + threadIdx.y + 1;
@@ -333,12 +341,12 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
     // before the stencil work so that their latency is hidden behind it
     int expl = 0, ex0 = 0, ex1 = 0, rc0 = 0, rc1 = 0;
     if (a.sw == 0) {
-        const long so = (long)s * (a.nbx * a.nby + 1) + blk;
-        expl = gmem(a.explosive)[s];
-        ex0 = gmem(a.exp.start)[so];
-        ex1 = gmem(a.exp.start)[so + 1];
-        rc0 = gmem(a.rec.start)[so];
-        rc1 = gmem(a.rec.start)[so + 1];
+        const int so = __builtin_amdgcn_readfirstlane(s * (a.nbx * a.nby + 1) + blk);  // uniform
+        expl = cmem(a.explosive)[s];
+        ex0 = cmem(a.exp.start)[so];
+        ex1 = cmem(a.exp.start)[so + 1];
+        rc0 = cmem(a.rec.start)[so];
+        rc1 = cmem(a.rec.start)[so + 1];
     }
 
     float sxx = 0.0f, syy = 0.0f, sxy = 0.0f, ux = 0.0f, uy = 0.0f, uxy = 0.0f;
@@ -460,6 +468,9 @@ __global__ __launch_bounds__(512) void k_str_step(const StepArgs *__restrict__ p
     for (int it = first; it < end; it += stride) {
         int blk, s0;
         item_of(m, it, SPB, blk, s0);
+        // uniform across the workgroup: keep them in scalar registers
+        blk = __builtin_amdgcn_readfirstlane(blk);
+        s0 = __builtin_amdgcn_readfirstlane(s0);
         const int by = (int)fdiv(blk, m.nbx_mul, m.nbx_sh), bx = blk - by * m.nbx;
         const int i = bx * kTileX + threadIdx.x + 1;
         const int j = by * kTileY + threadIdx.y + 1;
@@ -517,13 +528,40 @@ StepArgs shaped_args(const StepArgs &a0, const LaunchShape &ls)
     make_div((uint32_t)a.nbx, a.nbx_mul, a.nbx_sh);
     make_div((uint32_t)a.ngroups, a.ng_mul, a.ng_sh);
     make_div((uint32_t)(a.nbx * a.nby), a.nt_mul, a.nt_sh);
+    // XCD runs of equal estimated cost: a tile in an absorbing strip costs
+    // 1 + cpml_weight (its cells update the CPML memory and divide by K);
+    // the tiles of the bottom strip would otherwise all fall to the last XCD
+    const StepGeom &g = a.g;
+    const int ntiles = a.nbx * a.nby, nitems = ntiles * a.ngroups;
+    auto in_cpml = [&](int blk) {
+        const int by = blk / a.nbx, bx = blk - by * a.nbx;
+        const int i0 = bx * kTileX + 1, i1 = i0 + kTileX - 1, j0 = by * kTileY + 1, j1 = j0 + kTileY - 1;
+        return (!g.BOUNDARY && (i0 <= g.FW || i1 >= g.NX - g.FW + 1)) || (!g.FREE_SURF && j0 <= g.FW) ||
+               j1 >= g.NY - g.FW + 1;
+    };
+    std::vector<double> cum(nitems + 1, 0.0);
+    for (int it = 0; it < nitems; it++) {
+        const int blk = a.slot_major ? it % ntiles : it / a.ngroups;
+        cum[it + 1] = cum[it] + 1.0 + (in_cpml(blk) ? (double)ls.cpml_weight : 0.0);
+    }
+    a.xlo[0] = 0;
+    for (int x = 1; x < 8; x++) {
+        const double target = cum[nitems] * x / 8.0;
+        a.xlo[x] = (int)(std::lower_bound(cum.begin(), cum.end(), target) - cum.begin());
+        a.xlo[x] = std::max(a.xlo[x], a.xlo[x - 1]);
+    }
+    a.xlo[8] = nitems;
     return a;
 }
 
 static int grid_of(const StepArgs &a, const LaunchShape &ls, int grid)
 {
     const int nitems = a.nbx * a.nby * ((a.nslots + ls.spb - 1) / ls.spb);
-    return grid > 0 ? std::min(grid, (nitems + 7) / 8 * 8) : nitems;
+    if (grid > 0) return std::min(grid, (nitems + 7) / 8 * 8);
+    if (!a.swizzle) return nitems;
+    int widest = 0;  // one workgroup per item of the largest XCD run
+    for (int x = 0; x < 8; x++) widest = std::max(widest, a.xlo[x + 1] - a.xlo[x]);
+    return 8 * widest;
 }
 
 template <int N, int SPB>
