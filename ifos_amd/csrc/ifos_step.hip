@@ -35,7 +35,11 @@ __device__ __forceinline__ int step_number(const StepArgs &a, int k) { return a.
 // (src/IFOS2D.c:2381).  Inside a graph unit (a.base set) step nt is
 // base + k + 1 with base a multiple of the unit, and the unit a multiple of
 // DTINV dividing NT, so both tests depend on k alone.
-constexpr int kSnapStep = 1, kCorrStep = 2;
+// The other flags restate fields of the argument block that every wave
+// would otherwise load before it can branch: kForward (sw == 0), kAdjInt
+// (adjoint with VELOCITY == 0: the integrated adjoint velocity,
+// src/update_v_PML.c:140-143) and kEnergy (the preconditioner's energy sums).
+constexpr int kSnapStep = 1, kCorrStep = 2, kForward = 4, kAdjInt = 8, kEnergy = 16;
 
 static int step_flags(const StepArgs &a, int k)
 {
@@ -43,6 +47,9 @@ static int step_flags(const StepArgs &a, int k)
     int f = 0;
     if (a.sw == 0 && (a.snap[0] || a.snap16[0]) && (a.base ? k % D == 0 : (k - 1) % D == 0)) f |= kSnapStep;
     if (a.sw == 1 && (a.base ? (k + 1) % D == 0 : (a.NT - k) % D == 0)) f |= kCorrStep;
+    if (a.sw == 0) f |= kForward;
+    if (a.sw == 1 && a.VELOCITY == 0) f |= kAdjInt;
+    if (a.wacc) f |= kEnergy;
     return f;
 }
 
@@ -240,13 +247,13 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     // stencil loads so that their latency overlaps them
     const int so = __builtin_amdgcn_readfirstlane(s * (a.nbx * a.nby + 1) + blk);  // uniform
     const int q0 = cmem(a.inj.start)[so], q1 = cmem(a.inj.start)[so + 1];
-    if (a.sw == 0) {
+    if (flags & kForward) {
         if (flags & kSnapStep) {
             const float val[2] = {vel_rate(rip, sxx_x + sxy_y, g.DH), vel_rate(rjp, sxy_x + syy_y, g.DH)};
             snap_store<2>(a, s, 0, (nt - 1) / a.DTINV, e, blk, live, val, s_red);
         }
     } else if (!live) {
-    } else if (a.VELOCITY == 0) {
+    } else if (flags & kAdjInt) {
         gmem(a.vxp1)[wo + c] += vx0 * g.DT;
         gmem(a.vyp1)[wo + c] += vy0 * g.DT;
     } else if (flags & kCorrStep) {
@@ -280,7 +287,7 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     vxs[c] = vx;
     vys[c] = vy;
     // energy of the new particle velocities (eprecond, src/eprecond.c:8-21)
-    if (a.wacc) {
+    if (flags & kEnergy) {
         const auto W = gmem(a.wacc) + (long)s * a.wacs;
         W[e] += ((vx * vx) + (vy * vy));
     }
@@ -340,7 +347,7 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
     // bounds of this tile's explosive-source and receiver lists, loaded
     // before the stencil work so that their latency is hidden behind it
     int expl = 0, ex0 = 0, ex1 = 0, rc0 = 0, rc1 = 0;
-    if (a.sw == 0) {
+    if (flags & kForward) {
         const int so = __builtin_amdgcn_readfirstlane(s * (a.nbx * a.nby + 1) + blk);  // uniform
         expl = cmem(a.explosive)[s];
         ex0 = cmem(a.exp.start)[so];
@@ -439,7 +446,7 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
         gmem(a.wc)[wco + e] = w;
         gmem(a.wc_u)[wco + e] = wu;
     }
-    if (a.sw == 0) {
+    if (flags & kForward) {
         for (int q = rc0; q < rc1; q++) {
             const int t = gmem(a.rec.idx)[q];
             if (gmem(a.rec.ix)[t] != i || gmem(a.rec.iy)[t] != j) continue;
