@@ -520,6 +520,33 @@ __global__ __launch_bounds__(256) void k_smooth(const float *in, float *out, int
     out[(long)y * NX + x] = funkey(prefix);
 }
 
+// spat_filt (src/spat_filt.c:27): per output cell of the window a weighted
+// mean along x, accumulated in float with double products in the
+// reference's order, acc += g * pref * e[d], norm += pref * e[d]
+constexpr int kSpatMax = 64;
+struct SpatArgs {
+    const float *in;
+    float *out;
+    int NX, NY, S, J1;
+    double pref;
+    double e[2 * kSpatMax + 1];
+};
+
+__global__ __launch_bounds__(256) void k_spat_filt(SpatArgs a)
+{
+    const int i = a.S + 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = a.J1 + (int)blockIdx.y;
+    if (i > a.NX - a.S || j > a.NY - a.S) return;
+    const float *row = a.in + (long)(j - 1) * a.NX;
+    float acc = 0.0f, norm = 0.0f;
+    for (int i1 = i - a.S; i1 <= i + a.S; i1++) {
+        const double w = a.e[i1 - i + a.S];
+        acc = (float)((double)acc + ((double)row[i1 - 1] * a.pref) * w);
+        norm = (float)((double)norm + a.pref * w);
+    }
+    a.out[(long)(j - 1) * a.NX + (i - 1)] = acc / norm;
+}
+
 int ensure_work(ifos_opt *o, size_t floats, size_t bytes)
 {
     if (floats > o->work_cap) {
@@ -575,6 +602,40 @@ extern "C" int ifos_opt_smooth(ifos_opt *o, int NX, int NY, int filtsize, float 
     dim3 grid((NX + kSmTile - 1) / kSmTile, (NY + kSmTile - 1) / kSmTile);
     hipLaunchKernelGGL(k_smooth, grid, dim3(kSmTile * kSmTile), sizeof(unsigned int) * W * W, o->stream, o->work,
                        grad, NX, NY, h);
+    OPT_TRY(hipGetLastError());
+    OPT_TRY(hipStreamSynchronize(o->stream));
+    return 0;
+}
+
+extern "C" int ifos_opt_spat_filt(ifos_opt *o, int NX, int NY, int SPAT_FILT_SIZE, int SPAT_FILT_1, float *grad)
+{
+    OPT_TRY(hipSetDevice(o->device));
+    if (NX < 1 || NY < 1 || !grad) return ifos_set_error("ifos_opt_spat_filt: bad arguments"), -2;
+    const int S = SPAT_FILT_SIZE;
+    if (S < 1) return 0;
+    if (S > kSpatMax) return ifos_set_error("ifos_opt_spat_filt: SPAT_FILT_SIZE %d exceeds %d", S, kSpatMax), -2;
+    const int ni = NX - 2 * S, nj = NY - S - SPAT_FILT_1 + 1;
+    if (ni < 1 || nj < 1) return 0;
+    const size_t n = (size_t)NX * NY;
+    if (ensure_work(o, n, 0)) return -1;
+    OPT_TRY(hipMemcpyAsync(o->work, grad, sizeof(float) * n, hipMemcpyDeviceToDevice, o->stream));
+    SpatArgs a;
+    a.in = o->work;
+    a.out = grad;
+    a.NX = NX;
+    a.NY = NY;
+    a.S = S;
+    a.J1 = SPAT_FILT_1;
+    // the weight of offset d, with the reference's integer (-1/2) exponent
+    // factor (src/spat_filt.c:97-98), evaluated on the host
+    const double PI = 3.141592653589793;
+    a.pref = 1 / (sqrt(2 * PI) * S);
+    for (int d = -S; d <= S; d++) {
+        const double r = sqrt((double)(d * d + d * d));
+        a.e[d + S] = exp((-1 / 2) * (r * r) / (S * S));
+    }
+    dim3 b(256), gr((ni + 255) / 256, nj);
+    hipLaunchKernelGGL(k_spat_filt, gr, b, 0, o->stream, a);
     OPT_TRY(hipGetLastError());
     OPT_TRY(hipStreamSynchronize(o->stream));
     return 0;

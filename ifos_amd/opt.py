@@ -86,6 +86,7 @@ class Optimizer:
                                      _VOIDP, C.POINTER(C.c_double), _VOIDP, C.c_int, C.c_int, C.c_int]
         L.ifos_opt_taper.argtypes = [_VOIDP, C.c_long, C.POINTER(C.c_float), C.POINTER(C.c_ubyte), _VOIDP]
         L.ifos_opt_smooth.argtypes = [_VOIDP, C.c_int, C.c_int, C.c_int, _VOIDP, C.POINTER(C.c_int)]
+        L.ifos_opt_spat_filt.argtypes = [_VOIDP, C.c_int, C.c_int, C.c_int, C.c_int, _VOIDP]
         h = _VOIDP()
         _check(L.ifos_opt_create(device, C.byref(h)))
         self.h = h
@@ -158,3 +159,7 @@ class Optimizer:
         side = C.c_int()
         _check(lib().ifos_opt_smooth(self.h, NX, NY, filtsize, _p(grad), C.byref(side)))
         return side.value
+
+    def spat_filt(self, NX, NY, size, first_row, grad):
+        """spat_filt() of src/spat_filt.c on a device gradient, in place."""
+        _check(lib().ifos_opt_spat_filt(self.h, NX, NY, size, first_row, _p(grad)))

@@ -110,6 +110,13 @@ int ifos_opt_taper(ifos_opt *o, long n, const float *coef, const unsigned char *
  * the window without its last element, and it picks element k/2+1 of k.
  * *side receives the window side used.  Window sides up to 113. */
 int ifos_opt_smooth(ifos_opt *o, int NX, int NY, int filtsize, float *grad, int *side);
+/* Device, in place: spat_filt() (src/spat_filt.c:27, SPATFILTER, applied in
+ * PCG after the tapers, src/PCG.c:101-105) for columns S+1..NX-S and rows
+ * SPAT_FILT_1..NY-S, S = SPAT_FILT_SIZE (the caller reduces it by the
+ * iteration when SPAT_FILT_ITER is set; S < 1 does nothing; S <= 64).  The
+ * reference's weights are exp((-1/2) ...) with integer (-1/2), so the filter
+ * is a mean over the 2S+1 cells of the row; same values. */
+int ifos_opt_spat_filt(ifos_opt *o, int NX, int NY, int SPAT_FILT_SIZE, int SPAT_FILT_1, float *grad);
 
 /* Source taper table helpers (host): table side n and entries m[n][n]
  * (returns n; fills m when cap >= n*n), the table cell that sits on the

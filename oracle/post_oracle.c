@@ -7,6 +7,7 @@
  *                         taper_grad sws 3 src/taper_grad.c:252-398
  *   orc_taper_vertical    taper_grad sws 1 src/taper_grad.c:60-172
  *   orc_taper_horizontal  taper_grad sws 2 src/taper_grad.c:179-246
+ *   orc_spat_filt         spat_filt() src/spat_filt.c:27-150 (SPATFILTER)
  *   orc_median_smooth     smooth() src/smooth.c:24-160 (filter 1 and 2 differ
  *                         only in which global holds the size), median2d()
  *                         src/median2D.c:3-20 with the reference's off-by-one
@@ -244,4 +245,32 @@ int orc_median_smooth(int NX, int NY, int filtsize, const float *in, float *out)
         }
     free(win);
     return side;
+}
+
+/* --- spatial filter --------------------------------------------------------- */
+
+/* spat_filt (src/spat_filt.c:27-150) on the whole gradient: for columns
+ * S+1..NX-S and rows J1..NY-S, a weighted mean over columns i-S..i+S of the
+ * same row.  The weight is (1/(sqrt(2 pi) S)) * exp((-1/2) * ...) where
+ * (-1/2) is integer division, so every weight is exp(0) = 1 times the
+ * prefactor: a boxcar mean, accumulated in float with double products as
+ * there.  Other cells keep their values.  S = SPAT_FILT_SIZE (already
+ * reduced by the iteration when SPAT_FILT_ITER is set), J1 = SPAT_FILT_1. */
+void orc_spat_filt(int NX, int NY, int S, int J1, const float *in, float *out)
+{
+    const long plane = (long)NX * NY;
+    memcpy(out, in, sizeof(float) * plane);
+    if (S < 1) return;
+    const double PI = 3.141592653589793;
+    for (int i = 1 + S; i <= NX - S; i++)
+        for (int j = J1; j <= NY - S; j++) {
+            float acc = 0.0f, norm = 0.0f;
+            for (int i1 = i - S; i1 <= i + S; i1++) {
+                const double d = sqrt((double)((i - i1) * (i - i1) + (i - i1) * (i - i1)));
+                const double wgt = (1 / (sqrt(2 * PI) * S)) * exp((-1 / 2) * (d * d) / (S * S));
+                acc = (float)((double)acc + (double)in[(long)(j - 1) * NX + (i1 - 1)] * wgt);
+                norm = (float)((double)norm + wgt);
+            }
+            out[(long)(j - 1) * NX + (i - 1)] = acc / norm;
+        }
 }

@@ -31,6 +31,10 @@ void orc_taper_horizontal(int NX, int NY, int GRADT1, int GRADT2, float *g, floa
  * {0, window without its (+hfs,+hfs) corner}, k = window size (identity for
  * k = 1).  Returns the window size used (0: nothing done, out = in). */
 int orc_median_smooth(int NX, int NY, int filtsize, const float *in, float *out);
+/* spat_filt() (src/spat_filt.c:27): the boxcar mean over columns i-S..i+S
+ * its integer (-1/2) makes of the Gaussian, for columns S+1..NX-S and rows
+ * J1..NY-S (S = SPAT_FILT_SIZE, J1 = SPAT_FILT_1); other cells unchanged. */
+void orc_spat_filt(int NX, int NY, int S, int J1, const float *in, float *out);
 
 #ifdef __cplusplus
 }

@@ -676,3 +676,24 @@ void ref_time_filter(float *data, int ntr, int ns, float dt, float fc, int order
         for (int j = 1; j <= ns; j++) data[(size_t)(t - 1) * ns + j - 1] = d[t][j];
     free_matrix(d, 1, ntr, 1, ns);
 }
+
+/* spat_filt (src/spat_filt.c:27) with sws 1 on gradient g [NY][NX]: the
+ * routine reads the gradient from <prefix>_g.old as PCG writes it
+ * (src/PCG.c:77-85, x outer) and writes the filtered one back there */
+void ref_spat_filt(int nx, int ny, int size, int first_row, const char *prefix, float *g)
+{
+    harness_setup_opt(nx, ny);
+    SPAT_FILT_SIZE = size;
+    SPAT_FILT_1 = first_row;
+    SPAT_FILT_ITER = 0;
+    snprintf(JACOBIAN, STRING_SIZE, "%s", prefix);
+    char path[STRING_SIZE + 16];
+    snprintf(path, sizeof path, "%s_g.old", prefix);
+    FILE *f = fopen(path, "wb");
+    for (int i = 1; i <= nx; i++)
+        for (int j = 1; j <= ny; j++) fwrite(&g[(size_t)(j - 1) * nx + i - 1], sizeof(float), 1, f);
+    fclose(f);
+    float **w = harness_grid_in(nx, ny, g);
+    spat_filt(w, 1, 1);
+    harness_grid_out(nx, ny, w, g);
+}

@@ -408,3 +408,14 @@ def time_filter(data, DT, fc, order, method=1, fhp=0.0, which="oracle"):
     f = ref().ref_time_filter if which == "ref" else oracle().orc_time_filter
     f(_ptr(out), ntr, ns, C.c_float(DT), C.c_float(fc), order, method, C.c_float(fhp))
     return out
+
+
+def spat_filt(g, size, first_row, which="oracle", prefix=None):
+    ny, nx = g.shape
+    if which == "ref":
+        out = _f32(g).copy()
+        ref().ref_spat_filt(nx, ny, size, first_row, prefix.encode(), _ptr(out))
+        return out
+    out = np.empty((ny, nx), np.float32)
+    oracle().orc_spat_filt(nx, ny, size, first_row, _ptr(_f32(g)), _ptr(out))
+    return out

@@ -94,3 +94,16 @@ def test_smooth_matches_oracle(gpu_lib, fs):
     np.testing.assert_array_equal(buf.to_array((NY, NX)), want)
     buf.free()
     o.close()
+
+
+@pytest.mark.parametrize("size,first", [(3, 1), (6, 5), (1, 2)])
+def test_spat_filt_matches_oracle(gpu_lib, size, first):
+    NX, NY = 70, 45
+    g = np.random.default_rng(size + first).standard_normal((NY, NX)).astype(np.float32)
+    o = opt.Optimizer()
+    buf = ifos_amd.DeviceBuffer.from_array(g)
+    o.spat_filt(NX, NY, size, first, buf)
+    want = O.spat_filt(g, size, first)
+    np.testing.assert_array_equal(buf.to_array((NY, NX)).view(np.uint32), want.view(np.uint32))
+    buf.free()
+    o.close()

@@ -203,3 +203,25 @@ def test_host_horizontal_taper_matches_oracle(gradt):
     assert not zero.any()
     want = ol.taper_horizontal(g, gradt)
     np.testing.assert_array_equal(_apply(g, coef, zero).view(np.uint32), want.view(np.uint32))
+
+
+@needs_ref
+@pytest.mark.parametrize("size,first", [(3, 1), (5, 4), (1, 10), (0, 1)])
+def test_spat_filt_matches_reference(size, first, tmp_path):
+    g = _grad(25, 40, 31)
+    ref = ol.spat_filt(g, size, first, which="ref", prefix=str(tmp_path / "jac"))
+    orc = ol.spat_filt(g, size, first)
+    np.testing.assert_array_equal(orc.view(np.uint32), ref.view(np.uint32))
+
+
+def test_spat_filt_is_a_boxcar_mean():
+    """The integer (-1/2) in the reference's Gaussian makes every weight 1:
+    inside the filtered window each value is the mean of 2S+1 neighbours."""
+    g = _grad(12, 30, 32)
+    out = ol.spat_filt(g, 2, 3)
+    want = g.copy()
+    for j in range(3, 12 - 2 + 1):
+        for i in range(3, 30 - 2 + 1):
+            want[j - 1, i - 1] = g[j - 1, i - 3:i + 2].astype(np.float64).mean()
+    np.testing.assert_allclose(out, want, rtol=2e-6, atol=1e-7)
+    np.testing.assert_array_equal(out[:2], g[:2])
