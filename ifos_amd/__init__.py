@@ -301,6 +301,14 @@ class Engine:
     def apply_eprecond(self):
         _check(lib().ifos_engine_apply_eprecond(self.h))
 
+    def set_shot_taper_file(self, slot, vp=None, vs=None, rho=None):
+        """Per-shot taper grids (SWS_TAPER_FILE_PER_SHOT) of one slot; no grids turns it off."""
+        L = lib()
+        L.ifos_engine_set_shot_taper_file.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        arrs = [None if a is None else _f32(a) for a in (vp, vs, rho)]
+        ptrs = [None if a is None else a.ctypes.data_as(C.c_void_p) for a in arrs]
+        _check(L.ifos_engine_set_shot_taper_file(self.h, slot, *ptrs))
+
     def set_time_filter(self, f_low_pass, order=4, f_high_pass=0.0):
         """Butterworth filter of wavelets and observed traces (TIME_FILT); 0 turns it off."""
         L = lib()

@@ -168,6 +168,8 @@ struct ifos_engine {
     float *tap_lohi = nullptr;      // [slots][2] normalisation
     std::vector<float> tap_host;    // the table on the host
     std::vector<int> first_src;     // [slots][2] first source of each shot (0, 0: none)
+    float *tfbuf = nullptr;         // [slots][3][plane] per-shot taper grids from files
+    bool tf_on = false;
     // time-domain Butterworth filter of wavelets and observed traces (TIME_FILT)
     int filt_nlow = 0, filt_nhigh = 0;
     double filt_c[2 * kMaxFiltSections][5];
@@ -362,7 +364,7 @@ extern "C" void ifos_engine_destroy(ifos_engine *E)
     dfree(E->slab); dfree(E->mslab); dfree(E->interior); dfree(E->wcbuf); dfree(E->pml);
     dfree(E->psi); dfree(E->ctl); dfree(E->explosive); dfree(E->dargs);
     dfree(E->wbuf); dfree(E->we_sum); dfree(E->ep_geom); dfree(E->we_max); dfree(E->red);
-    dfree(E->tap_m); dfree(E->tap_ij); dfree(E->tap_lohi);
+    dfree(E->tap_m); dfree(E->tap_ij); dfree(E->tap_lohi); dfree(E->tfbuf);
     dfree(E->src_i); dfree(E->src_d); dfree(E->sig); dfree(E->rec_i); dfree(E->seis);
     dfree(E->trace_sums); dfree(E->snap); dfree(E->tiles);
     if (E->stream) hipStreamDestroy(E->stream);
@@ -1114,6 +1116,34 @@ extern "C" int ifos_time_filter(float *data, int ntr, int ns, float DT, float fc
     return 0;
 }
 
+extern "C" int ifos_engine_set_shot_taper_file(ifos_engine *E, int slot, const float *vp, const float *vs,
+                                               const float *rho)
+{
+    HIP_TRY(hipSetDevice(E->device));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    if (!vp && !vs && !rho) {
+        E->tf_on = false;
+        return 0;
+    }
+    if (!vp || !vs || !rho) return ifos_set_error("per-shot taper files need the Vp, Vs and rho grids"), -2;
+    if (slot < 0 || slot >= E->slots) return ifos_set_error("slot %d outside 0..%d", slot, E->slots - 1), -2;
+    const size_t pb = sizeof(float) * E->plane;
+    if (!E->tfbuf) {
+        HIP_TRY(dmalloc(&E->tfbuf, 3 * pb * E->slots));
+        // slots without a file of their own keep their gradient (coefficient 1)
+        std::vector<float> ones((size_t)3 * E->plane * E->slots, 1.0f);
+        HIP_TRY(hipMemcpyAsync(E->tfbuf, ones.data(), 3 * pb * E->slots, hipMemcpyHostToDevice, E->stream));
+        HIP_TRY(hipStreamSynchronize(E->stream));
+    }
+    float *d = E->tfbuf + (size_t)slot * 3 * E->plane;
+    HIP_TRY(hipMemcpyAsync(d, vp, pb, hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipMemcpyAsync(d + E->plane, vs, pb, hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipMemcpyAsync(d + 2 * E->plane, rho, pb, hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    E->tf_on = true;
+    return 0;
+}
+
 extern "C" int ifos_engine_set_time_filter(ifos_engine *E, float F_LOW_PASS, int ORDER, float F_HIGH_PASS)
 {
     HIP_TRY(hipSetDevice(E->device));
@@ -1184,6 +1214,10 @@ extern "C" int ifos_engine_adjoint(ifos_engine *E)
     memset(&ga, 0, sizeof ga);
     if (E->ep_mode) ENG_TRY(energy_weights(E, ga));
     if (E->tap_on) ENG_TRY(shot_taper_args(E, ga));
+    if (E->tf_on) {
+        ga.tf = E->tfbuf;
+        ga.tfs = 3 * E->plane;
+    }
     ga.n = (int)E->plane; ga.DT = E->c.DT;
     ga.vp = E->vp_i; ga.vs = E->vs_i; ga.rho = E->rho_i;
     ga.wc = E->wcbuf; ga.wc_u = E->wcbuf + E->plane; ga.wc_rho = E->wcbuf + 2 * E->plane;

@@ -192,6 +192,10 @@ struct GradArgs {
     const int *tap_ij;
     const float *tap_lohi;
     int tap_n, tap_c, tap_fs, NX, NY;
+    // per-shot taper grids read from files (taper_grad_shot sws 2-4): slot s
+    // holds Vp, Vs and rho coefficients [3][n] at tf + s * tfs (null: none)
+    const float *tf;
+    long tfs;
 };
 
 // adjoint sources from seismograms (src/calc_res.c, LNORM 2)

@@ -591,6 +591,13 @@ IFOS_KERNEL void k_gradient(GradArgs a)
             srho *= t;
             if (in_box(ci, cj, is, js, a.tap_fs, a.NX, a.NY)) svp = svs = srho = 0.0f;
         }
+        // taper grids of this shot read from files (src/IFOS2D.c:2691-2698)
+        if (a.tf) {
+            const float *t = a.tf + (long)s * a.tfs;
+            svp *= t[e];
+            svs *= t[a.n + e];
+            srho *= t[2L * a.n + e];
+        }
         gvp = s ? gvp + svp : svp;
         gvs = s ? gvs + svs : svs;
         grho = s ? grho + srho : srho;

@@ -265,18 +265,21 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     float vy = vel_new(vy0, g.DT, rjp, sxy_x + syy_y, g.DH);
 
     // point forces (forward) / residual injection (adjoint) of this tile, in order
+    // the point index is the same for the whole wave: the list, position and
+    // signal reads are scalar loads, the cell match is per lane
     for (int q = q0; q < q1; q++) {
-        const int l = gmem(a.inj.idx)[q];
-        if (gmem(a.inj.ix)[l] != i || gmem(a.inj.iy)[l] != j) continue;
-        const float s0 = gmem(a.inj.sig)[(long)l * a.NT + nt - 1];
-        const float s1 = gmem(a.inj.sig1)[(long)l * a.NT + nt - 1];
+        const int l = cmem(a.inj.idx)[q];
+        const int px = cmem(a.inj.ix)[l], py = cmem(a.inj.iy)[l];
+        const float s0 = cmem(a.inj.sig)[(long)l * a.NT + nt - 1];
+        const float s1 = cmem(a.inj.sig1)[(long)l * a.NT + nt - 1];
+        if (px != i || py != j) continue;
         if (a.sw == 0) {
-            const int type = gmem(a.inj.type)[l];
+            const int type = cmem(a.inj.type)[l];
             if (type == 2) vx += (g.DT * rip * s1) / (g.DH * g.DH);
             if (type == 3) vy += (g.DT * rjp * s1) / (g.DH * g.DH);
             if (type == 4) {
-                vx += (g.DT * rip * gmem(a.inj.sin_azi)[l] * s0) / (g.DH * g.DH);
-                vy += (g.DT * rjp * gmem(a.inj.cos_azi)[l] * s0) / (g.DH * g.DH);
+                vx += (g.DT * rip * cmem(a.inj.sin_azi)[l] * s0) / (g.DH * g.DH);
+                vy += (g.DT * rjp * cmem(a.inj.cos_azi)[l] * s0) / (g.DH * g.DH);
             }
         } else {
             if (a.ADJOINT_TYPE == 1) { vx += s0; vy += s1; }
@@ -383,8 +386,8 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
         // and before the free surface (src/IFOS2D.c:1758-1773)
         if (expl) {
             for (int q = ex0; q < ex1; q++) {
-                const int l = gmem(a.exp.idx)[q];
-                if (gmem(a.exp.ix)[l] != i || gmem(a.exp.iy)[l] != j) continue;
+                const int l = cmem(a.exp.idx)[q];
+                if (cmem(a.exp.ix)[l] != i || cmem(a.exp.iy)[l] != j) continue;
                 const float amp = explosive_amp(a.exp.sig + (long)l * a.NT - 1, nt, a.NT, g.DH);
                 sxx = add_half(sxx, amp);
                 syy = add_half(syy, amp);
@@ -448,8 +451,8 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
     }
     if (flags & kForward) {
         for (int q = rc0; q < rc1; q++) {
-            const int t = gmem(a.rec.idx)[q];
-            if (gmem(a.rec.ix)[t] != i || gmem(a.rec.iy)[t] != j) continue;
+            const int t = cmem(a.rec.idx)[q];
+            if (cmem(a.rec.ix)[t] != i || cmem(a.rec.iy)[t] != j) continue;
             gmem(a.seis_vx)[(long)t * a.NT + nt - 1] = vxs[c];
             gmem(a.seis_vy)[(long)t * a.NT + nt - 1] = vys[c];
         }

@@ -175,6 +175,15 @@ int  ifos_engine_energy_weight(ifos_engine *e, int which, float *out);
  * three gradients.  The taper centre is the shot's first source (the
  * reference's srcpos[.][ishot]).  on = 0 turns it off. */
 int  ifos_engine_set_shot_taper(ifos_engine *e, int on, float SRTRADIUS, int SRTSHAPE, int FILTSIZE);
+/* Per-shot taper grids read from files (SWS_TAPER_FILE_PER_SHOT,
+ * taper_grad_shot sws 2-4, src/taper_grad_shot.c:199-288: the files
+ * TAPER_FILE_NAME.shot<n>.vp/.vs/.rho as ifos_model_read returns them): the
+ * shot in `slot` has its Vp, Vs and rho gradients multiplied by these
+ * [NY][NX] grids, after the circular taper and before the shot sum, as
+ * src/IFOS2D.c:2691-2698 orders them.  Slots never given grids keep their
+ * gradients; set the grids again when the slot's shot changes.  All three
+ * NULL turns it off. */
+int  ifos_engine_set_shot_taper_file(ifos_engine *e, int slot, const float *vp, const float *vs, const float *rho);
 /* Time-domain Butterworth filter of the multi-scale inversion (TIME_FILT,
  * timedomain_filt src/timedomain_filt.c:27 with the vendored libcseife):
  * lowpass of order ORDER at F_LOW_PASS Hz, then, when F_HIGH_PASS != 0, a

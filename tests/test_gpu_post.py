@@ -107,3 +107,44 @@ def test_spat_filt_matches_oracle(gpu_lib, size, first):
     np.testing.assert_array_equal(buf.to_array((NY, NX)).view(np.uint32), want.view(np.uint32))
     buf.free()
     o.close()
+
+
+def test_shot_taper_files(gpu_lib):
+    """Per-shot taper grids multiply each shot's gradients after the
+    circular taper and before the sum (src/IFOS2D.c:2681-2698)."""
+    w = workloads.small_case(seed=47, NX=80, NY=44, NT=140, FDORDER=4, L=0, VELOCITY=1, DTINV=2)
+    base = np.array(w.shots[0].srcpos, np.float32).reshape(-1, 8)
+    shots = []
+    for x in (25, 55):
+        sp = base.copy()
+        sp[:, 0] = x
+        shots.append((sp, w.shots[0].recpos))
+    rng = np.random.default_rng(3)
+    tapers = [[rng.random((44, 80)).astype(np.float32) for _ in range(3)] for _ in shots]
+    eng1 = ifos_amd.Engine(w.cfg, slots=1)
+    eng1.set_model(**w.true)
+    obs = []
+    for sp, rp in shots:
+        eng1.set_shot(sp, rp)
+        eng1.forward()
+        obs.append(eng1.seismograms())
+    eng1.set_model(**w.start)
+    eng1.set_shot_taper(True, 10.0, 1, 1)
+    want = None
+    for (sp, rp), (ovx, ovy), t in zip(shots, obs, tapers):
+        eng1.set_shot(sp, rp)
+        eng1.shot_gradient(ovx, ovy)
+        g = eng1.gradient() * np.stack(t)
+        want = g if want is None else want + g
+    eng1.close()
+    eng = ifos_amd.Engine(w.cfg, slots=2)
+    eng.set_model(**w.start)
+    eng.set_shot_taper(True, 10.0, 1, 1)
+    for k, t in enumerate(tapers):
+        eng.set_shot_taper_file(k, *t)
+    eng.set_shots(shots)
+    eng.shot_gradient(np.concatenate([o[0] for o in obs]), np.concatenate([o[1] for o in obs]))
+    got = eng.gradient()
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    eng.set_shot_taper_file(0)
+    eng.close()
