@@ -91,13 +91,22 @@ def serving(args):
     rng = np.random.default_rng(args.seed)
     n = max(1, int(round(args.rate * args.duration)))
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n))
-    reqs = []
-    for k in range(n):
+
+    def survey(k):
         nx = int(rng.integers(300, args.max_nx + 1)) // 8 * 8
         ny = int(nx * rng.uniform(0.4, 0.5)) // 8 * 8
-        w = workloads.small_case(seed=1000 + k, NX=nx, NY=ny, NT=int(rng.integers(args.min_nt, args.max_nt + 1)) // 4 * 4,
-                                 FDORDER=int(rng.choice([4, 8])), L=int(rng.choice([0, 2])), VELOCITY=1, DTINV=4,
-                                 FW=10)
+        return workloads.small_case(seed=1000 + k, NX=nx, NY=ny,
+                                    NT=int(rng.integers(args.min_nt, args.max_nt + 1)) // 4 * 4,
+                                    FDORDER=int(rng.choice([4, 8])), L=int(rng.choice([0, 2])), VELOCITY=1, DTINV=4,
+                                    FW=10)
+
+    # --surveys K: requests come from K surveys (grid, NT, order, physics and
+    # model fixed per survey), so engines and models can stay resident
+    # between requests; 0: every request is a survey of its own
+    surveys = [survey(k) for k in range(args.surveys)]
+    reqs = []
+    for k in range(n):
+        w = surveys[int(rng.integers(0, len(surveys)))] if surveys else survey(k)
         nshots = int(rng.integers(1, args.max_shots + 1))
         shots = []
         for _ in range(nshots):
@@ -133,6 +142,7 @@ def serving(args):
            "value": round(gshots / wall, 4), "unit": "shot-gradients/s", "n_gpus": 1, "higher_is_better": True,
            "data": "synthetic (seeded layered/anomaly models, zero observed traces)",
            "config": {"workload": "serving", "requests": n, "rate_per_s": args.rate, "duration_s": args.duration,
+                      "surveys": args.surveys or n,
                       "grid_nx": [300, args.max_nx], "nt": [args.min_nt, args.max_nt], "shots": [1, args.max_shots],
                       "workers": args.workers, "max_slots": args.max_slots, "budget_gb": args.budget_gb,
                       "snapshots": args.snap},
@@ -174,6 +184,8 @@ def main():
     ap.add_argument("--max-shots", type=int, default=8)
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--max-slots", type=int, default=4)
+    ap.add_argument("--surveys", type=int, default=8,
+                    help="serving: distinct survey configurations the requests draw from (0: one per request)")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--seed", type=int, default=7)
     args = ap.parse_args()
