@@ -52,6 +52,8 @@ def build_workload(name):
         return workloads.toy_elastic()
     if name == "batched_aperture":
         return workloads.batched_aperture()
+    if name == "precision_large":
+        return workloads.precision_large()
     raise SystemExit(f"unknown workload {name}")
 
 
