@@ -29,6 +29,16 @@ __device__ __forceinline__ void stress_derivs(const float *hc, const float *sxx_
     }
 }
 
+// x / K[h] of the CPML update.  With every profile value K = 1 (k_max_PML = 1,
+// the usual case) the division is skipped: x / 1 is exactly x, so the
+// result is the reference's bit for bit and the correctly rounded division
+// sequence is saved in every absorbing cell.
+__device__ __forceinline__ float divK(float x, const float *K, int h, int K_one)
+{
+    if (K_one) return x;
+    return x / cmem(K)[h];
+}
+
 // velocity-side CPML (src/update_v_PML.c:383-434)
 __device__ __forceinline__ void cpml_velocity(const StepGeom &g, int j, int i, float *psi_sxx_x, float *psi_sxy_x,
                                               float *psi_syy_y, float *psi_sxy_y, float &sxx_x, float &sxy_x,
@@ -40,36 +50,36 @@ __device__ __forceinline__ void cpml_velocity(const StepGeom &g, int j, int i, f
         auto p1 = gmem(psi_sxx_x) + (long)(j - 1) * g.px + h;
         auto p2 = gmem(psi_sxy_x) + (long)(j - 1) * g.px + h;
         *p1 = cmem(pm.b_xh)[h] * *p1 + cmem(pm.a_xh)[h] * sxx_x;
-        sxx_x = sxx_x / cmem(pm.K_xh)[h] + *p1;
+        sxx_x = divK(sxx_x, pm.K_xh, h, g.K_one) + *p1;
         *p2 = cmem(pm.b_x)[h] * *p2 + cmem(pm.a_x)[h] * sxy_x;
-        sxy_x = sxy_x / cmem(pm.K_x)[h] + *p2;
+        sxy_x = divK(sxy_x, pm.K_x, h, g.K_one) + *p2;
     }
     if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
         const int h = i - g.NX + 2 * g.FW - 1;
         auto p1 = gmem(psi_sxx_x) + (long)(j - 1) * g.px + h;
         auto p2 = gmem(psi_sxy_x) + (long)(j - 1) * g.px + h;
         *p1 = cmem(pm.b_xh)[h] * *p1 + cmem(pm.a_xh)[h] * sxx_x;
-        sxx_x = sxx_x / cmem(pm.K_xh)[h] + *p1;
+        sxx_x = divK(sxx_x, pm.K_xh, h, g.K_one) + *p1;
         *p2 = cmem(pm.b_x)[h] * *p2 + cmem(pm.a_x)[h] * sxy_x;
-        sxy_x = sxy_x / cmem(pm.K_x)[h] + *p2;
+        sxy_x = divK(sxy_x, pm.K_x, h, g.K_one) + *p2;
     }
     if (!g.FREE_SURF && j <= g.FW) {
         const int h = j - 1;
         auto p1 = gmem(psi_syy_y) + (long)h * g.py + (i - 1);
         auto p2 = gmem(psi_sxy_y) + (long)h * g.py + (i - 1);
         *p1 = cmem(pm.b_yh)[h] * *p1 + cmem(pm.a_yh)[h] * syy_y;
-        syy_y = syy_y / cmem(pm.K_yh)[h] + *p1;
+        syy_y = divK(syy_y, pm.K_yh, h, g.K_one) + *p1;
         *p2 = cmem(pm.b_y)[h] * *p2 + cmem(pm.a_y)[h] * sxy_y;
-        sxy_y = sxy_y / cmem(pm.K_y)[h] + *p2;
+        sxy_y = divK(sxy_y, pm.K_y, h, g.K_one) + *p2;
     }
     if (j >= g.NY - g.FW + 1) {
         const int h = j - g.NY + 2 * g.FW - 1;
         auto p1 = gmem(psi_syy_y) + (long)h * g.py + (i - 1);
         auto p2 = gmem(psi_sxy_y) + (long)h * g.py + (i - 1);
         *p1 = cmem(pm.b_yh)[h] * *p1 + cmem(pm.a_yh)[h] * syy_y;
-        syy_y = syy_y / cmem(pm.K_yh)[h] + *p1;
+        syy_y = divK(syy_y, pm.K_yh, h, g.K_one) + *p1;
         *p2 = cmem(pm.b_y)[h] * *p2 + cmem(pm.a_y)[h] * sxy_y;
-        sxy_y = sxy_y / cmem(pm.K_y)[h] + *p2;
+        sxy_y = divK(sxy_y, pm.K_y, h, g.K_one) + *p2;
     }
 }
 
@@ -108,18 +118,18 @@ __device__ __forceinline__ void cpml_stress(const StepGeom &g, int j, int i, flo
         auto p1 = gmem(psi_vxx) + (long)(j - 1) * g.px + h;
         auto p2 = gmem(psi_vyx) + (long)(j - 1) * g.px + h;
         *p1 = cmem(pm.b_x)[h] * *p1 + cmem(pm.a_x)[h] * vxx;
-        vxx = vxx / cmem(pm.K_x)[h] + *p1;
+        vxx = divK(vxx, pm.K_x, h, g.K_one) + *p1;
         *p2 = cmem(pm.b_xh)[h] * *p2 + cmem(pm.a_xh)[h] * vyx;
-        vyx = vyx / cmem(pm.K_xh)[h] + *p2;
+        vyx = divK(vyx, pm.K_xh, h, g.K_one) + *p2;
     }
     if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
         const int h = i - g.NX + 2 * g.FW - 1;
         auto p1 = gmem(psi_vxx) + (long)(j - 1) * g.px + h;
         auto p2 = gmem(psi_vyx) + (long)(j - 1) * g.px + h;
         *p1 = cmem(pm.b_x)[h] * *p1 + cmem(pm.a_x)[h] * vxx;
-        vxx = vxx / cmem(pm.K_x)[h] + *p1;
+        vxx = divK(vxx, pm.K_x, h, g.K_one) + *p1;
         *p2 = cmem(pm.b_xh)[h] * *p2 + cmem(pm.a_xh)[h] * vyx;
-        vyx = vyx / cmem(pm.K_xh)[h] + *p2;
+        vyx = divK(vyx, pm.K_xh, h, g.K_one) + *p2;
     }
     if (!g.FREE_SURF && j <= g.FW) {
         const int h = j - 1;
@@ -127,17 +137,17 @@ __device__ __forceinline__ void cpml_stress(const StepGeom &g, int j, int i, flo
         auto p2 = gmem(psi_vxy) + (long)h * g.py + (i - 1);
         *p1 = cmem(pm.b_y)[h] * *p1 + cmem(pm.a_y)[h] * vyy;
         *p2 = cmem(pm.b_yh)[h] * *p2 + cmem(pm.a_yh)[h] * vxy;
-        vyy = vyy / cmem(pm.K_y)[h] + *p1;
-        vxy = vxy / cmem(pm.K_yh)[h] + *p2;
+        vyy = divK(vyy, pm.K_y, h, g.K_one) + *p1;
+        vxy = divK(vxy, pm.K_yh, h, g.K_one) + *p2;
     }
     if (j >= g.NY - g.FW + 1) {
         const int h = j - g.NY + 2 * g.FW - 1;
         auto p1 = gmem(psi_vyy) + (long)h * g.py + (i - 1);
         auto p2 = gmem(psi_vxy) + (long)h * g.py + (i - 1);
         *p1 = cmem(pm.b_y)[h] * *p1 + cmem(pm.a_y)[h] * vyy;
-        vyy = vyy / cmem(pm.K_y)[h] + *p1;
+        vyy = divK(vyy, pm.K_y, h, g.K_one) + *p1;
         *p2 = cmem(pm.b_yh)[h] * *p2 + cmem(pm.a_yh)[h] * vxy;
-        vxy = vxy / cmem(pm.K_yh)[h] + *p2;
+        vxy = divK(vxy, pm.K_yh, h, g.K_one) + *p2;
     }
 }
 
@@ -277,13 +287,13 @@ __device__ __forceinline__ void surface_derivs(const StepGeom &g, int i, const f
         const int h = i - 1;
         auto ps = gmem(psi_vxxs) + h;
         *ps = cmem(pm.b_x)[h] * *ps + cmem(pm.a_x)[h] * vxx;
-        vxx = vxx / cmem(pm.K_x)[h] + *ps;
+        vxx = divK(vxx, pm.K_x, h, g.K_one) + *ps;
     }
     if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
         const int h = i - g.NX + 2 * g.FW - 1;
         auto ps = gmem(psi_vxxs) + h;
         *ps = cmem(pm.b_x)[h] * *ps + cmem(pm.a_x)[h] * vxx;
-        vxx = vxx / cmem(pm.K_x)[h] + *ps;
+        vxx = divK(vxx, pm.K_x, h, g.K_one) + *ps;
     }
 }
 

@@ -181,6 +181,7 @@ StepGeom make_geom(int nx1, int nx2, int ny1, int ny2, int pitch, const float *h
              K_yh ? vec1_dev(K_yh, n) : z, a_yh ? vec1_dev(a_yh, n) : z, b_yh ? vec1_dev(b_yh, n) : z};
     g.px = n > 0 ? n : 1;
     g.py = nx2;
+    g.K_one = 0;  // the drop-in layer always divides
     return g;
 }
 

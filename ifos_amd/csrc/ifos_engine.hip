@@ -170,6 +170,7 @@ struct ifos_engine {
     std::vector<int> first_src;     // [slots][2] first source of each shot (0, 0: none)
     float *tfbuf = nullptr;         // [slots][3][plane] per-shot taper grids from files
     bool tf_on = false;
+    int K_one = 0;                  // every CPML K value is 1
     // time-domain Butterworth filter of wavelets and observed traces (TIME_FILT)
     int filt_nlow = 0, filt_nhigh = 0;
     double filt_c[2 * kMaxFiltSections][5];
@@ -312,6 +313,11 @@ static int engine_create(const ifos_config *cfg, int device, int slots, ifos_eng
     const int fw2 = c.FW > 0 ? 2 * c.FW : 1;
     std::vector<float> prof((size_t)12 * fw2, 0.0f);
     if (c.FW > 0) ifos_pml_profiles(&c, prof.data());
+    // K_x, K_xh, K_y, K_yh are profiles 0, 3, 6 and 9
+    E->K_one = 1;
+    for (int q : {0, 3, 6, 9})
+        for (int h = 0; h < fw2; h++)
+            if (prof[(size_t)q * fw2 + h] != 1.0f) E->K_one = 0;
     if (dmalloc(&E->pml, sizeof(float) * prof.size()) != hipSuccess) return fail(-3);
     hipMemcpyAsync(E->pml, prof.data(), sizeof(float) * prof.size(), hipMemcpyHostToDevice, E->stream);
     const size_t xs = (size_t)c.NY * fw2, ys = (size_t)fw2 * c.NX;
@@ -607,6 +613,7 @@ static StepGeom geom(const ifos_engine *E)
              P + 6 * fw2, P + 7 * fw2, P + 8 * fw2, P + 9 * fw2, P + 10 * fw2, P + 11 * fw2};
     g.px = fw2;
     g.py = c.NX;
+    g.K_one = E->K_one;
     return g;
 }
 

@@ -67,6 +67,7 @@ struct StepGeom {
     float hc[7];
     Cpml pml;
     int px, py;       // CPML strip pitches
+    int K_one;        // every CPML K value is 1 (the divisions by K are identities)
 };
 
 // update_v_PML (src/update_v_PML.c:29-596)

@@ -124,6 +124,38 @@ def test_forward_matches_oracle(gpu_lib, FDORDER, L, VEL, FS, stype):
     eng.close()
 
 
+@pytest.mark.parametrize("kmax,npower", [(4.0, 2.0), (2.5, 3.0)])
+def test_stretched_pml_matches_oracle(gpu_lib, kmax, npower):
+    """CPML with a coordinate stretch K > 1 (k_max_PML, src/PML_pro.c:
+    73-119): the memory-variable updates divide by K, a division the
+    kernels skip when every K is one."""
+    w = workloads.small_case(seed=17, NX=70, NY=50, NT=160, FDORDER=6, L=1, VELOCITY=1, DTINV=2, FW=12)
+    print(f"[oracle stretched pml] kmax={kmax} npower={npower}", flush=True)
+    cfg = w.cfg
+    cfg.k_max_PML = kmax
+    cfg.npower = npower
+    p = O.make_params(FL=[cfg.FL[i] for i in range(1, 8)],
+                      **{k: getattr(cfg, k) for k, _ in cfg._fields_ if k != "FL"})
+    shot = w.shots[0]
+    ovx, ovy, ofin = O.forward(p, w.true, shot.srcpos, shot.recpos)
+    ref = O.gradient(p, w.start, shot.srcpos, shot.recpos, ovx, ovy)
+    eng = ifos_amd.Engine(cfg)
+    eng.set_model(**w.true)
+    eng.set_shot(shot.srcpos, shot.recpos)
+    eng.forward()
+    vx, vy = eng.seismograms()
+    assert_bitexact(vx, ovx, "vx vs oracle (stretched PML)")
+    assert_bitexact(vy, ovy, "vy vs oracle (stretched PML)")
+    assert_bitexact(eng.fields(), ofin, "fields vs oracle (stretched PML)")
+    eng.set_model(**w.start)
+    eng.shot_gradient(ovx, ovy)
+    g = eng.gradient()
+    assert_bitexact(g[0], ref["grad_vp"], "Vp gradient (stretched PML)")
+    assert_bitexact(g[1], ref["grad_vs"], "Vs gradient (stretched PML)")
+    assert_bitexact(g[2], ref["grad_rho"], "rho gradient (stretched PML)")
+    eng.close()
+
+
 def test_gradient_matches_oracle_with_kill(gpu_lib):
     w = workloads.small_case(seed=5, NX=80, NY=50, NT=180, FDORDER=8, L=2, VELOCITY=1, DTINV=3)
     print("[oracle gradient] kill mask", flush=True)

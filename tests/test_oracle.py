@@ -153,6 +153,30 @@ def test_forward_matches_reference(fd, L, vel, fs, st):
 
 
 @needs_ref
+@pytest.mark.parametrize("kmax,npower", [(4.0, 2.0), (2.5, 3.0)])
+def test_stretched_pml_matches_reference(kmax, npower):
+    """A CPML coordinate stretch K > 1 (k_max_PML): forward and gradient."""
+    w = workloads.small_case(seed=17, NX=50, NY=38, NT=100, FDORDER=6, L=1, VELOCITY=1, DTINV=2, FW=10)
+    p = params_of(w)
+    p.k_max_PML = kmax
+    p.npower = npower
+    s = w.shots[0]
+    a = O.forward(p, w.true, s.srcpos, s.recpos)
+    b = O.forward(p, w.true, s.srcpos, s.recpos, which="ref")
+    for x, y, n in zip(a, b, ("vx", "vy", "fields")):
+        np.testing.assert_array_equal(x, y, err_msg=n)
+    ovx, ovy = a[0].copy(), a[1].copy()
+    ovx[:, 0] = 0
+    ovy[:, 0] = 0
+    g = O.gradient(p, w.start, s.srcpos, s.recpos, ovx, ovy)
+    dvx, _ = O.calc_res(p, g["seis_vx"], ovx)
+    dvy, _ = O.calc_res(p, g["seis_vy"], ovy)
+    r = O.ref_gradient(p, w.start, s.srcpos, s.recpos, dvx, dvy)
+    for k in ("seis_vx", "seis_vy", "grad_vp", "grad_vs", "grad_rho"):
+        np.testing.assert_array_equal(g[k], r[k], err_msg=k)
+
+
+@needs_ref
 @pytest.mark.parametrize("fd,L,vel", list(itertools.product([4, 8], [0, 2], [0, 1])))
 def test_gradient_matches_reference(fd, L, vel):
     w = workloads.small_case(seed=3 + fd + L, NX=48, NY=36, NT=120, FDORDER=fd, L=L, VELOCITY=vel, DTINV=3, FW=8)
