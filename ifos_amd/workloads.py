@@ -79,7 +79,7 @@ def toy_elastic(seed=0, NT=2000, NX=400, NY=200):
     vmax = float(true["vp"].max())
     DT = stable_dt(1.0, vmax, 4, 1)
     cfg = make_config(NX=NX, NY=NY, DH=1.0, DT=float(DT), NT=NT, FDORDER=4, MAXRELERROR=1, FW=FW,
-                      FREE_SURF=1, VPPML=vp0, FPML=30.0, DTINV=2, VELOCITY=0, L=0)
+                      FREE_SURF=1, VPPML=vp0, FPML=30.0, DTINV=2, VELOCITY=0, L=0, npower=4.0, k_max_PML=8.0)
     rec = np.array([[21 + 2 * k, 5] for k in range(min(180, (NX - 22) // 2))], np.int32)
     shots = [Shot(srcpos=source_row(min(100, NX // 4), min(10, NY // 4), 30.0)[None, :], recpos=rec)]
     return Workload("toy_elastic", cfg, true, start, shots,
@@ -127,9 +127,11 @@ def layered_visc(seed=1, NX=2000, NY=800, DH=2.0, NT=4500, DTINV=3, nshots=64, n
     vmax = float(vp.max()) * math.sqrt(1.0 + L * float(tau))
     DT = stable_dt(DH, vmax, FDORDER, 1)
     FL = [10.0, 40.0, 80.0][:L]
+    # CPML exponent and coordinate stretch as in the reference's own
+    # parameter file (par/in_and_out/IFOS2D_FW.json: npower 4, k_max_PML 8)
     cfg = make_config(NX=NX, NY=NY, DH=DH, DT=float(DT), NT=NT, FDORDER=FDORDER, MAXRELERROR=1, FW=FW,
                       FREE_SURF=1, VPPML=float(vp.max()), FPML=fc, DTINV=DTINV, VELOCITY=1, L=L, FL=FL,
-                      F_REF=fc)
+                      F_REF=fc, npower=4.0, k_max_PML=8.0)
     rx = np.array([FW + 1 + rec_step * k for k in range(nrec) if FW + 1 + rec_step * k <= NX - FW], np.int32)
     rec = np.stack([rx, np.full_like(rx, 2)], 1).astype(np.int32)
     sx = np.linspace(FW + 5, NX - FW - 5, nshots).round().astype(int)

@@ -410,11 +410,21 @@ def time_filter(data, DT, fc, order, method=1, fhp=0.0, which="oracle"):
     return out
 
 
-def spat_filt(g, size, first_row, which="oracle", prefix=None):
+def spat_filt(g, size, first_row, which="oracle"):
     ny, nx = g.shape
     if which == "ref":
-        out = _f32(g).copy()
-        ref().ref_spat_filt(nx, ny, size, first_row, prefix.encode(), _ptr(out))
+        # the reference builds "<JACOBIAN>_g.old" in a 74-byte buffer
+        # (STRING_SIZE, src/fd.h:22; src/spat_filt.c:45-62): keep the path short
+        import shutil
+        import tempfile
+        d = tempfile.mkdtemp(prefix="sf", dir="/tmp")
+        try:
+            prefix = os.path.join(d, "jac")
+            assert len(prefix) + 10 < 74, prefix
+            out = _f32(g).copy()
+            ref().ref_spat_filt(nx, ny, size, first_row, prefix.encode(), _ptr(out))
+        finally:
+            shutil.rmtree(d)
         return out
     out = np.empty((ny, nx), np.float32)
     oracle().orc_spat_filt(nx, ny, size, first_row, _ptr(_f32(g)), _ptr(out))
