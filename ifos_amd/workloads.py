@@ -79,7 +79,7 @@ def toy_elastic(seed=0, NT=2000, NX=400, NY=200):
     vmax = float(true["vp"].max())
     DT = stable_dt(1.0, vmax, 4, 1)
     cfg = make_config(NX=NX, NY=NY, DH=1.0, DT=float(DT), NT=NT, FDORDER=4, MAXRELERROR=1, FW=FW,
-                      FREE_SURF=1, VPPML=vp0, FPML=30.0, DTINV=2, VELOCITY=0, L=0, npower=4.0, k_max_PML=8.0)
+                      FREE_SURF=1, VPPML=vp0, FPML=30.0, DTINV=2, VELOCITY=0, L=0, npower=2.0, k_max_PML=1.0)
     rec = np.array([[21 + 2 * k, 5] for k in range(min(180, (NX - 22) // 2))], np.int32)
     shots = [Shot(srcpos=source_row(min(100, NX // 4), min(10, NY // 4), 30.0)[None, :], recpos=rec)]
     return Workload("toy_elastic", cfg, true, start, shots,
