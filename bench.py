@@ -171,6 +171,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape (no timing sweep)")
     ap.add_argument("--order", default="auto", choices=["auto", "tile", "slot"],
                     help="work order of several-shot launches (auto: the autotuner's pick)")
+    ap.add_argument("--vel-rows", type=int, default=0, choices=[0, 1, 2],
+                    help="velocity step cells per thread along y (0: the autotuner's pick, or 1 without it)")
     ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"],
                     help="snapshot storage (fp32 keeps the gradient bit-identical to the reference)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -226,7 +228,10 @@ def main():
                 f"{'slot' if eng.get_work_order() else 'tile'}-major order, {ms:.4f} ms per step")
         if args.order != "auto":
             eng.set_work_order(args.order == "slot")
-        log(f"[bench] launch {eng.get_launch()}, {'slot' if eng.get_work_order() else 'tile'}-major")
+        if args.vel_rows:
+            eng.set_velocity_rows(args.vel_rows)
+        log(f"[bench] launch {eng.get_launch()}, {'slot' if eng.get_work_order() else 'tile'}-major, "
+            f"velocity rows per thread {eng.get_velocity_rows()}")
         for kern in (0, 1):
             ms = eng.time_kernel(kern, reps=20)
             gbs = eng.kernel_bytes_per_cell(kern) * cfg.NX * cfg.NY * S / (ms * 1e-3) / 1e9
@@ -257,7 +262,10 @@ def main():
         gv, gs = eng.get_launch_grids()
         log(f"[bench] autotune: {shape[0]} shots per workgroup, grids {gv or 'one per item'} (velocity) / "
             f"{gs or 'one per item'} (stress), {'slot' if shape[3] else 'tile'}-major order, "
-            f"{shape[2]:.4f} ms per step")
+            f"velocity rows per thread {eng.get_velocity_rows()}, {shape[2]:.4f} ms per step")
+    if args.vel_rows:
+        eng.set_velocity_rows(args.vel_rows)
+    vel_rows = eng.get_velocity_rows()
 
     def step(k):
         batch = batches[k]
@@ -316,6 +324,7 @@ def main():
         "data": "synthetic (seeded layered model with faults; observed traces from the true model)",
         "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "dh": cfg.DH, "FDORDER": cfg.FDORDER, "L": cfg.L,
                    "NT": cfg.NT, "DTINV": cfg.DTINV, "snapshots": args.snap, "shots_per_step": S, "receivers": int(len(w.shots[0].recpos)),
+                   "velocity_rows_per_thread": vel_rows,
                    "parallelism": "one GPU, one shot at a time" if S == 1 else
                    f"one GPU, {S} shots per launch" + (f" ({shape[0]} per workgroup, {'slot' if shape[3] else 'tile'}-major)"
                                                        if shape else "")},

@@ -227,6 +227,14 @@ class Engine:
         _check(lib().ifos_engine_get_launch_grids(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def set_velocity_rows(self, rows):
+        _check(lib().ifos_engine_set_velocity_rows(self.h, int(rows)))
+
+    def get_velocity_rows(self):
+        v = C.c_int()
+        _check(lib().ifos_engine_get_velocity_rows(self.h, C.byref(v)))
+        return v.value
+
     def set_work_order(self, slot_major):
         _check(lib().ifos_engine_set_work_order(self.h, int(bool(slot_major))))
 

@@ -29,6 +29,39 @@ __device__ __forceinline__ void stress_derivs(const float *hc, const float *sxx_
     }
 }
 
+// stress_derivs for two vertically adjacent cells, rows j (r = 0, at the
+// given pointers) and j+1 (r = 1): the y derivatives of both rows read one
+// column of 2N+1 values of sxy (rows j-N .. j+N) and of syy (rows j-N+1 ..
+// j+N+1) instead of 4N; each row's sums keep stress_derivs' order
+template <int N>
+__device__ __forceinline__ void stress_derivs2(const float *hc, const float *sxx_, const float *syy_,
+                                               const float *sxy_, int P, float (&sxx_x)[2], float (&sxy_x)[2],
+                                               float (&sxy_y)[2], float (&syy_y)[2])
+{
+    const auto sxx = gmem(sxx_), syy = gmem(syy_), sxy = gmem(sxy_);
+    float cy[2 * N + 1], cs[2 * N + 1];  // cy[k]: sxy row k-N; cs[k]: syy row k-N+1
+#pragma unroll
+    for (int k = 0; k <= 2 * N; k++) {
+        cy[k] = sxy[(k - N) * P];
+        cs[k] = syy[(k - N + 1) * P];
+    }
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const auto sx = sxx + r * P, sy = sxy + r * P;
+        sxx_x[r] = hc[1] * (sx[1] - sx[0]);
+        sxy_x[r] = hc[1] * (cy[N + r] - sy[-1]);
+        sxy_y[r] = hc[1] * (cy[N + r] - cy[N + r - 1]);
+        syy_y[r] = hc[1] * (cs[N + r] - cs[N + r - 1]);
+#pragma unroll
+        for (int m = 2; m <= N; m++) {
+            sxx_x[r] = sxx_x[r] + hc[m] * (sx[m] - sx[1 - m]);
+            sxy_x[r] = sxy_x[r] + hc[m] * (sy[m - 1] - sy[-m]);
+            sxy_y[r] = sxy_y[r] + hc[m] * (cy[N + r + m - 1] - cy[N + r - m]);
+            syy_y[r] = syy_y[r] + hc[m] * (cs[N + r + m - 1] - cs[N + r - m]);
+        }
+    }
+}
+
 // x / K[h] of the CPML update.  With every profile value K = 1 (k_max_PML = 1,
 // the usual case) the division is skipped: x / 1 is exactly x, so the
 // result is the reference's bit for bit and the correctly rounded division

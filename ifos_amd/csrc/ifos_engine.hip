@@ -721,7 +721,8 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
     int unit = c.DTINV;
     while (unit < 32 && NT % (unit * 2) == 0) unit *= 2;
     const int kind = sw;
-    const int key[6] = {E->nsrc, E->ntr, (snapshots ? 1 : 0) + 2 * (int)(E->shape.cpml_weight * 4.0f), E->nslots,
+    const int key[6] = {E->nsrc, E->ntr,
+                        (snapshots ? 1 : 0) + 2 * (int)(E->shape.cpml_weight * 4.0f) + 64 * E->shape.rows_v, E->nslots,
                         ((E->shape.order * 8 + E->shape.spb) * 4096 + E->shape.grid_v) * 4096 + E->shape.grid, unit};
     const LaunchShape ls = E->shape;
     StepArgs a = shaped_args(step_args(E, sw, snapshots), ls);
@@ -1340,6 +1341,19 @@ extern "C" int ifos_engine_get_launch(ifos_engine *E, int *spb, int *grid)
     return 0;
 }
 
+extern "C" int ifos_engine_set_velocity_rows(ifos_engine *E, int rows)
+{
+    if (rows != 1 && rows != 2) return ifos_set_error("velocity rows per thread must be 1 or 2"), -2;
+    E->shape.rows_v = rows;
+    return 0;
+}
+
+extern "C" int ifos_engine_get_velocity_rows(ifos_engine *E, int *rows)
+{
+    *rows = E->shape.rows_v;
+    return 0;
+}
+
 extern "C" int ifos_engine_set_work_order(ifos_engine *E, int slot_major)
 {
     E->shape.order = slot_major ? 1 : 0;
@@ -1469,10 +1483,11 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
     HIP_TRY(hipEventCreate(&t1));
     double best = 1e30;
     LaunchShape pick = E->shape;
-    // each kernel's persistent grid is chosen on its own, timed in the
-    // alternation the time loop runs (velocity, stress, velocity, ...) with
-    // events between the kernels; shots per workgroup and the work order are
-    // common to both (one argument block)
+    // each kernel's persistent grid (and the velocity step's rows per
+    // thread) is chosen on its own, timed in the alternation the time loop
+    // runs (velocity, stress, velocity, ...) with events between the
+    // kernels; shots per workgroup and the work order are common to both
+    // (one argument block)
     constexpr int kMaxReps = 16;
     reps = std::min(reps, kMaxReps);
     hipEvent_t ev[2 * kMaxReps + 1];
@@ -1488,9 +1503,11 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
             const StepArgs a = shaped_args(a0, ls);
             ENG_TRY(upload_args(E, a, 3, &da));
             float best_v = 1e30f, best_s = 1e30f;
-            int grid_v = 0, grid_s = 0;
-            for (int per_cu : {0, 2, 3, 4}) {
+            int grid_v = 0, grid_s = 0, rows_v = 1;
+            for (int per_cu : {0, 2, 3, 4})
+            for (int rows : {1, 2}) {
                 ls.grid = ls.grid_v = per_cu ? (per_cu * ncu + 7) / 8 * 8 : 0;
+                ls.rows_v = rows;
                 launch_vel_step(a, da, ls, 2, E->stream);
                 launch_str_step(a, da, ls, 2, E->stream);
                 HIP_TRY(hipEventRecord(ev[0], E->stream));
@@ -1511,13 +1528,14 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
                 }
                 mv /= reps;
                 ms /= reps;
-                if (mv < best_v) best_v = mv, grid_v = ls.grid_v;
+                if (mv < best_v) best_v = mv, grid_v = ls.grid_v, rows_v = rows;
                 if (ms < best_s) best_s = ms, grid_s = ls.grid;
             }
             if (best_v + best_s < best) {
                 best = best_v + best_s;
                 pick = ls;
                 pick.grid_v = grid_v;
+                pick.rows_v = rows_v;
                 pick.grid = grid_s;
             }
         }

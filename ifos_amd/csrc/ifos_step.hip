@@ -168,9 +168,9 @@ __device__ __forceinline__ float dec16(uint16_t h, float scale, int prec)
     return (float)__builtin_bit_cast(_Float16, h) * scale;
 }
 
-// largest |v| over the 64x8 block, for NF values at once; every thread of the
-// block must call it (it holds two barriers)
-template <int NF>
+// largest |v| over the block of NW waves (blockDim.y == NW), for NF values at
+// once; every thread of the block must call it (it holds two barriers)
+template <int NF, int NW = kTileY>
 __device__ __forceinline__ void block_absmax(float (&v)[NF], float (*lds)[kTileY])
 {
 #pragma unroll
@@ -185,7 +185,7 @@ __device__ __forceinline__ void block_absmax(float (&v)[NF], float (*lds)[kTileY
     for (int f = 0; f < NF; f++) {
         float m = lds[f][0];
 #pragma unroll
-        for (int w = 1; w < kTileY; w++) m = fmaxf(m, lds[f][w]);
+        for (int w = 1; w < NW; w++) m = fmaxf(m, lds[f][w]);
         v[f] = m;
     }
     __syncthreads();
@@ -213,6 +213,41 @@ __device__ __forceinline__ void snap_store(const StepArgs &a, int s, int f0, lon
     for (int f = 0; f < NF; f++) {
         const int ex = scale_exp(m[f]);
         if (live) gmem(a.snap16[f0 + f])[o] = enc16(val[f], ldexpf(1.0f, -ex), a.snap_prec);
+        if (threadIdx.x == 0 && threadIdx.y == 0) gmem(a.sscale[f0 + f])[(long)s * a.scs + k * ntile + blk] = ldexpf(1.0f, ex);
+    }
+}
+
+// snap_store for the two cells (rows j and j+1, interior indices e and
+// e + NX) of a thread in a 64x4 block of two-row threads; the 16-bit scale
+// is the same 64x8 tile maximum as in snap_store
+template <int NF>
+__device__ __forceinline__ void snap_store2(const StepArgs &a, int s, int f0, long k, long e, int blk,
+                                            const bool (&live)[2], const float (&val)[2][NF],
+                                            float (*lds)[kTileY])
+{
+    const long o = (long)s * a.sns + k * a.plane + e;
+    const int NX = a.g.NX;
+    if (a.snap_prec == 0) {
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+            if (live[r])
+#pragma unroll
+                for (int f = 0; f < NF; f++) gmem(a.snap[f0 + f])[o + r * NX] = val[r][f];
+        return;
+    }
+    float m[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+        m[f] = fmaxf(live[0] ? fabsf(val[0][f]) : 0.0f, live[1] ? fabsf(val[1][f]) : 0.0f);
+    block_absmax<NF, kTileY / 2>(m, lds);
+    const long ntile = (long)a.nbx * a.nby;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const int ex = scale_exp(m[f]);
+        const float inv = ldexpf(1.0f, -ex);
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+            if (live[r]) gmem(a.snap16[f0 + f])[o + r * NX] = enc16(val[r][f], inv, a.snap_prec);
         if (threadIdx.x == 0 && threadIdx.y == 0) gmem(a.sscale[f0 + f])[(long)s * a.scs + k * ntile + blk] = ldexpf(1.0f, ex);
     }
 }
@@ -296,7 +331,119 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     }
 }
 
-template <int N, int SPB>
+// velocity step of the two vertically adjacent cells (rows j and j+1) of a
+// two-row thread, slot s; the same arithmetic as vel_cell for each cell.  All
+// loads of both cells issue before any store, so both cells' memory
+// latencies overlap; live[1] implies live[0].
+template <int N>
+__device__ __forceinline__ void vel_cell2(const StepArgs &a, int s, int blk, int i, int j, const bool (&live)[2], long c,
+                                          long e, int nt, int flags, const float (&rip)[2], const float (&rjp)[2],
+                                          float (*s_red)[kTileY])
+{
+    const StepGeom &g = a.g;
+    const int P = g.pitch, NX = g.NX;
+    const long wo = (long)s * a.ws, po = (long)s * a.pss;
+    const auto vxs = gmem(a.vx) + wo, vys = gmem(a.vy) + wo;
+    float sxx_x[2] = {0.0f, 0.0f}, sxy_x[2] = {0.0f, 0.0f}, sxy_y[2] = {0.0f, 0.0f}, syy_y[2] = {0.0f, 0.0f};
+    float vx0[2] = {0.0f, 0.0f}, vy0[2] = {0.0f, 0.0f};
+    if (live[0]) {
+        // row j+1 <= NY + 1: its stencil stays inside the halo rows
+        stress_derivs2<N>(g.hc, a.sxx + wo + c, a.syy + wo + c, a.sxy + wo + c, P, sxx_x, sxy_x, sxy_y, syy_y);
+        vx0[0] = vxs[c];
+        vy0[0] = vys[c];
+        if (live[1]) {
+            vx0[1] = vxs[c + P];
+            vy0[1] = vys[c + P];
+        }
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+            if (live[r])
+                cpml_velocity(g, j + r, i, a.psi_sxx_x + po, a.psi_sxy_x + po, a.psi_syy_y + po, a.psi_sxy_y + po,
+                              sxx_x[r], sxy_x[r], sxy_y[r], syy_y[r]);
+    }
+    const int so = __builtin_amdgcn_readfirstlane(s * (a.nbx * a.nby + 1) + blk);  // uniform
+    const int q0 = cmem(a.inj.start)[so], q1 = cmem(a.inj.start)[so + 1];
+    if (flags & kForward) {
+        if (flags & kSnapStep) {
+            float val[2][2];
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                val[r][0] = vel_rate(rip[r], sxx_x[r] + sxy_y[r], g.DH);
+                val[r][1] = vel_rate(rjp[r], sxy_x[r] + syy_y[r], g.DH);
+            }
+            snap_store2<2>(a, s, 0, (nt - 1) / a.DTINV, e, blk, live, val, s_red);
+        }
+    } else if (flags & kAdjInt) {
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+            if (live[r]) {
+                gmem(a.vxp1)[wo + c + r * P] += vx0[r] * g.DT;
+                gmem(a.vyp1)[wo + c + r * P] += vy0[r] * g.DT;
+            }
+    } else if (flags & kCorrStep) {
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+            if (live[r]) {
+                gmem(a.vxp1)[wo + c + r * P] = vx0[r];
+                gmem(a.vyp1)[wo + c + r * P] = vy0[r];
+            }
+    }
+    if (!live[0]) return;
+    float vx[2], vy[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        vx[r] = vel_new(vx0[r], g.DT, rip[r], sxx_x[r] + sxy_y[r], g.DH);
+        vy[r] = vel_new(vy0[r], g.DT, rjp[r], sxy_x[r] + syy_y[r], g.DH);
+    }
+    // point forces / residual injection of this tile, in list order; a point
+    // lands on at most one of the two cells
+    for (int q = q0; q < q1; q++) {
+        const int l = cmem(a.inj.idx)[q];
+        const int px = cmem(a.inj.ix)[l], py = cmem(a.inj.iy)[l];
+        const float s0 = cmem(a.inj.sig)[(long)l * a.NT + nt - 1];
+        const float s1 = cmem(a.inj.sig1)[(long)l * a.NT + nt - 1];
+        if (px != i || (py != j && py != j + 1)) continue;
+        const int r = py - j;
+        float x = r ? vx[1] : vx[0], y = r ? vy[1] : vy[0];
+        const float rp = r ? rip[1] : rip[0], rq = r ? rjp[1] : rjp[0];
+        if (a.sw == 0) {
+            const int type = cmem(a.inj.type)[l];
+            if (type == 2) x += (g.DT * rp * s1) / (g.DH * g.DH);
+            if (type == 3) y += (g.DT * rq * s1) / (g.DH * g.DH);
+            if (type == 4) {
+                x += (g.DT * rp * cmem(a.inj.sin_azi)[l] * s0) / (g.DH * g.DH);
+                y += (g.DT * rq * cmem(a.inj.cos_azi)[l] * s0) / (g.DH * g.DH);
+            }
+        } else {
+            if (a.ADJOINT_TYPE == 1) { x += s0; y += s1; }
+            if (a.ADJOINT_TYPE == 2) y += s1;
+            if (a.ADJOINT_TYPE == 3) x += s0;
+        }
+        if (r) {
+            vx[1] = x;
+            vy[1] = y;
+        } else {
+            vx[0] = x;
+            vy[0] = y;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+        if (live[r]) {
+            vxs[c + r * P] = vx[r];
+            vys[c + r * P] = vy[r];
+        }
+    if (flags & kEnergy) {
+        const auto W = gmem(a.wacc) + (long)s * a.wacs;
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+            if (live[r]) W[e + r * NX] += ((vx[r] * vx[r]) + (vy[r] * vy[r]));
+    }
+}
+
+// RY = 1: a 64x8 block, one cell per thread.  RY = 2: a 64x4 block, each
+// thread owns rows 2*ty and 2*ty+1 of the 64x8 tile (vel_cell2).
+template <int N, int SPB, int RY>
 __global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ pa, int k, int persistent, int flags)
 {
     const StepArgs &a = *pa;
@@ -314,14 +461,27 @@ __global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ p
         s0 = __builtin_amdgcn_readfirstlane(s0);
         const int by = (int)fdiv(blk, m.nbx_mul, m.nbx_sh), bx = blk - by * m.nbx;
         const int i = bx * kTileX + threadIdx.x + 1;
-        const int j = by * kTileY + threadIdx.y + 1;
-        const bool live = i <= g.NX && j <= g.NY;
+        const int j = by * kTileY + RY * threadIdx.y + 1;
         const long c = (long)j * g.pitch + i;
         const long e = (long)(j - 1) * g.NX + (i - 1);
-        const float rip = live ? gmem(a.rip)[c] : 0.0f, rjp = live ? gmem(a.rjp)[c] : 0.0f;
+        if constexpr (RY == 1) {
+            const bool live = i <= g.NX && j <= g.NY;
+            const float rip = live ? gmem(a.rip)[c] : 0.0f, rjp = live ? gmem(a.rjp)[c] : 0.0f;
 #pragma unroll
-        for (int u = 0; u < SPB; u++)
-            if (s0 + u < m.nslots) vel_cell<N>(a, s0 + u, blk, i, j, live, c, e, nt, flags, rip, rjp, s_red);
+            for (int u = 0; u < SPB; u++)
+                if (s0 + u < m.nslots) vel_cell<N>(a, s0 + u, blk, i, j, live, c, e, nt, flags, rip, rjp, s_red);
+        } else {
+            const bool live[2] = {i <= g.NX && j <= g.NY, i <= g.NX && j + 1 <= g.NY};
+            float rip[2], rjp[2];
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                rip[r] = live[r] ? gmem(a.rip)[c + r * g.pitch] : 0.0f;
+                rjp[r] = live[r] ? gmem(a.rjp)[c + r * g.pitch] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < SPB; u++)
+                if (s0 + u < m.nslots) vel_cell2<N>(a, s0 + u, blk, i, j, live, c, e, nt, flags, rip, rjp, s_red);
+        }
     }
 }
 
@@ -578,8 +738,12 @@ template <int N, int SPB>
 static void vel_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
 {
     const int gr = grid_of(a, ls, ls.grid_v);
-    hipLaunchKernelGGL((k_vel_step<N, SPB>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid_v > 0 ? 1 : 0,
-                       step_flags(a, k));
+    if (ls.rows_v == 2)
+        hipLaunchKernelGGL((k_vel_step<N, SPB, 2>), dim3(gr), dim3(kTileX, kTileY / 2), 0, s, da, k,
+                           ls.grid_v > 0 ? 1 : 0, step_flags(a, k));
+    else
+        hipLaunchKernelGGL((k_vel_step<N, SPB, 1>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k,
+                           ls.grid_v > 0 ? 1 : 0, step_flags(a, k));
 }
 
 template <int N>

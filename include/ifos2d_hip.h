@@ -123,6 +123,12 @@ int  ifos_engine_get_launch_grids(ifos_engine *e, int *grid_vel, int *grid_str);
  * neighbouring tiles, whose halo rows they share, run close together). */
 int  ifos_engine_set_work_order(ifos_engine *e, int slot_major);
 int  ifos_engine_get_work_order(ifos_engine *e, int *slot_major);
+/* Cells per thread along y in the velocity step: 1 (one cell per thread of
+ * a 64x8 workgroup) or 2 (two adjacent rows per thread of a 64x4 workgroup,
+ * which share the y-stencil loads).  Results are the same bits either way;
+ * the autotuner picks the faster. */
+int  ifos_engine_set_velocity_rows(ifos_engine *e, int rows);
+int  ifos_engine_get_velocity_rows(ifos_engine *e, int *rows);
 /* Time every candidate launch shape (and, with several slots, both work
  * orders) on one forward step of the current model and shots (reps launches
  * of each kernel), keep the fastest, zero the stepped wavefields; reports
