@@ -171,7 +171,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape (no timing sweep)")
     ap.add_argument("--order", default="auto", choices=["auto", "tile", "slot"],
                     help="work order of several-shot launches (auto: the autotuner's pick)")
-    ap.add_argument("--vel-rows", type=int, default=0, choices=[0, 1, 2],
+    ap.add_argument("--vel-rows", type=int, default=0, choices=[0, 1, 2, 4],
                     help="velocity step cells per thread along y (0: the autotuner's pick, or 1 without it)")
     ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"],
                     help="snapshot storage (fp32 keeps the gradient bit-identical to the reference)")

@@ -29,25 +29,31 @@ __device__ __forceinline__ void stress_derivs(const float *hc, const float *sxx_
     }
 }
 
-// stress_derivs for two vertically adjacent cells, rows j (r = 0, at the
-// given pointers) and j+1 (r = 1): the y derivatives of both rows read one
-// column of 2N+1 values of sxy (rows j-N .. j+N) and of syy (rows j-N+1 ..
-// j+N+1) instead of 4N; each row's sums keep stress_derivs' order
-template <int N>
-__device__ __forceinline__ void stress_derivs2(const float *hc, const float *sxx_, const float *syy_,
-                                               const float *sxy_, int P, float (&sxx_x)[2], float (&sxy_x)[2],
-                                               float (&sxy_y)[2], float (&syy_y)[2])
+// stress_derivs for R vertically adjacent cells, rows j + r (r = 0 .. R-1;
+// row j at the given pointers): the y derivatives of all R rows read one
+// column of 2N+R-1 values of sxy (rows j-N .. j+N+R-2) and of syy (rows
+// j-N+1 .. j+N+R-1) instead of 2N per row; each row's sums keep
+// stress_derivs' order.  lim is the last row offset from j inside the
+// padded grid: for R > 2 the reads of rows past the model's last row, which
+// only feed cells that are not stored, are clamped to it (R <= 2 never reads
+// past it)
+template <int N, int R>
+__device__ __forceinline__ void stress_derivs_rows(const float *hc, const float *sxx_, const float *syy_,
+                                                   const float *sxy_, int P, int lim, float (&sxx_x)[R],
+                                                   float (&sxy_x)[R], float (&sxy_y)[R], float (&syy_y)[R])
 {
     const auto sxx = gmem(sxx_), syy = gmem(syy_), sxy = gmem(sxy_);
-    float cy[2 * N + 1], cs[2 * N + 1];  // cy[k]: sxy row k-N; cs[k]: syy row k-N+1
+    constexpr int K = 2 * N + R - 1;
+    auto row = [&](int o) { return R > 2 ? min(o, lim) : o; };
+    float cy[K], cs[K];  // cy[k]: sxy row k-N; cs[k]: syy row k-N+1
 #pragma unroll
-    for (int k = 0; k <= 2 * N; k++) {
-        cy[k] = sxy[(k - N) * P];
-        cs[k] = syy[(k - N + 1) * P];
+    for (int k = 0; k < K; k++) {
+        cy[k] = sxy[row(k - N) * P];
+        cs[k] = syy[row(k - N + 1) * P];
     }
 #pragma unroll
-    for (int r = 0; r < 2; r++) {
-        const auto sx = sxx + r * P, sy = sxy + r * P;
+    for (int r = 0; r < R; r++) {
+        const auto sx = sxx + row(r) * P, sy = sxy + row(r) * P;
         sxx_x[r] = hc[1] * (sx[1] - sx[0]);
         sxy_x[r] = hc[1] * (cy[N + r] - sy[-1]);
         sxy_y[r] = hc[1] * (cy[N + r] - cy[N + r - 1]);

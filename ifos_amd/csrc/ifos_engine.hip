@@ -1343,7 +1343,7 @@ extern "C" int ifos_engine_get_launch(ifos_engine *E, int *spb, int *grid)
 
 extern "C" int ifos_engine_set_velocity_rows(ifos_engine *E, int rows)
 {
-    if (rows != 1 && rows != 2) return ifos_set_error("velocity rows per thread must be 1 or 2"), -2;
+    if (rows != 1 && rows != 2 && rows != 4) return ifos_set_error("velocity rows per thread must be 1, 2 or 4"), -2;
     E->shape.rows_v = rows;
     return 0;
 }
@@ -1505,7 +1505,7 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
             float best_v = 1e30f, best_s = 1e30f;
             int grid_v = 0, grid_s = 0, rows_v = 1;
             for (int per_cu : {0, 2, 3, 4})
-            for (int rows : {1, 2}) {
+            for (int rows : {1, 2, 4}) {
                 ls.grid = ls.grid_v = per_cu ? (per_cu * ncu + 7) / 8 * 8 : 0;
                 ls.rows_v = rows;
                 launch_vel_step(a, da, ls, 2, E->stream);

@@ -89,7 +89,7 @@ struct LaunchShape {
     int grid = 0;    // stress step
     int grid_v = 0;  // velocity step
     int order = 0;   // StepArgs::slot_major
-    int rows_v = 1;  // velocity step: cells per thread along y (1: 64x8 block, 2: 64x4 block)
+    int rows_v = 1;  // velocity step: cells per thread along y (1, 2 or 4: a 64x(8/rows) block)
     float cpml_weight = 1.0f;  // extra cost of a tile in an absorbing strip, for the XCD split
 };
 

@@ -217,19 +217,19 @@ __device__ __forceinline__ void snap_store(const StepArgs &a, int s, int f0, lon
     }
 }
 
-// snap_store for the two cells (rows j and j+1, interior indices e and
-// e + NX) of a thread in a 64x4 block of two-row threads; the 16-bit scale
-// is the same 64x8 tile maximum as in snap_store
-template <int NF>
-__device__ __forceinline__ void snap_store2(const StepArgs &a, int s, int f0, long k, long e, int blk,
-                                            const bool (&live)[2], const float (&val)[2][NF],
-                                            float (*lds)[kTileY])
+// snap_store for the R cells (rows j .. j+R-1, interior indices e + r*NX) of
+// a thread in a 64x(8/R) block of R-row threads; the 16-bit scale is the
+// same 64x8 tile maximum as in snap_store
+template <int NF, int R>
+__device__ __forceinline__ void snap_store_rows(const StepArgs &a, int s, int f0, long k, long e, int blk,
+                                                const bool (&live)[R], const float (&val)[R][NF],
+                                                float (*lds)[kTileY])
 {
     const long o = (long)s * a.sns + k * a.plane + e;
     const int NX = a.g.NX;
     if (a.snap_prec == 0) {
 #pragma unroll
-        for (int r = 0; r < 2; r++)
+        for (int r = 0; r < R; r++)
             if (live[r])
 #pragma unroll
                 for (int f = 0; f < NF; f++) gmem(a.snap[f0 + f])[o + r * NX] = val[r][f];
@@ -237,16 +237,19 @@ __device__ __forceinline__ void snap_store2(const StepArgs &a, int s, int f0, lo
     }
     float m[NF];
 #pragma unroll
-    for (int f = 0; f < NF; f++)
-        m[f] = fmaxf(live[0] ? fabsf(val[0][f]) : 0.0f, live[1] ? fabsf(val[1][f]) : 0.0f);
-    block_absmax<NF, kTileY / 2>(m, lds);
+    for (int f = 0; f < NF; f++) {
+        m[f] = live[0] ? fabsf(val[0][f]) : 0.0f;
+#pragma unroll
+        for (int r = 1; r < R; r++) m[f] = fmaxf(m[f], live[r] ? fabsf(val[r][f]) : 0.0f);
+    }
+    block_absmax<NF, kTileY / R>(m, lds);
     const long ntile = (long)a.nbx * a.nby;
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         const int ex = scale_exp(m[f]);
         const float inv = ldexpf(1.0f, -ex);
 #pragma unroll
-        for (int r = 0; r < 2; r++)
+        for (int r = 0; r < R; r++)
             if (live[r]) gmem(a.snap16[f0 + f])[o + r * NX] = enc16(val[r][f], inv, a.snap_prec);
         if (threadIdx.x == 0 && threadIdx.y == 0) gmem(a.sscale[f0 + f])[(long)s * a.scs + k * ntile + blk] = ldexpf(1.0f, ex);
     }
@@ -331,32 +334,35 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     }
 }
 
-// velocity step of the two vertically adjacent cells (rows j and j+1) of a
-// two-row thread, slot s; the same arithmetic as vel_cell for each cell.  All
-// loads of both cells issue before any store, so both cells' memory
-// latencies overlap; live[1] implies live[0].
-template <int N>
-__device__ __forceinline__ void vel_cell2(const StepArgs &a, int s, int blk, int i, int j, const bool (&live)[2], long c,
-                                          long e, int nt, int flags, const float (&rip)[2], const float (&rjp)[2],
-                                          float (*s_red)[kTileY])
+// velocity step of the R vertically adjacent cells (rows j .. j+R-1) of an
+// R-row thread, slot s; the same arithmetic as vel_cell for each cell.  All
+// loads of the R cells issue before any store, so their memory latencies
+// overlap; live[r] implies live[r-1].
+template <int N, int R>
+__device__ __forceinline__ void vel_cell_rows(const StepArgs &a, int s, int blk, int i, int j, const bool (&live)[R],
+                                              long c, long e, int nt, int flags, const float (&rip)[R],
+                                              const float (&rjp)[R], float (*s_red)[kTileY])
 {
     const StepGeom &g = a.g;
     const int P = g.pitch, NX = g.NX;
     const long wo = (long)s * a.ws, po = (long)s * a.pss;
     const auto vxs = gmem(a.vx) + wo, vys = gmem(a.vy) + wo;
-    float sxx_x[2] = {0.0f, 0.0f}, sxy_x[2] = {0.0f, 0.0f}, sxy_y[2] = {0.0f, 0.0f}, syy_y[2] = {0.0f, 0.0f};
-    float vx0[2] = {0.0f, 0.0f}, vy0[2] = {0.0f, 0.0f};
-    if (live[0]) {
-        // row j+1 <= NY + 1: its stencil stays inside the halo rows
-        stress_derivs2<N>(g.hc, a.sxx + wo + c, a.syy + wo + c, a.sxy + wo + c, P, sxx_x, sxy_x, sxy_y, syy_y);
-        vx0[0] = vxs[c];
-        vy0[0] = vys[c];
-        if (live[1]) {
-            vx0[1] = vxs[c + P];
-            vy0[1] = vys[c + P];
-        }
+    float sxx_x[R], sxy_x[R], sxy_y[R], syy_y[R], vx0[R], vy0[R];
 #pragma unroll
-        for (int r = 0; r < 2; r++)
+    for (int r = 0; r < R; r++) sxx_x[r] = sxy_x[r] = sxy_y[r] = syy_y[r] = vx0[r] = vy0[r] = 0.0f;
+    if (live[0]) {
+        // the padded grid holds rows 1-H .. NY+H with H = N+1; with j <= NY,
+        // R = 2 reads up to row NY+N+1 and larger R clamp to it
+        stress_derivs_rows<N, R>(g.hc, a.sxx + wo + c, a.syy + wo + c, a.sxy + wo + c, P, g.NY + N + 1 - j, sxx_x,
+                                 sxy_x, sxy_y, syy_y);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (live[r]) {
+                vx0[r] = vxs[c + r * P];
+                vy0[r] = vys[c + r * P];
+            }
+#pragma unroll
+        for (int r = 0; r < R; r++)
             if (live[r])
                 cpml_velocity(g, j + r, i, a.psi_sxx_x + po, a.psi_sxy_x + po, a.psi_syy_y + po, a.psi_sxy_y + po,
                               sxx_x[r], sxy_x[r], sxy_y[r], syy_y[r]);
@@ -365,70 +371,64 @@ __device__ __forceinline__ void vel_cell2(const StepArgs &a, int s, int blk, int
     const int q0 = cmem(a.inj.start)[so], q1 = cmem(a.inj.start)[so + 1];
     if (flags & kForward) {
         if (flags & kSnapStep) {
-            float val[2][2];
+            float val[R][2];
 #pragma unroll
-            for (int r = 0; r < 2; r++) {
+            for (int r = 0; r < R; r++) {
                 val[r][0] = vel_rate(rip[r], sxx_x[r] + sxy_y[r], g.DH);
                 val[r][1] = vel_rate(rjp[r], sxy_x[r] + syy_y[r], g.DH);
             }
-            snap_store2<2>(a, s, 0, (nt - 1) / a.DTINV, e, blk, live, val, s_red);
+            snap_store_rows<2, R>(a, s, 0, (nt - 1) / a.DTINV, e, blk, live, val, s_red);
         }
     } else if (flags & kAdjInt) {
 #pragma unroll
-        for (int r = 0; r < 2; r++)
+        for (int r = 0; r < R; r++)
             if (live[r]) {
                 gmem(a.vxp1)[wo + c + r * P] += vx0[r] * g.DT;
                 gmem(a.vyp1)[wo + c + r * P] += vy0[r] * g.DT;
             }
     } else if (flags & kCorrStep) {
 #pragma unroll
-        for (int r = 0; r < 2; r++)
+        for (int r = 0; r < R; r++)
             if (live[r]) {
                 gmem(a.vxp1)[wo + c + r * P] = vx0[r];
                 gmem(a.vyp1)[wo + c + r * P] = vy0[r];
             }
     }
     if (!live[0]) return;
-    float vx[2], vy[2];
+    float vx[R], vy[R];
 #pragma unroll
-    for (int r = 0; r < 2; r++) {
+    for (int r = 0; r < R; r++) {
         vx[r] = vel_new(vx0[r], g.DT, rip[r], sxx_x[r] + sxy_y[r], g.DH);
         vy[r] = vel_new(vy0[r], g.DT, rjp[r], sxy_x[r] + syy_y[r], g.DH);
     }
     // point forces / residual injection of this tile, in list order; a point
-    // lands on at most one of the two cells
+    // lands on at most one of the R cells
     for (int q = q0; q < q1; q++) {
         const int l = cmem(a.inj.idx)[q];
         const int px = cmem(a.inj.ix)[l], py = cmem(a.inj.iy)[l];
         const float s0 = cmem(a.inj.sig)[(long)l * a.NT + nt - 1];
         const float s1 = cmem(a.inj.sig1)[(long)l * a.NT + nt - 1];
-        if (px != i || (py != j && py != j + 1)) continue;
-        const int r = py - j;
-        float x = r ? vx[1] : vx[0], y = r ? vy[1] : vy[0];
-        const float rp = r ? rip[1] : rip[0], rq = r ? rjp[1] : rjp[0];
-        if (a.sw == 0) {
-            const int type = cmem(a.inj.type)[l];
-            if (type == 2) x += (g.DT * rp * s1) / (g.DH * g.DH);
-            if (type == 3) y += (g.DT * rq * s1) / (g.DH * g.DH);
-            if (type == 4) {
-                x += (g.DT * rp * cmem(a.inj.sin_azi)[l] * s0) / (g.DH * g.DH);
-                y += (g.DT * rq * cmem(a.inj.cos_azi)[l] * s0) / (g.DH * g.DH);
+        if (px != i || py < j || py >= j + R) continue;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (py != j + r) continue;
+            if (a.sw == 0) {
+                const int type = cmem(a.inj.type)[l];
+                if (type == 2) vx[r] += (g.DT * rip[r] * s1) / (g.DH * g.DH);
+                if (type == 3) vy[r] += (g.DT * rjp[r] * s1) / (g.DH * g.DH);
+                if (type == 4) {
+                    vx[r] += (g.DT * rip[r] * cmem(a.inj.sin_azi)[l] * s0) / (g.DH * g.DH);
+                    vy[r] += (g.DT * rjp[r] * cmem(a.inj.cos_azi)[l] * s0) / (g.DH * g.DH);
+                }
+            } else {
+                if (a.ADJOINT_TYPE == 1) { vx[r] += s0; vy[r] += s1; }
+                if (a.ADJOINT_TYPE == 2) vy[r] += s1;
+                if (a.ADJOINT_TYPE == 3) vx[r] += s0;
             }
-        } else {
-            if (a.ADJOINT_TYPE == 1) { x += s0; y += s1; }
-            if (a.ADJOINT_TYPE == 2) y += s1;
-            if (a.ADJOINT_TYPE == 3) x += s0;
-        }
-        if (r) {
-            vx[1] = x;
-            vy[1] = y;
-        } else {
-            vx[0] = x;
-            vy[0] = y;
         }
     }
 #pragma unroll
-    for (int r = 0; r < 2; r++)
+    for (int r = 0; r < R; r++)
         if (live[r]) {
             vxs[c + r * P] = vx[r];
             vys[c + r * P] = vy[r];
@@ -436,13 +436,14 @@ __device__ __forceinline__ void vel_cell2(const StepArgs &a, int s, int blk, int
     if (flags & kEnergy) {
         const auto W = gmem(a.wacc) + (long)s * a.wacs;
 #pragma unroll
-        for (int r = 0; r < 2; r++)
+        for (int r = 0; r < R; r++)
             if (live[r]) W[e + r * NX] += ((vx[r] * vx[r]) + (vy[r] * vy[r]));
     }
 }
 
-// RY = 1: a 64x8 block, one cell per thread.  RY = 2: a 64x4 block, each
-// thread owns rows 2*ty and 2*ty+1 of the 64x8 tile (vel_cell2).
+// RY = 1: a 64x8 block, one cell per thread.  RY = 2 or 4: a 64x(8/RY)
+// block, each thread owns rows RY*ty .. RY*ty+RY-1 of the 64x8 tile
+// (vel_cell_rows).
 template <int N, int SPB, int RY>
 __global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ pa, int k, int persistent, int flags)
 {
@@ -471,16 +472,18 @@ __global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ p
             for (int u = 0; u < SPB; u++)
                 if (s0 + u < m.nslots) vel_cell<N>(a, s0 + u, blk, i, j, live, c, e, nt, flags, rip, rjp, s_red);
         } else {
-            const bool live[2] = {i <= g.NX && j <= g.NY, i <= g.NX && j + 1 <= g.NY};
-            float rip[2], rjp[2];
+            bool live[RY];
+            float rip[RY], rjp[RY];
 #pragma unroll
-            for (int r = 0; r < 2; r++) {
+            for (int r = 0; r < RY; r++) {
+                live[r] = i <= g.NX && j + r <= g.NY;
                 rip[r] = live[r] ? gmem(a.rip)[c + r * g.pitch] : 0.0f;
                 rjp[r] = live[r] ? gmem(a.rjp)[c + r * g.pitch] : 0.0f;
             }
 #pragma unroll
             for (int u = 0; u < SPB; u++)
-                if (s0 + u < m.nslots) vel_cell2<N>(a, s0 + u, blk, i, j, live, c, e, nt, flags, rip, rjp, s_red);
+                if (s0 + u < m.nslots)
+                    vel_cell_rows<N, RY>(a, s0 + u, blk, i, j, live, c, e, nt, flags, rip, rjp, s_red);
         }
     }
 }
@@ -738,7 +741,10 @@ template <int N, int SPB>
 static void vel_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
 {
     const int gr = grid_of(a, ls, ls.grid_v);
-    if (ls.rows_v == 2)
+    if (ls.rows_v == 4)
+        hipLaunchKernelGGL((k_vel_step<N, SPB, 4>), dim3(gr), dim3(kTileX, kTileY / 4), 0, s, da, k,
+                           ls.grid_v > 0 ? 1 : 0, step_flags(a, k));
+    else if (ls.rows_v == 2)
         hipLaunchKernelGGL((k_vel_step<N, SPB, 2>), dim3(gr), dim3(kTileX, kTileY / 2), 0, s, da, k,
                            ls.grid_v > 0 ? 1 : 0, step_flags(a, k));
     else

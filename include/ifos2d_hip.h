@@ -124,9 +124,9 @@ int  ifos_engine_get_launch_grids(ifos_engine *e, int *grid_vel, int *grid_str);
 int  ifos_engine_set_work_order(ifos_engine *e, int slot_major);
 int  ifos_engine_get_work_order(ifos_engine *e, int *slot_major);
 /* Cells per thread along y in the velocity step: 1 (one cell per thread of
- * a 64x8 workgroup) or 2 (two adjacent rows per thread of a 64x4 workgroup,
- * which share the y-stencil loads).  Results are the same bits either way;
- * the autotuner picks the faster. */
+ * a 64x8 workgroup), 2 or 4 (adjacent rows per thread of a 64x4 or 64x2
+ * workgroup, which share the y-stencil loads).  Results are the same bits
+ * every way; the autotuner picks the fastest. */
 int  ifos_engine_set_velocity_rows(ifos_engine *e, int rows);
 int  ifos_engine_get_velocity_rows(ifos_engine *e, int *rows);
 /* Time every candidate launch shape (and, with several slots, both work

@@ -1,5 +1,6 @@
-"""Velocity step with two cells per thread (two adjacent rows of the 64x8
-tile per thread of a 64x4 workgroup): the same bits as one cell per thread
+"""Velocity step with two or four cells per thread (adjacent rows of the
+64x8 tile per thread of a 64x4 or 64x2 workgroup): the same bits as one
+cell per thread
 and as the oracle, for every part of the velocity step -- CPML strips on all
 sides, force sources and adjoint injection on either row of a thread, grids
 whose last tile row is partly outside the model, acceleration snapshots in
@@ -15,8 +16,8 @@ from test_precision import params_of
 
 pytestmark = pytest.mark.gpu
 
-# NY not a multiple of 8 (the last tile row is partly outside the model) and
-# odd (the last thread row has its second cell outside)
+# NY not a multiple of 8 (the last tile row is partly outside the model),
+# odd, and 1 or 3 mod 4 (the last thread row has cells outside)
 CASES = [dict(seed=61, NX=75, NY=51, NT=150, FDORDER=8, L=2, VELOCITY=1, DTINV=3, stype=1),
          dict(seed=62, NX=70, NY=45, NT=140, FDORDER=4, L=0, VELOCITY=0, DTINV=2, stype=4, FREE_SURF=0),
          dict(seed=63, NX=66, NY=50, NT=130, FDORDER=12, L=1, VELOCITY=1, DTINV=2, stype=2),
@@ -46,40 +47,43 @@ def run(w, rows, obs=None, slots=1, shots=None, spb=1, grid=0, prec=0, eprecond=
     return out
 
 
+@pytest.mark.parametrize("rows", [2, 4])
 @pytest.mark.parametrize("case", CASES, ids=["visc8_odd", "el4_v0_pml_top", "visc12_force", "el2_force"])
-def test_two_rows_match_oracle(gpu_lib, case):
+def test_rows_match_oracle(gpu_lib, case, rows):
     w = workloads.small_case(FW=10, **case)
-    print(f"[rows 2] {case}", flush=True)
+    print(f"[rows {rows}] {case}", flush=True)
     p = params_of(w)
     s = w.shots[0]
     ovx, ovy, ofin = O.forward(p, w.true, s.srcpos, s.recpos)
-    (vx, vy), fin = run(w, 2)
+    (vx, vy), fin = run(w, rows)
     np.testing.assert_array_equal(vx, ovx)
     np.testing.assert_array_equal(vy, ovy)
     np.testing.assert_array_equal(fin, ofin)
     ovx[:, 0] = 0
     ovy[:, 0] = 0
     ref = O.gradient(p, w.start, s.srcpos, s.recpos, ovx, ovy)
-    _, g, _ = run(w, 2, obs=(ovx, ovy))
+    _, g, _ = run(w, rows, obs=(ovx, ovy))
     for k, n in enumerate(("vp", "vs", "rho")):
         np.testing.assert_array_equal(g[k], ref["grad_" + n], err_msg=n)
 
 
+@pytest.mark.parametrize("rows", [2, 4])
 @pytest.mark.parametrize("prec", [1, 2])
-def test_two_rows_snapshot16(gpu_lib, prec):
+def test_rows_snapshot16(gpu_lib, prec, rows):
     w = workloads.small_case(seed=65, NX=80, NY=47, NT=150, FDORDER=8, L=2, VELOCITY=1, DTINV=3, FW=10)
-    print(f"[rows 2] 16-bit snapshots prec {prec}", flush=True)
+    print(f"[rows {rows}] 16-bit snapshots prec {prec}", flush=True)
     s = w.shots[0]
     ovx, ovy, _ = O.forward(params_of(w), w.true, s.srcpos, s.recpos)
     want = O.gradient(params_of(w, SNAP_PREC=prec), w.start, s.srcpos, s.recpos, ovx, ovy)
-    _, g, _ = run(w, 2, obs=(ovx, ovy), prec=prec)
+    _, g, _ = run(w, rows, obs=(ovx, ovy), prec=prec)
     for k, n in enumerate(("vp", "vs", "rho")):
         np.testing.assert_array_equal(g[k], want["grad_" + n], err_msg=n)
 
 
-def test_two_rows_energy_and_batches(gpu_lib):
+@pytest.mark.parametrize("rows", [2, 4])
+def test_rows_energy_and_batches(gpu_lib, rows):
     w = workloads.small_case(seed=66, NX=90, NY=53, NT=160, FDORDER=6, L=1, VELOCITY=1, DTINV=2, FW=10)
-    print("[rows 2] energy sums, 3 shots, 2 per workgroup, persistent grid", flush=True)
+    print(f"[rows {rows}] energy sums, 3 shots, 2 per workgroup, persistent grid", flush=True)
     base = np.array(w.shots[0].srcpos, np.float32).reshape(-1, 8)
     shots = []
     for x, t in ((20, 1), (44, 2), (70, 4)):
@@ -93,7 +97,7 @@ def test_two_rows_energy_and_batches(gpu_lib):
     ovy = np.concatenate([o[1] for o in obs])
     one = run(w, 1, obs=(ovx, ovy), slots=3, shots=shots, eprecond=1)
     for spb, grid in ((1, 0), (2, 0), (2, 16)):
-        two = run(w, 2, obs=(ovx, ovy), slots=3, shots=shots, spb=spb, grid=grid, eprecond=1)
+        two = run(w, rows, obs=(ovx, ovy), slots=3, shots=shots, spb=spb, grid=grid, eprecond=1)
         np.testing.assert_array_equal(two[0][0], one[0][0], err_msg=f"vx spb {spb} grid {grid}")
         np.testing.assert_array_equal(two[1], one[1], err_msg=f"gradient spb {spb} grid {grid}")
         np.testing.assert_array_equal(two[2], one[2], err_msg=f"energy spb {spb} grid {grid}")
