@@ -211,6 +211,8 @@ def main():
         f"DTINV={cfg.DTINV}, {len(shots)} shots in batches of {S}, {args.snap} snapshots")
 
     eng = ifos_amd.Engine(cfg, slots=S)
+    if args.vel_rows:
+        eng.set_velocity_rows(args.vel_rows)
     if args.copy_probe:
         nb = args.copy_probe * (1 << 20)
         a, c = ifos_amd.DeviceBuffer(nb), ifos_amd.DeviceBuffer(nb)
