@@ -30,6 +30,7 @@
 #include "host/ifos_setup.h"
 #include "ifos2d_dropin.h"
 #include "ifos2d_hip.h"
+#include "ifos_cellmath.h"
 #include "ifos_internal.h"
 
 using namespace ifos;
@@ -728,4 +729,205 @@ extern "C" void zero_fdveps_visc(int ny1, int ny2, int nx1, int nx2, float **vx,
     zero_dev(pq);
     if (pt && lookup(pt)) zero_dev(pt);
     if (po && lookup(po)) zero_dev(po);
+}
+
+// ---------------------------------------------------------------------------
+// Snapshot store, zero-lag correlation and chain rule.  The reference runs
+// them inline in main (src/IFOS2D.c:1805-1871, 2381-2419, 2471-2611) on the
+// arrays main holds; these entry points run the same loops on device arrays
+// (WAVETYPE 1, not ACOUSTIC, IDXI = IDYI = IDX = IDY = 1, PARAMETERIZATION 1).
+// Snapshot tensors keep the reference's f3tensor layout, time fastest.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Mat {
+    float *o;  // origin: element [j][i] at o + j * pitch + i
+};
+
+// the kernels below touch rows 1..NY and columns 1..NX of every array
+void check_covers(const void *arr, const char *name)
+{
+    const DevArr *d = lookup(arr);
+    if (!d || d->nrl > 1 || d->nrh < NY || d->ncl > 1 || d->nch < NX) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "%s must be a device array covering rows 1..NY and columns 1..NX", name);
+        fatal(buf);
+    }
+}
+
+// origin of a device matrix whose row length must equal P (P <= 0: set it)
+Mat mat_of(float **m, int &P, const char *name)
+{
+    int p;
+    float *o = origin_of(m, &p, name);
+    check_covers(m, name);
+    if (P <= 0) P = p;
+    if (p != P) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "%s must have the wavefield grid's column bounds", name);
+        fatal(buf);
+    }
+    return Mat{o};
+}
+
+struct SnapStoreArgs {
+    int NX, NY, P, nd, k;  // k: 0-based time slot
+    const float *src[5];
+    float *dst[5];         // tensor origins: element (j, i, l) at dst + (j * P + i) * nd + l - 1
+};
+
+__global__ void k_dropin_snapshot(SnapStoreArgs a)
+{
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y;
+    if (i > a.NX) return;
+    const long c = (long)j * a.P + i, t = c * a.nd + a.k;
+#pragma unroll
+    for (int f = 0; f < 5; f++) a.dst[f][t] = a.src[f][c];
+}
+
+struct CorrStoreArgs {
+    int NX, NY, P, nd, k;
+    const float *vxp1, *vyp1, *sxx, *syy, *sxy, *rho, *vp, *vs;
+    const float *F[5];     // forward_prop_rho_x, _rho_y, _x, _y, _u
+    float *wc, *wc_u, *wc_rho;
+};
+
+__global__ void k_dropin_correlate(CorrStoreArgs a)
+{
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y;
+    if (i > a.NX) return;
+    const long c = (long)j * a.P + i, t = c * a.nd + a.k;
+    const float rh = a.rho[c], u = a.vs[c], pv = a.vp[c];
+    // muss, lamss as main forms them (src/IFOS2D.c:2395-2396)
+    const float muss = rh * u * u;
+    const float lamss = (float)((double)(rh * pv * pv) - 2.0 * (double)muss);
+    float wr = a.wc_rho[c], wl = a.wc[c], wu = a.wc_u[c];
+    correlate_cell(a.vxp1[c], a.vyp1[c], a.sxx[c], a.syy[c], a.sxy[c], a.F[0][t], a.F[1][t], a.F[2][t], a.F[3][t],
+                   a.F[4][t], muss, lamss, u > 0.0f, wr, wl, wu);
+    a.wc_rho[c] = wr;
+    a.wc[c] = wl;
+    a.wc_u[c] = wu;
+}
+
+struct AssembleStoreArgs {
+    int NX, NY, P;
+    float DT;
+    const float *rho, *vp, *vs;
+    float *wc, *wc_u, *wc_rho;
+    float *lam, *mu, *rho_s;  // may be null
+};
+
+__global__ void k_dropin_assemble(AssembleStoreArgs a)
+{
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y;
+    if (i > a.NX) return;
+    const long c = (long)j * a.P + i;
+    float svp, svs, srho, wl, wmu, wrs;
+    chain_rule_cell(a.DT, a.rho[c], a.vs[c], a.vp[c], a.wc[c], a.wc_u[c], a.wc_rho[c], svp, svs, srho, wl, wmu, wrs);
+    a.wc[c] = svp;
+    a.wc_u[c] = svs;
+    a.wc_rho[c] = srho;
+    if (a.lam) a.lam[c] = wl;
+    if (a.mu) a.mu[c] = wmu;
+    if (a.rho_s) a.rho_s[c] = wrs;
+}
+
+void check_loop_domain(const char *who)
+{
+    if (NPROCX != 1 || NPROCY != 1) fatal("one GPU holds the whole domain: NPROCX = NPROCY = 1 required");
+    if (NX < 1 || NY < 1) {
+        char buf[128];
+        snprintf(buf, sizeof buf, "%s: NX and NY must be set", who);
+        fatal(buf);
+    }
+}
+
+}  // namespace
+
+extern "C" void ifos_snapshot_store(int hin, float **pvxp1, float **pvyp1, float **psxx, float **psyy, float **psxy,
+                                    float **ux, float **uy, float **uxy, float ***forward_prop_rho_x,
+                                    float ***forward_prop_rho_y, float ***forward_prop_x, float ***forward_prop_y,
+                                    float ***forward_prop_u)
+{
+    check_loop_domain("ifos_snapshot_store");
+    int P = 0;
+    SnapStoreArgs a;
+    a.src[0] = mat_of(pvxp1, P, "pvxp1").o;
+    a.src[1] = mat_of(pvyp1, P, "pvyp1").o;
+    if (VELOCITY == 0) {
+        a.src[2] = mat_of(psxx, P, "psxx").o;
+        a.src[3] = mat_of(psyy, P, "psyy").o;
+        a.src[4] = mat_of(psxy, P, "psxy").o;
+    } else {
+        a.src[2] = mat_of(ux, P, "ux").o;
+        a.src[3] = mat_of(uy, P, "uy").o;
+        a.src[4] = mat_of(uxy, P, "uxy").o;
+    }
+    float ***t[5] = {forward_prop_rho_x, forward_prop_rho_y, forward_prop_x, forward_prop_y, forward_prop_u};
+    const char *tn[5] = {"forward_prop_rho_x", "forward_prop_rho_y", "forward_prop_x", "forward_prop_y",
+                         "forward_prop_u"};
+    int nd = 0;
+    for (int f = 0; f < 5; f++) {
+        nd = tensor_origin(t[f], &a.dst[f], P, tn[f]);
+        check_covers(t[f], tn[f]);
+    }
+    if (hin < 1 || hin > nd) fatal("ifos_snapshot_store: slot outside the snapshot tensors");
+    a.NX = NX; a.NY = NY; a.P = P; a.nd = nd; a.k = hin - 1;
+    hipLaunchKernelGGL(k_dropin_snapshot, dim3((NX + 255) / 256, NY), dim3(256), 0, stream(), a);
+}
+
+extern "C" void ifos_correlate(int k, float **pvxp1, float **pvyp1, float **psxx, float **psyy, float **psxy,
+                               float **prho, float **ppi, float **pu, float ***forward_prop_rho_x,
+                               float ***forward_prop_rho_y, float ***forward_prop_x, float ***forward_prop_y,
+                               float ***forward_prop_u, float **waveconv_shot, float **waveconv_u_shot,
+                               float **waveconv_rho_shot)
+{
+    check_loop_domain("ifos_correlate");
+    int P = 0;
+    CorrStoreArgs a;
+    a.vxp1 = mat_of(pvxp1, P, "pvxp1").o;
+    a.vyp1 = mat_of(pvyp1, P, "pvyp1").o;
+    a.sxx = mat_of(psxx, P, "psxx").o;
+    a.syy = mat_of(psyy, P, "psyy").o;
+    a.sxy = mat_of(psxy, P, "psxy").o;
+    a.rho = mat_of(prho, P, "prho").o;
+    a.vp = mat_of(ppi, P, "ppi").o;
+    a.vs = mat_of(pu, P, "pu").o;
+    a.wc = mat_of(waveconv_shot, P, "waveconv_shot").o;
+    a.wc_u = mat_of(waveconv_u_shot, P, "waveconv_u_shot").o;
+    a.wc_rho = mat_of(waveconv_rho_shot, P, "waveconv_rho_shot").o;
+    float ***t[5] = {forward_prop_rho_x, forward_prop_rho_y, forward_prop_x, forward_prop_y, forward_prop_u};
+    const char *tn[5] = {"forward_prop_rho_x", "forward_prop_rho_y", "forward_prop_x", "forward_prop_y",
+                         "forward_prop_u"};
+    int nd = 0;
+    for (int f = 0; f < 5; f++) {
+        float *o;
+        nd = tensor_origin(t[f], &o, P, tn[f]);
+        check_covers(t[f], tn[f]);
+        a.F[f] = o;
+    }
+    if (k < 1 || k > nd) fatal("ifos_correlate: slot outside the snapshot tensors");
+    a.NX = NX; a.NY = NY; a.P = P; a.nd = nd; a.k = k - 1;
+    hipLaunchKernelGGL(k_dropin_correlate, dim3((NX + 255) / 256, NY), dim3(256), 0, stream(), a);
+}
+
+extern "C" void ifos_gradient_assemble(float **prho, float **ppi, float **pu, float **waveconv_shot,
+                                       float **waveconv_u_shot, float **waveconv_rho_shot, float **waveconv_lam,
+                                       float **waveconv_mu, float **waveconv_rho_s)
+{
+    check_loop_domain("ifos_gradient_assemble");
+    if (PARAMETERIZATION != 1) fatal("ifos_gradient_assemble: PARAMETERIZATION 1 only");
+    int P = 0;
+    AssembleStoreArgs a;
+    a.rho = mat_of(prho, P, "prho").o;
+    a.vp = mat_of(ppi, P, "ppi").o;
+    a.vs = mat_of(pu, P, "pu").o;
+    a.wc = mat_of(waveconv_shot, P, "waveconv_shot").o;
+    a.wc_u = mat_of(waveconv_u_shot, P, "waveconv_u_shot").o;
+    a.wc_rho = mat_of(waveconv_rho_shot, P, "waveconv_rho_shot").o;
+    a.lam = waveconv_lam ? mat_of(waveconv_lam, P, "waveconv_lam").o : nullptr;
+    a.mu = waveconv_mu ? mat_of(waveconv_mu, P, "waveconv_mu").o : nullptr;
+    a.rho_s = waveconv_rho_s ? mat_of(waveconv_rho_s, P, "waveconv_rho_s").o : nullptr;
+    a.NX = NX; a.NY = NY; a.P = P; a.DT = DT;
+    hipLaunchKernelGGL(k_dropin_assemble, dim3((NX + 255) / 256, NY), dim3(256), 0, stream(), a);
 }

@@ -236,4 +236,24 @@ void launch_correlate(const CorrArgs &a, hipStream_t s);
 void launch_gradient(const GradArgs &a, hipStream_t s);
 void launch_residual(const ResArgs &a, hipStream_t s);
 
+// chain rule of one cell (src/IFOS2D.c:2471-2611, PARAMETERIZATION 1): the
+// correlations wc (lambda), wc_u (mu) and wc_rho of a shot become the Vp, Vs
+// and rho gradients; wl, wmu, wrs are the intermediate lambda', mu' and
+// rho' (waveconv_lam, waveconv_mu, waveconv_rho_s)
+__device__ __forceinline__ void chain_rule_cell(float DT, float rh, float u, float pv, float wc, float wcu, float wcr,
+                                                float &svp, float &svs, float &srho, float &wl, float &wmu,
+                                                float &wrs)
+{
+    const float muss = rh * u * u;
+    const float lamss = (float)((double)(rh * pv * pv) - 2.0 * (double)muss);
+    wl = -DT * wc;
+    wl = (float)((1.0 / (4.0 * (double)(lamss + muss) * (double)(lamss + muss))) * (double)wl);
+    svp = (float)(2.0 * (double)pv * (double)rh * (double)wl);
+    wmu = -DT * wcu;
+    svs = (float)(-4.0 * (double)rh * (double)u * (double)wl + 2.0 * (double)rh * (double)u * (double)wmu);
+    wrs = -DT * wcr;
+    srho = (float)(((double)(pv * pv) - (2.0 * (double)u * (double)u)) * (double)wl + (double)(u * u * wmu) +
+                   (double)wrs);
+}
+
 }  // namespace ifos

@@ -543,8 +543,6 @@ IFOS_KERNEL void k_gradient(GradArgs a)
     if (e >= a.n) return;
     const float DT = a.DT;
     const float rh = a.rho[e], u = a.vs[e], pv = a.vp[e];
-    const float muss = rh * u * u;
-    const float lamss = (float)((double)(rh * pv * pv) - 2.0 * (double)muss);
     float gvp = 0.0f, gvs = 0.0f, grho = 0.0f;
     float tvp = 0.0f, tvs = 0.0f, trho = 0.0f;
     const int ci = a.tap_m ? (int)(e % a.NX) + 1 : 0, cj = a.tap_m ? (int)(e / a.NX) + 1 : 0;
@@ -557,15 +555,8 @@ IFOS_KERNEL void k_gradient(GradArgs a)
     // adds them (src/IFOS2D.c:2703-2713)
     for (int s = 0; s < a.nslots; s++) {
         const long o = (long)s * a.wcs + e;
-        float wl = -DT * a.wc[o];
-        wl = (float)((1.0 / (4.0 * (double)(lamss + muss) * (double)(lamss + muss))) * (double)wl);
-        float svp = (float)(2.0 * (double)pv * (double)rh * (double)wl);
-        const float wmu = -DT * a.wc_u[o];
-        float svs =
-            (float)(-4.0 * (double)rh * (double)u * (double)wl + 2.0 * (double)rh * (double)u * (double)wmu);
-        const float wrs = -DT * a.wc_rho[o];
-        float srho = (float)(((double)(pv * pv) - (2.0 * (double)u * (double)u)) * (double)wl +
-                             (double)(u * u * wmu) + (double)wrs);
+        float svp, svs, srho, wl, wmu, wrs;
+        chain_rule_cell(DT, rh, u, pv, a.wc[o], a.wc_u[o], a.wc_rho[o], svp, svs, srho, wl, wmu, wrs);
         // energy preconditioning of the shot (src/IFOS2D.c:2627-2654)
         if (a.we) {
             const float We = a.we[(long)s * a.wes + e];

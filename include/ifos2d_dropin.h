@@ -99,6 +99,33 @@ void zero_fdveps_visc(int ny1, int ny2, int nx1, int nx2, float **vx, float **vy
                       float **psi_syz_y, float **psi_vyy, float **psi_vxy, float **psi_vzy, float **psi_vxxs,
                       float ***pr, float ***pp, float ***pq, float ***pt, float ***po);
 
+/* The loops main runs inline, promoted to entry points on device arrays
+ * (WAVETYPE 1, not ACOUSTIC, IDXI = IDYI = IDX = IDY = 1).  The snapshot
+ * tensors are f3tensor(-nd+1,NY+nd,-nd+1,NX+nd,1,NT/DTINV) from
+ * ifos_dev_f3tensor(), time fastest as in the reference. */
+
+/* replaces the snapshot store of src/IFOS2D.c:1805-1871 (slot hin): vxp1,
+ * vyp1 and, with VELOCITY 0, sxx, syy, sxy, else ux, uy, uxy */
+void ifos_snapshot_store(int hin, float **pvxp1, float **pvyp1, float **psxx, float **psyy, float **psxy,
+                         float **ux, float **uy, float **uxy, float ***forward_prop_rho_x,
+                         float ***forward_prop_rho_y, float ***forward_prop_x, float ***forward_prop_y,
+                         float ***forward_prop_u);
+
+/* replaces the zero-lag correlation of src/IFOS2D.c:2381-2419 with snapshot
+ * slot k = NTDTINV-hin+1, accumulated into the waveconv_*_shot arrays */
+void ifos_correlate(int k, float **pvxp1, float **pvyp1, float **psxx, float **psyy, float **psxy, float **prho,
+                    float **ppi, float **pu, float ***forward_prop_rho_x, float ***forward_prop_rho_y,
+                    float ***forward_prop_x, float ***forward_prop_y, float ***forward_prop_u,
+                    float **waveconv_shot, float **waveconv_u_shot, float **waveconv_rho_shot);
+
+/* replaces the chain rule of src/IFOS2D.c:2471-2611 (PARAMETERIZATION 1):
+ * waveconv_shot, waveconv_u_shot and waveconv_rho_shot become the Vp, Vs
+ * and rho gradients of the shot; waveconv_lam, waveconv_mu and
+ * waveconv_rho_s (each may be NULL) receive lambda', mu' and rho' */
+void ifos_gradient_assemble(float **prho, float **ppi, float **pu, float **waveconv_shot, float **waveconv_u_shot,
+                            float **waveconv_rho_shot, float **waveconv_lam, float **waveconv_mu,
+                            float **waveconv_rho_s);
+
 #ifdef __cplusplus
 }
 #endif
