@@ -158,6 +158,57 @@ def serving(args):
     return 0
 
 
+def dropin(args):
+    """One shot gradient of the workload through the drop-in layer
+    (include/ifos2d_dropin.h), made the way the reference's shot loop makes
+    its calls (ifos_amd/dropin.py): forward with the snapshot store, the
+    residual on the host as the reference's calc_res forms it (LNORM 2,
+    VELOCITY 1: the time-reversed difference), adjoint with the zero-lag
+    correlation, chain rule.  Observed traces come from an untimed forward
+    run on the true model through the same layer."""
+    import ifos_amd
+    from ifos_amd.dropin import DropinShot
+    w = build_workload(args.workload)
+    cfg = w.cfg
+    if cfg.VELOCITY != 1:
+        raise SystemExit("--dropin times VELOCITY 1 workloads (the residual is formed on the host here)")
+    lib = ifos_amd.lib()
+    shot = w.shots[0]
+    src = np.array(shot.srcpos, np.float32).reshape(-1, 8)
+    rec = np.array(shot.recpos, np.int32).reshape(-1, 2)
+    log(f"[dropin] workload {w.name}: {cfg.NX}x{cfg.NY} L={cfg.L} FDORDER={cfg.FDORDER} NT={cfg.NT}")
+    run = DropinShot(lib, cfg, w.true)
+    ovx, ovy = run.forward(src, rec, snapshots=False)
+    run.close()
+    ovx[:, 0] = 0.0
+    ovy[:, 0] = 0.0
+    log("[dropin] observed data done")
+    run = DropinShot(lib, cfg, w.start)
+    t0 = time.perf_counter()
+    vx, vy = run.forward(src, rec, snapshots=True)
+    t1 = time.perf_counter()
+    log(f"[dropin] forward with snapshots {t1 - t0:.3f} s")
+    dvx = np.ascontiguousarray((vx - ovx)[:, ::-1])
+    dvy = np.ascontiguousarray((vy - ovy)[:, ::-1])
+    run.adjoint_gradient(rec, dvx, dvy)
+    t2 = time.perf_counter()
+    log(f"[dropin] adjoint and gradient {t2 - t1:.3f} s")
+    run.close()
+    steps = 2 * cfg.NT
+    out = {"metric": "shot-gradients/s through the drop-in layer (reference call sequence)",
+           "value": round(1.0 / (t2 - t0), 4), "unit": "shot-gradients/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": round((t2 - t0) * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic (seeded layered model with faults; observed traces from the true model)",
+           "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "FDORDER": cfg.FDORDER, "L": cfg.L,
+                      "NT": cfg.NT, "DTINV": cfg.DTINV, "path": "drop-in entry points driven from Python ctypes, "
+                      "one call per reference function and time step"},
+           "forward_s": round(t1 - t0, 3), "adjoint_gradient_s": round(t2 - t1, 3),
+           "us_per_time_step": round((t2 - t0) / steps * 1e6, 2)}
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +231,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of this many MB (counter calibration)")
     ap.add_argument("--serve", action="store_true", help="serving trace (Poisson arrivals) on the request server")
+    ap.add_argument("--dropin", action="store_true",
+                    help="one shot gradient through the drop-in layer (the reference's call sequence)")
     ap.add_argument("--rate", type=float, default=4.0, help="serving: requests per second")
     ap.add_argument("--duration", type=float, default=15.0, help="serving: seconds of arrivals")
     ap.add_argument("--max-nx", type=int, default=1200)
@@ -200,6 +253,8 @@ def main():
 
     if args.serve:
         return serving(args)
+    if args.dropin:
+        return dropin(args)
     import ifos_amd
     w = build_workload(args.workload)
     cfg = w.cfg

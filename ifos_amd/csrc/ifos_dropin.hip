@@ -25,6 +25,7 @@
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "host/ifos_setup.h"
@@ -40,6 +41,8 @@ void set_step_control(const int *nt_dev, int DTINV, int NT);
 void launch_inject_serial(const PointArgs &a, const StepGeom &g, float *vx, float *vy, const float *rip,
                           const float *rjp, int sw, int ADJOINT_TYPE, const int *nt_dev, int nt, int *nt_tick,
                           hipStream_t s);
+void launch_inject(const PointArgs &a, const StepGeom &g, float *vx, float *vy, const float *rip, const float *rjp,
+                   int sw, int ADJOINT_TYPE, const int *nt_dev, int nt, int *nt_tick, hipStream_t s);
 }
 extern "C" void ifos_set_error(const char *fmt, ...);
 
@@ -153,6 +156,8 @@ struct Stage {
     void *get(size_t n)
     {
         if (n > cap) {
+            // earlier calls' kernels may still read the old buffer
+            hip_check(hipStreamSynchronize(stream()), "stage sync");
             if (dev) hip_check(hipFree(dev), "hipFree");
             cap = n * 2 + 256;
             hip_check(hipMalloc(&dev, cap), "hipMalloc(stage)");
@@ -161,6 +166,35 @@ struct Stage {
     }
 };
 Stage g_stage[4];
+
+// Pinned host staging of the small per-call uploads (source positions and
+// samples): a ring of slots, each reused only after the copy that last read
+// it has completed, so a call returns without waiting for the device.
+struct PinnedRing {
+    static constexpr int kSlots = 16;
+    char *host[kSlots] = {};
+    size_t cap[kSlots] = {};
+    hipEvent_t ev[kSlots] = {};
+    int next = 0;
+    char *get(size_t n, int &slot)
+    {
+        slot = next;
+        next = (next + 1) % kSlots;
+        if (ev[slot])
+            hip_check(hipEventSynchronize(ev[slot]), "staging wait");
+        else
+            hip_check(hipEventCreateWithFlags(&ev[slot], hipEventDisableTiming), "staging event");
+        if (cap[slot] < n) {
+            if (host[slot]) hip_check(hipHostFree(host[slot]), "hipHostFree");
+            cap[slot] = n * 2 + 256;
+            hip_check(hipHostMalloc((void **)&host[slot], cap[slot]), "hipHostMalloc(staging)");
+        }
+        return host[slot];
+    }
+    // after the copy from slot has been queued
+    void queued(int slot) { hip_check(hipEventRecord(ev[slot], stream()), "staging record"); }
+};
+PinnedRing g_ring;
 
 StepGeom make_geom(int nx1, int nx2, int ny1, int ny2, int pitch, const float *hc, const float *K_x,
                    const float *a_x, const float *b_x, const float *K_xh, const float *a_xh, const float *b_xh,
@@ -294,9 +328,31 @@ extern "C" int ifos_dev_copy_array(void *dev_arr, const void *host_first_elem, i
 {
     const DevArr *a = lookup(dev_arr);
     if (!a) return ifos_set_error("not a device array"), -2;
-    hipError_t e = dir == 0 ? hipMemcpyAsync(a->dev, host_first_elem, a->bytes, hipMemcpyHostToDevice, stream())
-                            : hipMemcpyAsync((void *)host_first_elem, a->dev, a->bytes, hipMemcpyDeviceToHost, stream());
-    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    hipError_t e = hipSuccess;
+    if (a->kind != 3) {
+        e = dir == 0 ? hipMemcpyAsync(a->dev, host_first_elem, a->bytes, hipMemcpyHostToDevice, stream())
+                     : hipMemcpyAsync((void *)host_first_elem, a->dev, a->bytes, hipMemcpyDeviceToHost, stream());
+        if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    } else {
+        // the host tensor is util.c's contiguous [j][i][l] block; the device
+        // holds it plane by plane: transpose one plane at a time
+        const long plane = (long)(a->nrh - a->nrl + 1) * (a->nch - a->ncl + 1);
+        const int nd = a->ndh - a->ndl + 1;
+        std::vector<float> buf(plane);
+        float *h = (float *)host_first_elem;
+        for (int l = 0; l < nd && e == hipSuccess; l++) {
+            float *d = (float *)a->dev + (long)l * plane;
+            if (dir == 0) {
+                for (long c = 0; c < plane; c++) buf[c] = h[c * nd + l];
+                e = hipMemcpyAsync(d, buf.data(), sizeof(float) * plane, hipMemcpyHostToDevice, stream());
+                if (e == hipSuccess) e = hipStreamSynchronize(stream());
+            } else {
+                e = hipMemcpyAsync(buf.data(), d, sizeof(float) * plane, hipMemcpyDeviceToHost, stream());
+                if (e == hipSuccess) e = hipStreamSynchronize(stream());
+                for (long c = 0; c < plane; c++) h[c * nd + l] = buf[c];
+            }
+        }
+    }
     if (e != hipSuccess) return ifos_set_error("copy failed: %s", hipGetErrorString(e)), -1;
     return 0;
 }
@@ -383,14 +439,16 @@ extern "C" void update_v_PML(int nx1, int nx2, int ny1, int ny2, int nt, float *
         s[nsrc + l - 1] = signals1[l][nt];
     }
     if (!any) return;
-    char *st = (char *)g_stage[0].get(sizeof(int) * pos.size() + sizeof(double) * sc.size() + sizeof(float) * s.size());
-    std::vector<char> blob(sizeof(int) * pos.size() + sizeof(double) * sc.size() + sizeof(float) * s.size());
-    size_t o1 = sizeof(int) * pos.size(), o2 = o1 + sizeof(double) * sc.size();
-    memcpy(blob.data(), pos.data(), o1);
-    memcpy(blob.data() + o1, sc.data(), sizeof(double) * sc.size());
-    memcpy(blob.data() + o2, s.data(), sizeof(float) * s.size());
-    hip_check(hipMemcpyAsync(st, blob.data(), blob.size(), hipMemcpyHostToDevice, stream()), "stage sources");
-    hip_check(hipStreamSynchronize(stream()), "stage sync");
+    const size_t o1 = sizeof(int) * pos.size(), o2 = o1 + sizeof(double) * sc.size();
+    const size_t nb = o2 + sizeof(float) * s.size();
+    char *st = (char *)g_stage[0].get(nb);
+    int slot;
+    char *blob = g_ring.get(nb, slot);
+    memcpy(blob, pos.data(), o1);
+    memcpy(blob + o1, sc.data(), sizeof(double) * sc.size());
+    memcpy(blob + o2, s.data(), sizeof(float) * s.size());
+    hip_check(hipMemcpyAsync(st, blob, nb, hipMemcpyHostToDevice, stream()), "stage sources");
+    g_ring.queued(slot);
     PointArgs pa;
     pa.n = nsrc;
     pa.ix = (const int *)st;
@@ -401,8 +459,18 @@ extern "C" void update_v_PML(int nx1, int nx2, int ny1, int ny2, int nt, float *
     pa.sig = (const float *)(st + o2);
     pa.sig1 = (const float *)(st + o2) + nsrc;
     pa.ld = 1;  // one sample per source: column nt-1 with nt = 1
-    // serial in source order, as the reference loop (positions may repeat)
-    launch_inject_serial(pa, a.g, a.vx, a.vy, a.rip, a.rjp, sw, ADJOINT_TYPE, nullptr, 1, nullptr, stream());
+    // in source order, as the reference loop: serially where positions may
+    // repeat; adjoint sources at distinct cells are added in parallel (each
+    // cell then receives one addition, so the order does not matter)
+    bool distinct = sw == 1;
+    if (distinct) {
+        std::unordered_set<long> seen;
+        for (int l = 0; l < nsrc && distinct; l++) distinct = seen.insert((long)pos[nsrc + l] * P + pos[l]).second;
+    }
+    if (distinct)
+        launch_inject(pa, a.g, a.vx, a.vy, a.rip, a.rjp, sw, ADJOINT_TYPE, nullptr, 1, nullptr, stream());
+    else
+        launch_inject_serial(pa, a.g, a.vx, a.vy, a.rip, a.rjp, sw, ADJOINT_TYPE, nullptr, 1, nullptr, stream());
 }
 
 // ---------------------------------------------------------------------------
@@ -447,7 +515,9 @@ extern "C" void update_s_elastic_PML(int nx1, int nx2, int ny1, int ny2, float *
     launch_update_s_elastic(a, stream());
 }
 
-static int tensor_origin(float ***t, float **origin, int pitch, const char *name)
+// Device tensors are stored plane by plane (third index slowest): element
+// (j, i, l) at origin + j * pitch + i + (l - 1) * plane.  Returns the depth.
+static int tensor_origin(float ***t, float **origin, long *plane, int pitch, const char *name)
 {
     const DevArr *a = lookup(t);
     if (!a || a->kind != 3) {
@@ -456,8 +526,10 @@ static int tensor_origin(float ***t, float **origin, int pitch, const char *name
         fatal(buf);
     }
     const int nc = a->nch - a->ncl + 1, nd = a->ndh - a->ndl + 1;
+    const long pl = (long)(a->nrh - a->nrl + 1) * nc;
     if (nc != pitch) fatal("memory-variable tensors must share the wavefield grid bounds");
-    *origin = (float *)a->dev - ((long)a->nrl * nc + a->ncl) * nd - (a->ndl - 1);
+    *origin = (float *)a->dev - ((long)a->nrl * nc + a->ncl) + (long)(1 - a->ndl) * pl;
+    *plane = pl;
     return nd;
 }
 
@@ -491,14 +563,17 @@ extern "C" void update_s_visc_PML(int nx1, int nx2, int ny1, int ny2, float **vx
     a.f = origin_of(f, &p2, "f");
     a.gv = origin_of(g, &p2, "g");
     float *o;
-    int nd = tensor_origin(r, &o, P, "r"); a.r = o;
-    tensor_origin(p, &o, P, "p"); a.p = o;
-    tensor_origin(q, &o, P, "q"); a.q = o;
-    tensor_origin(d, &o, P, "d"); a.d = o;
-    tensor_origin(e, &o, P, "e"); a.e = o;
-    tensor_origin(dip, &o, P, "dip"); a.dip = o;
-    a.cs = nd;
-    a.ls = 1;
+    long pl = 0;
+    float ***ts[6] = {r, p, q, d, e, dip};
+    const char *tn[6] = {"r", "p", "q", "d", "e", "dip"};
+    float *org[6];
+    for (int k = 0; k < 6; k++) {
+        const int nd = tensor_origin(ts[k], &org[k], &pl, P, tn[k]);
+        if (nd < L) fatal("update_s_visc_PML: memory-variable tensors need L planes");
+    }
+    a.r = org[0]; a.p = org[1]; a.q = org[2]; a.d = org[3]; a.e = org[4]; a.dip = org[5];
+    a.cs = 1;
+    a.ls = (int)pl;
     for (int l = 0; l <= kMaxL; l++) {
         a.bip[l] = (l >= 1 && l <= L) ? bip[l] : 0.0f;
         a.bjm[l] = (l >= 1 && l <= L) ? bjm[l] : 0.0f;
@@ -545,12 +620,14 @@ extern "C" void psource(int nt, float **sxx, float **syy, float **sp, float **sr
         }
         amp[l - 1] = a;
     }
-    char *st = (char *)g_stage[1].get(sizeof(int) * pos.size() + sizeof(double) * amp.size());
-    std::vector<char> blob(sizeof(int) * pos.size() + sizeof(double) * amp.size());
-    memcpy(blob.data(), pos.data(), sizeof(int) * pos.size());
-    memcpy(blob.data() + sizeof(int) * pos.size(), amp.data(), sizeof(double) * amp.size());
-    hip_check(hipMemcpyAsync(st, blob.data(), blob.size(), hipMemcpyHostToDevice, stream()), "stage psource");
-    hip_check(hipStreamSynchronize(stream()), "stage sync");
+    const size_t nb = sizeof(int) * pos.size() + sizeof(double) * amp.size();
+    char *st = (char *)g_stage[1].get(nb);
+    int slot;
+    char *blob = g_ring.get(nb, slot);
+    memcpy(blob, pos.data(), sizeof(int) * pos.size());
+    memcpy(blob + sizeof(int) * pos.size(), amp.data(), sizeof(double) * amp.size());
+    hip_check(hipMemcpyAsync(st, blob, nb, hipMemcpyHostToDevice, stream()), "stage psource");
+    g_ring.queued(slot);
     hipLaunchKernelGGL(k_point_add, dim3(1), dim3(64), 0, stream(), nsrc, (const int *)st, (const int *)st + nsrc,
                        (const double *)(st + sizeof(int) * pos.size()), P, oxx, oyy);
 }
@@ -628,12 +705,13 @@ extern "C" void surface_PML(int ndepth, float **vx, float **vy, float **sxx, flo
     po.taus = origin_of(ptaus, &p2, "ptaus");
     po.taup = origin_of(ptaup, &p2, "ptaup");
     float *o;
-    int nd = tensor_origin(p, &o, po.g.pitch, "p");
+    long pl = 0;
+    if (tensor_origin(p, &o, &pl, po.g.pitch, "p") < L) fatal("surface_PML: p needs L planes");
     po.p = o;
-    tensor_origin(q, &o, po.g.pitch, "q");
+    if (tensor_origin(q, &o, &pl, po.g.pitch, "q") < L) fatal("surface_PML: q needs L planes");
     po.q = o;
-    po.cs = nd;
-    po.ls = 1;
+    po.cs = 1;
+    po.ls = (int)pl;
     // the relaxed-modulus sums of src/surface_PML.c:46-60
     float pts[kMaxL + 1];
     for (int l = 1; l <= L; l++) pts[l] = 1.0 / (2.0 * 3.141592653589793 * FL[l]);
@@ -771,22 +849,24 @@ Mat mat_of(float **m, int &P, const char *name)
 }
 
 struct SnapStoreArgs {
-    int NX, NY, P, nd, k;  // k: 0-based time slot
+    int NX, NY, P, k;  // k: 0-based time slot
+    long plane;
     const float *src[5];
-    float *dst[5];         // tensor origins: element (j, i, l) at dst + (j * P + i) * nd + l - 1
+    float *dst[5];     // tensor origins: element (j, i, l) at dst + j * P + i + (l - 1) * plane
 };
 
 __global__ void k_dropin_snapshot(SnapStoreArgs a)
 {
     const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y;
     if (i > a.NX) return;
-    const long c = (long)j * a.P + i, t = c * a.nd + a.k;
+    const long c = (long)j * a.P + i, t = c + (long)a.k * a.plane;
 #pragma unroll
     for (int f = 0; f < 5; f++) a.dst[f][t] = a.src[f][c];
 }
 
 struct CorrStoreArgs {
-    int NX, NY, P, nd, k;
+    int NX, NY, P, k;
+    long plane;
     const float *vxp1, *vyp1, *sxx, *syy, *sxy, *rho, *vp, *vs;
     const float *F[5];     // forward_prop_rho_x, _rho_y, _x, _y, _u
     float *wc, *wc_u, *wc_rho;
@@ -796,7 +876,7 @@ __global__ void k_dropin_correlate(CorrStoreArgs a)
 {
     const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y;
     if (i > a.NX) return;
-    const long c = (long)j * a.P + i, t = c * a.nd + a.k;
+    const long c = (long)j * a.P + i, t = c + (long)a.k * a.plane;
     const float rh = a.rho[c], u = a.vs[c], pv = a.vp[c];
     // muss, lamss as main forms them (src/IFOS2D.c:2395-2396)
     const float muss = rh * u * u;
@@ -866,13 +946,12 @@ extern "C" void ifos_snapshot_store(int hin, float **pvxp1, float **pvyp1, float
     float ***t[5] = {forward_prop_rho_x, forward_prop_rho_y, forward_prop_x, forward_prop_y, forward_prop_u};
     const char *tn[5] = {"forward_prop_rho_x", "forward_prop_rho_y", "forward_prop_x", "forward_prop_y",
                          "forward_prop_u"};
-    int nd = 0;
     for (int f = 0; f < 5; f++) {
-        nd = tensor_origin(t[f], &a.dst[f], P, tn[f]);
+        const int nd = tensor_origin(t[f], &a.dst[f], &a.plane, P, tn[f]);
         check_covers(t[f], tn[f]);
+        if (hin < 1 || hin > nd) fatal("ifos_snapshot_store: slot outside the snapshot tensors");
     }
-    if (hin < 1 || hin > nd) fatal("ifos_snapshot_store: slot outside the snapshot tensors");
-    a.NX = NX; a.NY = NY; a.P = P; a.nd = nd; a.k = hin - 1;
+    a.NX = NX; a.NY = NY; a.P = P; a.k = hin - 1;
     hipLaunchKernelGGL(k_dropin_snapshot, dim3((NX + 255) / 256, NY), dim3(256), 0, stream(), a);
 }
 
@@ -899,15 +978,14 @@ extern "C" void ifos_correlate(int k, float **pvxp1, float **pvyp1, float **psxx
     float ***t[5] = {forward_prop_rho_x, forward_prop_rho_y, forward_prop_x, forward_prop_y, forward_prop_u};
     const char *tn[5] = {"forward_prop_rho_x", "forward_prop_rho_y", "forward_prop_x", "forward_prop_y",
                          "forward_prop_u"};
-    int nd = 0;
     for (int f = 0; f < 5; f++) {
         float *o;
-        nd = tensor_origin(t[f], &o, P, tn[f]);
+        const int nd = tensor_origin(t[f], &o, &a.plane, P, tn[f]);
         check_covers(t[f], tn[f]);
+        if (k < 1 || k > nd) fatal("ifos_correlate: slot outside the snapshot tensors");
         a.F[f] = o;
     }
-    if (k < 1 || k > nd) fatal("ifos_correlate: slot outside the snapshot tensors");
-    a.NX = NX; a.NY = NY; a.P = P; a.nd = nd; a.k = k - 1;
+    a.NX = NX; a.NY = NY; a.P = P; a.k = k - 1;
     hipLaunchKernelGGL(k_dropin_correlate, dim3((NX + 255) / 256, NY), dim3(256), 0, stream(), a);
 }
 

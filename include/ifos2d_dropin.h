@@ -10,7 +10,10 @@
  * seismogram sections stay host arrays as in the reference.  Calls are queued
  * in order on one device stream; call ifos_dev_sync() before the host reads
  * device data and use ifos_dev_read() for single values (the stability probe
- * of src/IFOS2D.c:1660).
+ * of src/IFOS2D.c:1660).  A device tensor keeps its elements plane by plane
+ * (third index slowest), so every kernel reads and writes whole planes; the
+ * float*** it returns is a handle for these calls, not for host indexing,
+ * and ifos_dev_copy_array() converts from and to util.c's [j][i][l] block.
  *
  * Globals read by these functions, by the reference names (src/globvar.h):
  * DT, DH, NX, NY, NT, FDORDER, FW, FREE_SURF, BOUNDARY, L, FL, F_REF,
