@@ -209,6 +209,52 @@ def dropin(args):
     return 0
 
 
+def fwi_iteration(args):
+    """One FWI iteration of the workload (ifos_amd/fwi.py, in the order of
+    src/IFOS2D.c:1076-3876): gradient over every shot, PCG direction on the
+    device, the step-length search with its test forwards and the parabola,
+    the bounded model update.  Observed traces come from untimed forward
+    runs on the true model.  Each phase ends with a value copied back to the
+    host, so the phase times are wall-clock times of finished device work."""
+    from ifos_amd.fwi import FWI, FWISettings
+    w = build_workload(args.workload)
+    cfg = w.cfg
+    s = FWISettings(TESTSHOT_START=1, TESTSHOT_END=1)
+    f = FWI(w, s, slots=max(1, args.batch), snap=args.snap)
+    f.observe()
+    log(f"[fwi] {w.name}: {cfg.NX}x{cfg.NY} NT={cfg.NT}, {len(w.shots)} shots, observed data done")
+    times = []
+    for it in range(1, args.warmup + args.steps + 1):
+        t0 = time.perf_counter()
+        grads, L2_1 = f.gradient(f.model)
+        t1 = time.perf_counter()
+        f.direction(grads, it)
+        t2 = time.perf_counter()
+        opteps, L2t, epst, forwards = f.step_search(L2_1, it)
+        t3 = time.perf_counter()
+        f.model, _ = f.trial(opteps, it, 0)
+        t4 = time.perf_counter()
+        log(f"[fwi] iteration {it}: gradient {t1 - t0:.3f} s, direction {1e3 * (t2 - t1):.2f} ms, step search "
+            f"{t3 - t2:.3f} s ({forwards} test forwards, step {opteps:.4g}), update {1e3 * (t4 - t3):.2f} ms, "
+            f"misfit {L2_1:.6e}")
+        if it > args.warmup:
+            times.append((t0, t1, t2, t3, t4, forwards, L2_1, opteps))
+    f.close()
+    tot = sum(t[4] - t[0] for t in times)
+    ph = lambda a, b: round(1e3 * sum(t[b] - t[a] for t in times) / len(times), 3)
+    out = {"metric": "FWI iterations/s (gradient, PCG direction, step search, update)",
+           "value": round(len(times) / tot, 4), "unit": "iterations/s", "n_gpus": 1, "steps": len(times),
+           "warmup": args.warmup, "ms_per_step": round(1e3 * tot / len(times), 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic (seeded model; observed traces from the true model)",
+           "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "FDORDER": cfg.FDORDER, "L": cfg.L, "NT": cfg.NT,
+                      "shots": len(w.shots), "snapshots": args.snap, "gradient": "PCG", "test_shots": 1},
+           "phase_ms": {"gradient": ph(0, 1), "direction": ph(1, 2), "step_search": ph(2, 3), "update": ph(3, 4)},
+           "test_forwards": [t[5] for t in times], "misfit": [t[6] for t in times], "step": [t[7] for t in times]}
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +277,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of this many MB (counter calibration)")
     ap.add_argument("--serve", action="store_true", help="serving trace (Poisson arrivals) on the request server")
+    ap.add_argument("--fwi", action="store_true",
+                    help="time whole FWI iterations of the workload (gradient, PCG, step search, update)")
     ap.add_argument("--dropin", action="store_true",
                     help="one shot gradient through the drop-in layer (the reference's call sequence)")
     ap.add_argument("--rate", type=float, default=4.0, help="serving: requests per second")
@@ -255,6 +303,8 @@ def main():
         return serving(args)
     if args.dropin:
         return dropin(args)
+    if args.fwi:
+        return fwi_iteration(args)
     import ifos_amd
     w = build_workload(args.workload)
     cfg = w.cfg
