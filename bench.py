@@ -255,6 +255,53 @@ def fwi_iteration(args):
     return 0
 
 
+def post_kernels(args):
+    """Device times of the gradient post-processing and trace filtering on
+    the workload's sizes: median smoothing (GRAD_FILTER, 5x5 window),
+    spatial filter (SPATFILTER, half-width 3), whole-gradient taper,
+    Butterworth time filter of one shot's traces (order 4, 10 Hz lowpass).
+    Each figure is the wall time of `reps` calls ended by a device read."""
+    import ifos_amd
+    from ifos_amd import opt
+    w = build_workload(args.workload)
+    cfg = w.cfg
+    NX, NY, reps = cfg.NX, cfg.NY, 20
+    ntr = len(w.shots[0].recpos)
+    rng = np.random.default_rng(1)
+    g = rng.standard_normal((NY, NX)).astype(np.float32)
+    o = opt.Optimizer()
+    buf = ifos_amd.DeviceBuffer.from_array(g)
+    coef, zero = opt.taper_coeff(opt.TAPER_HORIZONTAL, NX, NY, gradt=(10, 40, 0, 0))
+    tr = ifos_amd.DeviceBuffer.from_array(rng.standard_normal((ntr, cfg.NT)).astype(np.float32))
+    out = {}
+
+    def timed(name, fn, nbytes):
+        fn()
+        o.max(buf, NX * NY)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        o.max(buf, NX * NY)
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[name] = {"ms": round(ms, 4), "GB_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+        log(f"[post] {name}: {ms:.4f} ms per call")
+
+    cell = 4 * NX * NY
+    timed("smooth_5x5", lambda: o.smooth(NX, NY, 5, buf), 2 * cell)
+    timed("spat_filt_3", lambda: o.spat_filt(NX, NY, 3, 1, buf), 2 * cell)
+    timed("taper", lambda: o.taper(buf, coef, zero), 3 * cell)
+    timed("time_filter", lambda: ifos_amd.time_filter(tr, ntr, cfg.NT, cfg.DT, 10.0, 4), 2 * 4 * ntr * cfg.NT)
+    buf.free()
+    tr.free()
+    o.close()
+    res = {"metric": "gradient post-processing and trace filter times", "value": out["smooth_5x5"]["ms"],
+           "unit": "ms per smoothing", "n_gpus": 1, "higher_is_better": False, "dtype": "fp32",
+           "data": "synthetic (random gradient and traces)",
+           "config": {"workload": w.name, "grid": [NX, NY], "traces": [ntr, cfg.NT], "reps": reps}, "kernels": out}
+    print(json.dumps(res))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +324,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of this many MB (counter calibration)")
     ap.add_argument("--serve", action="store_true", help="serving trace (Poisson arrivals) on the request server")
+    ap.add_argument("--post", action="store_true",
+                    help="time the gradient post-processing and trace-filter kernels on the workload's sizes")
     ap.add_argument("--fwi", action="store_true",
                     help="time whole FWI iterations of the workload (gradient, PCG, step search, update)")
     ap.add_argument("--dropin", action="store_true",
@@ -305,6 +354,8 @@ def main():
         return dropin(args)
     if args.fwi:
         return fwi_iteration(args)
+    if args.post:
+        return post_kernels(args)
     import ifos_amd
     w = build_workload(args.workload)
     cfg = w.cfg
