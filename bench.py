@@ -317,6 +317,8 @@ def main():
                     help="work order of several-shot launches (auto: the autotuner's pick)")
     ap.add_argument("--vel-rows", type=int, default=0, choices=[0, 1, 2, 4],
                     help="velocity step cells per thread along y (0: the autotuner's pick, or 1 without it)")
+    ap.add_argument("--vel-lds", action="store_true",
+                    help="velocity step with two rows per thread and the tile's stresses staged in LDS")
     ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"],
                     help="snapshot storage (fp32 keeps the gradient bit-identical to the reference)")
     ap.add_argument("--kernels-only", action="store_true",
@@ -369,6 +371,8 @@ def main():
     eng = ifos_amd.Engine(cfg, slots=S)
     if args.vel_rows:
         eng.set_velocity_rows(args.vel_rows)
+    if args.vel_lds:
+        eng.set_velocity_lds(True)
     if args.copy_probe:
         nb = args.copy_probe * (1 << 20)
         a, c = ifos_amd.DeviceBuffer(nb), ifos_amd.DeviceBuffer(nb)
@@ -388,8 +392,10 @@ def main():
             eng.set_work_order(args.order == "slot")
         if args.vel_rows:
             eng.set_velocity_rows(args.vel_rows)
+        if args.vel_lds:
+            eng.set_velocity_lds(True)
         log(f"[bench] launch {eng.get_launch()}, {'slot' if eng.get_work_order() else 'tile'}-major, "
-            f"velocity rows per thread {eng.get_velocity_rows()}")
+            f"velocity rows per thread {eng.get_velocity_rows()}, LDS {eng.get_velocity_lds()}")
         for kern in (0, 1):
             ms = eng.time_kernel(kern, reps=20)
             gbs = eng.kernel_bytes_per_cell(kern) * cfg.NX * cfg.NY * S / (ms * 1e-3) / 1e9
@@ -420,10 +426,14 @@ def main():
         gv, gs = eng.get_launch_grids()
         log(f"[bench] autotune: {shape[0]} shots per workgroup, grids {gv or 'one per item'} (velocity) / "
             f"{gs or 'one per item'} (stress), {'slot' if shape[3] else 'tile'}-major order, "
-            f"velocity rows per thread {eng.get_velocity_rows()}, {shape[2]:.4f} ms per step")
+            f"velocity rows per thread {eng.get_velocity_rows()} (LDS {eng.get_velocity_lds()}), "
+            f"{shape[2]:.4f} ms per step")
     if args.vel_rows:
         eng.set_velocity_rows(args.vel_rows)
+    if args.vel_lds:
+        eng.set_velocity_lds(True)
     vel_rows = eng.get_velocity_rows()
+    vel_lds = eng.get_velocity_lds()
 
     def step(k):
         batch = batches[k]
@@ -482,7 +492,7 @@ def main():
         "data": "synthetic (seeded layered model with faults; observed traces from the true model)",
         "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "dh": cfg.DH, "FDORDER": cfg.FDORDER, "L": cfg.L,
                    "NT": cfg.NT, "DTINV": cfg.DTINV, "snapshots": args.snap, "shots_per_step": S, "receivers": int(len(w.shots[0].recpos)),
-                   "velocity_rows_per_thread": vel_rows,
+                   "velocity_rows_per_thread": vel_rows, "velocity_lds": vel_lds,
                    "parallelism": "one GPU, one shot at a time" if S == 1 else
                    f"one GPU, {S} shots per launch" + (f" ({shape[0]} per workgroup, {'slot' if shape[3] else 'tile'}-major)"
                                                        if shape else "")},

@@ -235,6 +235,14 @@ class Engine:
         _check(lib().ifos_engine_get_velocity_rows(self.h, C.byref(v)))
         return v.value
 
+    def set_velocity_lds(self, on):
+        _check(lib().ifos_engine_set_velocity_lds(self.h, int(bool(on))))
+
+    def get_velocity_lds(self):
+        v = C.c_int()
+        _check(lib().ifos_engine_get_velocity_lds(self.h, C.byref(v)))
+        return v.value
+
     def set_work_order(self, slot_major):
         _check(lib().ifos_engine_set_work_order(self.h, int(bool(slot_major))))
 

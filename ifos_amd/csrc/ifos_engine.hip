@@ -722,7 +722,9 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
     while (unit < 32 && NT % (unit * 2) == 0) unit *= 2;
     const int kind = sw;
     const int key[6] = {E->nsrc, E->ntr,
-                        (snapshots ? 1 : 0) + 2 * (int)(E->shape.cpml_weight * 4.0f) + 64 * E->shape.rows_v, E->nslots,
+                        (snapshots ? 1 : 0) + 2 * (int)(E->shape.cpml_weight * 4.0f) + 64 * E->shape.rows_v +
+                            1024 * E->shape.lds_v,
+                        E->nslots,
                         ((E->shape.order * 8 + E->shape.spb) * 4096 + E->shape.grid_v) * 4096 + E->shape.grid, unit};
     const LaunchShape ls = E->shape;
     StepArgs a = shaped_args(step_args(E, sw, snapshots), ls);
@@ -1345,12 +1347,26 @@ extern "C" int ifos_engine_set_velocity_rows(ifos_engine *E, int rows)
 {
     if (rows != 1 && rows != 2 && rows != 4) return ifos_set_error("velocity rows per thread must be 1, 2 or 4"), -2;
     E->shape.rows_v = rows;
+    if (rows != 2) E->shape.lds_v = 0;
     return 0;
 }
 
 extern "C" int ifos_engine_get_velocity_rows(ifos_engine *E, int *rows)
 {
     *rows = E->shape.rows_v;
+    return 0;
+}
+
+extern "C" int ifos_engine_set_velocity_lds(ifos_engine *E, int on)
+{
+    E->shape.lds_v = on ? 1 : 0;
+    if (on) E->shape.rows_v = 2;
+    return 0;
+}
+
+extern "C" int ifos_engine_get_velocity_lds(ifos_engine *E, int *on)
+{
+    *on = E->shape.lds_v;
     return 0;
 }
 
@@ -1503,11 +1519,15 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
             const StepArgs a = shaped_args(a0, ls);
             ENG_TRY(upload_args(E, a, 3, &da));
             float best_v = 1e30f, best_s = 1e30f;
-            int grid_v = 0, grid_s = 0, rows_v = 1;
+            int grid_v = 0, grid_s = 0, rows_v = 1, lds_v = 0;
             for (int per_cu : {0, 2, 3, 4})
-            for (int rows : {1, 2, 4}) {
+            // rows per thread; the LDS-staged variant (lds_v) ran at half the
+            // speed of two rows per thread on every workload measured, so it is
+            // not a candidate here
+            for (int vshape : {1, 2, 4}) {
                 ls.grid = ls.grid_v = per_cu ? (per_cu * ncu + 7) / 8 * 8 : 0;
-                ls.rows_v = rows;
+                ls.rows_v = vshape == 3 ? 2 : vshape;
+                ls.lds_v = vshape == 3 ? 1 : 0;
                 launch_vel_step(a, da, ls, 2, E->stream);
                 launch_str_step(a, da, ls, 2, E->stream);
                 HIP_TRY(hipEventRecord(ev[0], E->stream));
@@ -1528,7 +1548,7 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
                 }
                 mv /= reps;
                 ms /= reps;
-                if (mv < best_v) best_v = mv, grid_v = ls.grid_v, rows_v = rows;
+                if (mv < best_v) best_v = mv, grid_v = ls.grid_v, rows_v = ls.rows_v, lds_v = ls.lds_v;
                 if (ms < best_s) best_s = ms, grid_s = ls.grid;
             }
             if (best_v + best_s < best) {
@@ -1536,6 +1556,7 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
                 pick = ls;
                 pick.grid_v = grid_v;
                 pick.rows_v = rows_v;
+                pick.lds_v = lds_v;
                 pick.grid = grid_s;
             }
         }

@@ -90,6 +90,7 @@ struct LaunchShape {
     int grid_v = 0;  // velocity step
     int order = 0;   // StepArgs::slot_major
     int rows_v = 1;  // velocity step: cells per thread along y (1, 2 or 4: a 64x(8/rows) block)
+    int lds_v = 0;   // velocity step: two rows per thread with the tile's stresses staged in LDS
     float cpml_weight = 1.0f;  // extra cost of a tile in an absorbing strip, for the XCD split
 };
 

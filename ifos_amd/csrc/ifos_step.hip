@@ -338,10 +338,13 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
 // R-row thread, slot s; the same arithmetic as vel_cell for each cell.  All
 // loads of the R cells issue before any store, so their memory latencies
 // overlap; live[r] implies live[r-1].
-template <int N, int R>
-__device__ __forceinline__ void vel_cell_rows(const StepArgs &a, int s, int blk, int i, int j, const bool (&live)[R],
+// Derivs(sxx_x, sxy_x, sxy_y, syy_y) fills the stress derivatives of the R
+// cells (called only when live[0]): from global memory (vel_cell_rows) or
+// from the workgroup's LDS copy of the tile (k_vel_step_lds).
+template <int N, int R, typename Derivs>
+__device__ __forceinline__ void vel_rows_with(const StepArgs &a, int s, int blk, int i, int j, const bool (&live)[R],
                                               long c, long e, int nt, int flags, const float (&rip)[R],
-                                              const float (&rjp)[R], float (*s_red)[kTileY])
+                                              const float (&rjp)[R], float (*s_red)[kTileY], Derivs derivs)
 {
     const StepGeom &g = a.g;
     const int P = g.pitch, NX = g.NX;
@@ -351,10 +354,7 @@ __device__ __forceinline__ void vel_cell_rows(const StepArgs &a, int s, int blk,
 #pragma unroll
     for (int r = 0; r < R; r++) sxx_x[r] = sxy_x[r] = sxy_y[r] = syy_y[r] = vx0[r] = vy0[r] = 0.0f;
     if (live[0]) {
-        // the padded grid holds rows 1-H .. NY+H with H = N+1; with j <= NY,
-        // R = 2 reads up to row NY+N+1 and larger R clamp to it
-        stress_derivs_rows<N, R>(g.hc, a.sxx + wo + c, a.syy + wo + c, a.sxy + wo + c, P, g.NY + N + 1 - j, sxx_x,
-                                 sxy_x, sxy_y, syy_y);
+        derivs(sxx_x, sxy_x, sxy_y, syy_y);
 #pragma unroll
         for (int r = 0; r < R; r++)
             if (live[r]) {
@@ -441,6 +441,22 @@ __device__ __forceinline__ void vel_cell_rows(const StepArgs &a, int s, int blk,
     }
 }
 
+template <int N, int R>
+__device__ __forceinline__ void vel_cell_rows(const StepArgs &a, int s, int blk, int i, int j, const bool (&live)[R],
+                                              long c, long e, int nt, int flags, const float (&rip)[R],
+                                              const float (&rjp)[R], float (*s_red)[kTileY])
+{
+    const StepGeom &g = a.g;
+    const long wo = (long)s * a.ws;
+    vel_rows_with<N, R>(a, s, blk, i, j, live, c, e, nt, flags, rip, rjp, s_red,
+                        [&](float(&sxx_x)[R], float(&sxy_x)[R], float(&sxy_y)[R], float(&syy_y)[R]) {
+                            // the padded grid holds rows 1-H .. NY+H with H = N+1; with
+                            // j <= NY, R = 2 reads up to row NY+N+1 and larger R clamp to it
+                            stress_derivs_rows<N, R>(g.hc, a.sxx + wo + c, a.syy + wo + c, a.sxy + wo + c, g.pitch,
+                                                     g.NY + N + 1 - j, sxx_x, sxy_x, sxy_y, syy_y);
+                        });
+}
+
 // RY = 1: a 64x8 block, one cell per thread.  RY = 2 or 4: a 64x(8/RY)
 // block, each thread owns rows RY*ty .. RY*ty+RY-1 of the 64x8 tile
 // (vel_cell_rows).
@@ -484,6 +500,114 @@ __global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ p
             for (int u = 0; u < SPB; u++)
                 if (s0 + u < m.nslots)
                     vel_cell_rows<N, RY>(a, s0 + u, blk, i, j, live, c, e, nt, flags, rip, rjp, s_red);
+        }
+    }
+}
+
+// Velocity step with the tile's stresses staged in LDS: a 64x4 workgroup of
+// two-row threads first copies, for its 64x8 tile, sxx (rows j0..j0+7,
+// columns i0-N..i0+63+N), sxy (rows j0-N..j0+7+N, same columns) and syy (rows
+// j0-N..j0+7+N, columns i0..i0+63) from HBM into LDS with row-contiguous
+// loads, then every stencil value comes from LDS.  Coordinates outside the
+// padded grid are clamped to its edge; they only feed cells outside the
+// model, which are not stored.  Same arithmetic as vel_cell.
+template <int N>
+struct VelTile {
+    static constexpr int W = kTileX + 2 * N;  // LDS column x <-> grid column i0 - N + x
+    static constexpr int H = kTileY + 2 * N;  // LDS row y <-> grid row j0 - N + y (sxy, syy)
+    float sxx[kTileY][W];                     // LDS row y <-> grid row j0 + y
+    float sxy[H][W];
+    float syy[H][kTileX];                     // LDS column x <-> grid column i0 + x
+};
+
+template <int N>
+__device__ __forceinline__ void stage_tile(VelTile<N> &t, const float *sxx_, const float *sxy_, const float *syy_,
+                                           int P, int i0, int j0, int NX, int NY)
+{
+    constexpr int W = VelTile<N>::W, H = VelTile<N>::H;
+    const auto sxx = gmem(sxx_), sxy = gmem(sxy_), syy = gmem(syy_);
+    const int lo_i = -N, hi_i = NX + N + 1, lo_j = -N, hi_j = NY + N + 1;  // inside the padded grid
+    const int tid = threadIdx.y * kTileX + threadIdx.x, nth = kTileX * (kTileY / 2);
+    for (int q = tid; q < kTileY * W; q += nth) {
+        const int y = q / W, x = q - y * W;
+        const int ii = min(max(i0 - N + x, lo_i), hi_i), jj = min(j0 + y, hi_j);
+        t.sxx[y][x] = sxx[(long)jj * P + ii];
+    }
+    for (int q = tid; q < H * W; q += nth) {
+        const int y = q / W, x = q - y * W;
+        const int ii = min(max(i0 - N + x, lo_i), hi_i), jj = min(max(j0 - N + y, lo_j), hi_j);
+        t.sxy[y][x] = sxy[(long)jj * P + ii];
+    }
+    for (int q = tid; q < H * kTileX; q += nth) {
+        const int y = q / kTileX, x = q - y * kTileX;
+        const int ii = min(i0 + x, hi_i), jj = min(max(j0 - N + y, lo_j), hi_j);
+        t.syy[y][x] = syy[(long)jj * P + ii];
+    }
+}
+
+template <int N, int SPB>
+__global__ __launch_bounds__(256) void k_vel_step_lds(const StepArgs *__restrict__ pa, int k, int persistent,
+                                                      int flags)
+{
+    constexpr int R = 2;
+    const StepArgs &a = *pa;
+    __shared__ float s_red[2][kTileY];
+    __shared__ VelTile<N> t;
+    const StepGeom &g = a.g;
+    const ItemMap m = item_map(a);
+    const int nt = step_number(a, k);
+    int first, end, stride;
+    work_range(m, m.ntiles * m.ngroups, persistent != 0, first, end, stride);
+    for (int it = first; it < end; it += stride) {
+        int blk, s0;
+        item_of(m, it, SPB, blk, s0);
+        blk = __builtin_amdgcn_readfirstlane(blk);
+        s0 = __builtin_amdgcn_readfirstlane(s0);
+        const int by = (int)fdiv(blk, m.nbx_mul, m.nbx_sh), bx = blk - by * m.nbx;
+        const int i0 = bx * kTileX + 1, j0 = by * kTileY + 1;
+        const int tx = threadIdx.x, jr = R * threadIdx.y;  // this thread's rows: j0 + jr, j0 + jr + 1
+        const int i = i0 + tx, j = j0 + jr;
+        const long c = (long)j * g.pitch + i;
+        const long e = (long)(j - 1) * g.NX + (i - 1);
+        bool live[R];
+        float rip[R], rjp[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            live[r] = i <= g.NX && j + r <= g.NY;
+            rip[r] = live[r] ? gmem(a.rip)[c + r * g.pitch] : 0.0f;
+            rjp[r] = live[r] ? gmem(a.rjp)[c + r * g.pitch] : 0.0f;
+        }
+#pragma unroll 1
+        for (int u = 0; u < SPB; u++) {
+            const int sl = s0 + u;
+            if (sl >= m.nslots) break;  // uniform: the whole workgroup leaves together
+            const long wo = (long)sl * a.ws;
+            __syncthreads();  // the previous item's or slot's reads of t are done
+            stage_tile<N>(t, a.sxx + wo, a.sxy + wo, a.syy + wo, g.pitch, i0, j0, g.NX, g.NY);
+            __syncthreads();
+            vel_rows_with<N, R>(a, sl, blk, i, j, live, c, e, nt, flags, rip, rjp, s_red,
+                                [&](float(&sxx_x)[R], float(&sxy_x)[R], float(&sxy_y)[R], float(&syy_y)[R]) {
+                                    const float *hc = g.hc;
+#pragma unroll
+                                    for (int r = 0; r < R; r++) {
+                                        const float *sx = &t.sxx[jr + r][tx + N];
+                                        const float *sy = &t.sxy[jr + r + N][tx + N];
+                                        const float *cy = &t.sxy[jr + r + N][tx + N];
+                                        const float *cs = &t.syy[jr + r + N][tx];
+                                        constexpr int Wd = VelTile<N>::W;
+                                        sxx_x[r] = hc[1] * (sx[1] - sx[0]);
+                                        sxy_x[r] = hc[1] * (sy[0] - sy[-1]);
+                                        sxy_y[r] = hc[1] * (cy[0] - cy[-Wd]);
+                                        syy_y[r] = hc[1] * (cs[kTileX] - cs[0]);
+#pragma unroll
+                                        for (int mm = 2; mm <= N; mm++) {
+                                            sxx_x[r] = sxx_x[r] + hc[mm] * (sx[mm] - sx[1 - mm]);
+                                            sxy_x[r] = sxy_x[r] + hc[mm] * (sy[mm - 1] - sy[-mm]);
+                                            sxy_y[r] = sxy_y[r] + hc[mm] * (cy[(mm - 1) * Wd] - cy[-mm * Wd]);
+                                            syy_y[r] = syy_y[r] + hc[mm] * (cs[mm * kTileX] - cs[(1 - mm) * kTileX]);
+                                        }
+                                    }
+                                });
         }
     }
 }
@@ -741,7 +865,10 @@ template <int N, int SPB>
 static void vel_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
 {
     const int gr = grid_of(a, ls, ls.grid_v);
-    if (ls.rows_v == 4)
+    if (ls.lds_v)
+        hipLaunchKernelGGL((k_vel_step_lds<N, SPB>), dim3(gr), dim3(kTileX, kTileY / 2), 0, s, da, k,
+                           ls.grid_v > 0 ? 1 : 0, step_flags(a, k));
+    else if (ls.rows_v == 4)
         hipLaunchKernelGGL((k_vel_step<N, SPB, 4>), dim3(gr), dim3(kTileX, kTileY / 4), 0, s, da, k,
                            ls.grid_v > 0 ? 1 : 0, step_flags(a, k));
     else if (ls.rows_v == 2)

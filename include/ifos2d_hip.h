@@ -129,6 +129,14 @@ int  ifos_engine_get_work_order(ifos_engine *e, int *slot_major);
  * every way; the autotuner picks the fastest. */
 int  ifos_engine_set_velocity_rows(ifos_engine *e, int rows);
 int  ifos_engine_get_velocity_rows(ifos_engine *e, int *rows);
+/* Velocity step with two rows per thread whose stencil values come from an
+ * LDS copy of the tile and its halo (one row-contiguous read per value and
+ * workgroup) instead of per-thread global loads; on = 1 also sets two rows
+ * per thread, set_velocity_rows with 1 or 4 clears it.  Same bits.  The
+ * autotuner does not pick it: on the 2000x800 grid it takes 28.7 us per
+ * launch against 15.0 us for two rows from global memory. */
+int  ifos_engine_set_velocity_lds(ifos_engine *e, int on);
+int  ifos_engine_get_velocity_lds(ifos_engine *e, int *on);
 /* Time every candidate launch shape (and, with several slots, both work
  * orders) on one forward step of the current model and shots (reps launches
  * of each kernel), keep the fastest, zero the stepped wavefields; reports
