@@ -19,6 +19,7 @@
 // own definitions, which take precedence at dynamic link time.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -196,6 +197,52 @@ struct PinnedRing {
 };
 PinnedRing g_ring;
 
+// Seismogram sections: seismo_ssg gathers each step's receiver samples into
+// a device buffer [2][ntr][NT] per pair of host sections and returns at
+// once; ifos_dev_sync() copies the samples gathered since the last flush
+// into the host sections (the reference reads them after its time loop,
+// src/IFOS2D.c:1903-1961).
+struct SectionBuf {
+    float **s0 = nullptr, **s1 = nullptr;  // host sections (s1 null for SEISMO 2)
+    int ntr = 0, ns = 0, lo = 0, hi = -1;   // samples lo..hi (1-based) not yet flushed
+    float *dev = nullptr;
+    int *dpos = nullptr;
+    std::vector<int> pos;                   // receiver cells of the positions in dpos
+};
+std::unordered_map<const void *, SectionBuf> g_sections;
+
+__global__ void k_gather_col(int n, const int *ix, const int *iy, int pitch, const float *a, const float *b,
+                             int mode, float *out, int ld, int col)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long c = (long)iy[t] * pitch + ix[t];
+    out[(long)t * ld + col] = mode == 2 ? -a[c] - b[c] : a[c];
+    if (mode == 1) out[(long)(n + t) * ld + col] = b[c];
+}
+
+// copy the unflushed samples of every section pair to the host (after a
+// stream synchronisation)
+hipError_t flush_sections()
+{
+    for (auto &kv : g_sections) {
+        SectionBuf &sb = kv.second;
+        if (sb.hi < sb.lo) continue;
+        const int n = sb.s1 ? 2 * sb.ntr : sb.ntr;
+        std::vector<float> h((size_t)n * sb.ns);
+        hipError_t e = hipMemcpy(h.data(), sb.dev, sizeof(float) * h.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return e;
+        for (int t = 1; t <= sb.ntr; t++)
+            for (int k = sb.lo; k <= sb.hi; k++) {
+                sb.s0[t][k] = h[(size_t)(t - 1) * sb.ns + k - 1];
+                if (sb.s1) sb.s1[t][k] = h[(size_t)(sb.ntr + t - 1) * sb.ns + k - 1];
+            }
+        sb.lo = 0;
+        sb.hi = -1;
+    }
+    return hipSuccess;
+}
+
 StepGeom make_geom(int nx1, int nx2, int ny1, int ny2, int pitch, const float *hc, const float *K_x,
                    const float *a_x, const float *b_x, const float *K_xh, const float *a_xh, const float *b_xh,
                    const float *K_y, const float *a_y, const float *b_y, const float *K_yh, const float *a_yh,
@@ -229,16 +276,6 @@ __global__ void k_point_add(int n, const int *ix, const int *iy, const double *a
         f1[c] = (float)((double)f1[c] + 0.5 * amp[l]);
         if (f2) f2[c] = (float)((double)f2[c] + 0.5 * amp[l]);
     }
-}
-
-__global__ void k_gather(int n, const int *ix, const int *iy, int pitch, const float *a, const float *b, int mode,
-                         float *out)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const long c = (long)iy[t] * pitch + ix[t];
-    out[t] = mode == 2 ? -a[c] - b[c] : a[c];
-    if (mode == 1) out[n + t] = b[c];
 }
 
 __global__ void k_zero_rect(float *origin, int pitch, int j0, int j1, int i0, int i1)
@@ -360,6 +397,7 @@ extern "C" int ifos_dev_copy_array(void *dev_arr, const void *host_first_elem, i
 extern "C" int ifos_dev_sync(void)
 {
     hipError_t e = hipStreamSynchronize(stream());
+    if (e == hipSuccess) e = flush_sections();
     if (e != hipSuccess) return ifos_set_error("%s", hipGetErrorString(e)), -1;
     return 0;
 }
@@ -748,27 +786,42 @@ extern "C" void seismo_ssg(int lsamp, int ntr, int **recpos, float **sectionvx, 
         a = origin_of(sxx, &P, "sxx");
         b = origin_of(syy, &p2, "syy");
     }
+    if (lsamp < 1 || lsamp > NT) fatal("seismo_ssg: sample index outside 1..NT");
     std::vector<int> pos(2 * ntr);
     for (int t = 1; t <= ntr; t++) {
         pos[t - 1] = recpos[1][t];
         pos[ntr + t - 1] = recpos[2][t];
     }
-    int *dpos = (int *)g_stage[2].get(sizeof(int) * pos.size());
-    float *dout = (float *)g_stage[3].get(sizeof(float) * 2 * ntr);
-    hip_check(hipMemcpyAsync(dpos, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice, stream()), "recpos");
-    hipLaunchKernelGGL(k_gather, dim3((ntr + 63) / 64), dim3(64), 0, stream(), ntr, dpos, dpos + ntr, P, a, b,
-                       SEISMO == 1 ? 1 : 2, dout);
-    std::vector<float> h(2 * ntr);
-    hip_check(hipMemcpyAsync(h.data(), dout, sizeof(float) * 2 * ntr, hipMemcpyDeviceToHost, stream()), "gather");
-    hip_check(hipStreamSynchronize(stream()), "gather sync");
-    for (int t = 1; t <= ntr; t++) {
-        if (SEISMO == 1) {
-            sectionvx[t][lsamp] = h[t - 1];
-            sectionvy[t][lsamp] = h[ntr + t - 1];
-        } else {
-            sectionp[t][lsamp] = h[t - 1];
-        }
+    float **s0 = SEISMO == 1 ? sectionvx : sectionp, **s1 = SEISMO == 1 ? sectionvy : nullptr;
+    SectionBuf &sb = g_sections[s0];
+    if (sb.ntr != ntr || sb.ns != NT || sb.s1 != s1) {
+        // a new section pair, or one reused with another shape: flush what it
+        // holds and size its buffer for this shape
+        hip_check(hipStreamSynchronize(stream()), "section sync");
+        hip_check(flush_sections(), "section flush");
+        if (sb.dev) hip_check(hipFree(sb.dev), "hipFree");
+        if (sb.dpos) hip_check(hipFree(sb.dpos), "hipFree");
+        sb = SectionBuf();
+        sb.s0 = s0;
+        sb.s1 = s1;
+        sb.ntr = ntr;
+        sb.ns = NT;
+        hip_check(hipMalloc(&sb.dev, sizeof(float) * 2 * (size_t)ntr * NT), "hipMalloc(section)");
+        hip_check(hipMalloc(&sb.dpos, sizeof(int) * 2 * ntr), "hipMalloc(section positions)");
     }
+    if (sb.pos != pos) {
+        int slot;
+        char *blob = g_ring.get(sizeof(int) * pos.size(), slot);
+        memcpy(blob, pos.data(), sizeof(int) * pos.size());
+        hip_check(hipMemcpyAsync(sb.dpos, blob, sizeof(int) * pos.size(), hipMemcpyHostToDevice, stream()),
+                  "recpos");
+        g_ring.queued(slot);
+        sb.pos = pos;
+    }
+    hipLaunchKernelGGL(k_gather_col, dim3((ntr + 63) / 64), dim3(64), 0, stream(), ntr, sb.dpos, sb.dpos + ntr, P,
+                       a, b, SEISMO == 1 ? 1 : 2, sb.dev, NT, lsamp - 1);
+    sb.lo = sb.hi < sb.lo ? lsamp : std::min(sb.lo, lsamp);
+    sb.hi = std::max(sb.hi, lsamp);
 }
 
 // ---------------------------------------------------------------------------

@@ -8,9 +8,11 @@
  * reference's matrix()/f3tensor() calls, src/IFOS2D.c:371-641).  The CPML
  * profile vectors, hc, bip..cjm, etajm/peta, srcpos_loc, signals and the
  * seismogram sections stay host arrays as in the reference.  Calls are queued
- * in order on one device stream; call ifos_dev_sync() before the host reads
- * device data and use ifos_dev_read() for single values (the stability probe
- * of src/IFOS2D.c:1660).  A device tensor keeps its elements plane by plane
+ * in order on one device stream and return without waiting for the device;
+ * call ifos_dev_sync() before the host reads device data or the seismogram
+ * sections (seismo_ssg gathers the samples on the device; ifos_dev_sync()
+ * writes them into the host sections) and use ifos_dev_read() for single
+ * values (the stability probe of src/IFOS2D.c:1660).  A device tensor keeps its elements plane by plane
  * (third index slowest), so every kernel reads and writes whole planes; the
  * float*** it returns is a handle for these calls, not for host indexing,
  * and ifos_dev_copy_array() converts from and to util.c's [j][i][l] block.
@@ -82,7 +84,8 @@ void surface_PML(int ndepth, float **vx, float **vy, float **sxx, float **syy, f
                  float *etajm, float *peta, float *hc, float *K_x, float *a_x, float *b_x, float **psi_vxx,
                  float **ux, float **uy, float **uxy, float **uyz, float **sxz, float **uxz);
 
-/* replaces seismo_ssg()              src/seismo_ssg.c:26, src/fd.h:262-265 (SEISMO 1, 2) */
+/* replaces seismo_ssg()              src/seismo_ssg.c:26, src/fd.h:262-265 (SEISMO 1, 2);
+ * the samples reach the host sections at the next ifos_dev_sync() */
 void seismo_ssg(int lsamp, int ntr, int **recpos, float **sectionvx, float **sectionvy, float **sectionvz,
                 float **sectionp, float **sectioncurl, float **sectiondiv, float **vx, float **vy, float **vz,
                 float **sxx, float **syy, float **sp, float **pi, float **u, float *hc);
