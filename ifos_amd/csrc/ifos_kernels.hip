@@ -11,6 +11,7 @@
 // device int (nt_dev) so that a captured graph of a block of time steps can
 // be replayed.  The single-block inject kernel advances the counter; kernels
 // that run before it within a step (update_v) see the previous value.
+#include "ifos_cellmath.h"
 #include "ifos_internal.h"
 
 namespace ifos {
@@ -32,57 +33,9 @@ IFOS_KERNEL void k_update_v(VArgs a, const int *nt_dev, int DTINV, int NT)
     if (i > g.NX || j > g.NY) return;
     const int P = g.pitch;
     const long c = (long)j * P + i;
-    const float *sxx = a.sxx + c, *syy = a.syy + c, *sxy = a.sxy + c;
-
-    float sxx_x = g.hc[1] * (sxx[1] - sxx[0]);
-    float sxy_x = g.hc[1] * (sxy[0] - sxy[-1]);
-    float sxy_y = g.hc[1] * (sxy[0] - sxy[-P]);
-    float syy_y = g.hc[1] * (syy[P] - syy[0]);
-#pragma unroll
-    for (int m = 2; m <= N; m++) {
-        sxx_x = sxx_x + g.hc[m] * (sxx[m] - sxx[1 - m]);
-        sxy_x = sxy_x + g.hc[m] * (sxy[m - 1] - sxy[-m]);
-        sxy_y = sxy_y + g.hc[m] * (sxy[(m - 1) * P] - sxy[-m * P]);
-        syy_y = syy_y + g.hc[m] * (syy[m * P] - syy[(1 - m) * P]);
-    }
-
-    const Cpml &pm = g.pml;
-    if (!g.BOUNDARY && i <= g.FW) {
-        const int h = i - 1;
-        float *p1 = a.psi_sxx_x + (long)(j - 1) * g.px + h;
-        float *p2 = a.psi_sxy_x + (long)(j - 1) * g.px + h;
-        *p1 = pm.b_xh[h] * *p1 + pm.a_xh[h] * sxx_x;
-        sxx_x = sxx_x / pm.K_xh[h] + *p1;
-        *p2 = pm.b_x[h] * *p2 + pm.a_x[h] * sxy_x;
-        sxy_x = sxy_x / pm.K_x[h] + *p2;
-    }
-    if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
-        const int h = i - g.NX + 2 * g.FW - 1;
-        float *p1 = a.psi_sxx_x + (long)(j - 1) * g.px + h;
-        float *p2 = a.psi_sxy_x + (long)(j - 1) * g.px + h;
-        *p1 = pm.b_xh[h] * *p1 + pm.a_xh[h] * sxx_x;
-        sxx_x = sxx_x / pm.K_xh[h] + *p1;
-        *p2 = pm.b_x[h] * *p2 + pm.a_x[h] * sxy_x;
-        sxy_x = sxy_x / pm.K_x[h] + *p2;
-    }
-    if (!g.FREE_SURF && j <= g.FW) {
-        const int h = j - 1;
-        float *p1 = a.psi_syy_y + (long)h * g.py + (i - 1);
-        float *p2 = a.psi_sxy_y + (long)h * g.py + (i - 1);
-        *p1 = pm.b_yh[h] * *p1 + pm.a_yh[h] * syy_y;
-        syy_y = syy_y / pm.K_yh[h] + *p1;
-        *p2 = pm.b_y[h] * *p2 + pm.a_y[h] * sxy_y;
-        sxy_y = sxy_y / pm.K_y[h] + *p2;
-    }
-    if (j >= g.NY - g.FW + 1) {
-        const int h = j - g.NY + 2 * g.FW - 1;
-        float *p1 = a.psi_syy_y + (long)h * g.py + (i - 1);
-        float *p2 = a.psi_sxy_y + (long)h * g.py + (i - 1);
-        *p1 = pm.b_yh[h] * *p1 + pm.a_yh[h] * syy_y;
-        syy_y = syy_y / pm.K_yh[h] + *p1;
-        *p2 = pm.b_y[h] * *p2 + pm.a_y[h] * sxy_y;
-        sxy_y = sxy_y / pm.K_y[h] + *p2;
-    }
+    float sxx_x, sxy_x, sxy_y, syy_y;
+    stress_derivs<N>(g.hc, a.sxx + c, a.syy + c, a.sxy + c, P, sxx_x, sxy_x, sxy_y, syy_y);
+    cpml_velocity(g, j, i, a.psi_sxx_x, a.psi_sxy_x, a.psi_syy_y, a.psi_sxy_y, sxx_x, sxy_x, sxy_y, syy_y);
 
     const float rip = a.rip[c], rjp = a.rjp[c];
     const float vx = a.vx[c], vy = a.vy[c];
@@ -161,75 +114,6 @@ IFOS_KERNEL void k_inject(PointArgs a, StepGeom g, float *vx, float *vy, const f
 }
 
 // ---------------------------------------------------------------------------
-// velocity derivatives + stress-side CPML (src/update_s_elastic_PML.c:
-// 151-219; identical in src/update_s_visc_PML.c)
-// ---------------------------------------------------------------------------
-template <int N>
-__device__ __forceinline__ void vel_derivs(const StepGeom &g, const float *vx, const float *vy, float scale,
-                                           float &vxx, float &vyx, float &vxy, float &vyy)
-{
-    const int P = g.pitch;
-    vxx = g.hc[1] * (vx[0] - vx[-1]);
-    vyx = g.hc[1] * (vy[1] - vy[0]);
-    vxy = g.hc[1] * (vx[P] - vx[0]);
-    vyy = g.hc[1] * (vy[0] - vy[-P]);
-#pragma unroll
-    for (int m = 2; m <= N; m++) {
-        vxx = vxx + g.hc[m] * (vx[m - 1] - vx[-m]);
-        vyx = vyx + g.hc[m] * (vy[m] - vy[1 - m]);
-        vxy = vxy + g.hc[m] * (vx[m * P] - vx[(1 - m) * P]);
-        vyy = vyy + g.hc[m] * (vy[(m - 1) * P] - vy[-m * P]);
-    }
-    vxx = vxx * scale;
-    vyx = vyx * scale;
-    vxy = vxy * scale;
-    vyy = vyy * scale;
-}
-
-__device__ __forceinline__ void cpml_stress(const StepGeom &g, int j, int i, float *psi_vxx, float *psi_vyx,
-                                            float *psi_vyy, float *psi_vxy, float &vxx, float &vyx, float &vxy,
-                                            float &vyy)
-{
-    const Cpml &pm = g.pml;
-    if (!g.BOUNDARY && i <= g.FW) {
-        const int h = i - 1;
-        float *p1 = psi_vxx + (long)(j - 1) * g.px + h;
-        float *p2 = psi_vyx + (long)(j - 1) * g.px + h;
-        *p1 = pm.b_x[h] * *p1 + pm.a_x[h] * vxx;
-        vxx = vxx / pm.K_x[h] + *p1;
-        *p2 = pm.b_xh[h] * *p2 + pm.a_xh[h] * vyx;
-        vyx = vyx / pm.K_xh[h] + *p2;
-    }
-    if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
-        const int h = i - g.NX + 2 * g.FW - 1;
-        float *p1 = psi_vxx + (long)(j - 1) * g.px + h;
-        float *p2 = psi_vyx + (long)(j - 1) * g.px + h;
-        *p1 = pm.b_x[h] * *p1 + pm.a_x[h] * vxx;
-        vxx = vxx / pm.K_x[h] + *p1;
-        *p2 = pm.b_xh[h] * *p2 + pm.a_xh[h] * vyx;
-        vyx = vyx / pm.K_xh[h] + *p2;
-    }
-    if (!g.FREE_SURF && j <= g.FW) {
-        const int h = j - 1;
-        float *p1 = psi_vyy + (long)h * g.py + (i - 1);
-        float *p2 = psi_vxy + (long)h * g.py + (i - 1);
-        *p1 = pm.b_y[h] * *p1 + pm.a_y[h] * vyy;
-        *p2 = pm.b_yh[h] * *p2 + pm.a_yh[h] * vxy;
-        vyy = vyy / pm.K_y[h] + *p1;
-        vxy = vxy / pm.K_yh[h] + *p2;
-    }
-    if (j >= g.NY - g.FW + 1) {
-        const int h = j - g.NY + 2 * g.FW - 1;
-        float *p1 = psi_vyy + (long)h * g.py + (i - 1);
-        float *p2 = psi_vxy + (long)h * g.py + (i - 1);
-        *p1 = pm.b_y[h] * *p1 + pm.a_y[h] * vyy;
-        vyy = vyy / pm.K_y[h] + *p1;
-        *p2 = pm.b_yh[h] * *p2 + pm.a_yh[h] * vxy;
-        vxy = vxy / pm.K_yh[h] + *p2;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // update_s_elastic_PML  (src/update_s_elastic_PML.c:28-569)
 // ---------------------------------------------------------------------------
 template <int N>
@@ -242,7 +126,7 @@ IFOS_KERNEL void k_update_s_elastic(SElArgs a, const int *nt_dev, int DTINV)
     const long c = (long)j * g.pitch + i;
     const float dhi = g.DT / g.DH;
     float vxx, vyx, vxy, vyy;
-    vel_derivs<N>(g, a.vx + c, a.vy + c, dhi, vxx, vyx, vxy, vyy);
+    velocity_derivs<N>(g.hc, a.vx + c, a.vy + c, g.pitch, dhi, vxx, vyx, vxy, vyy);
     cpml_stress(g, j, i, a.psi_vxx, a.psi_vyx, a.psi_vyy, a.psi_vxy, vxx, vyx, vxy, vyy);
 
     const float fipjp = a.uipjp[c];
@@ -282,7 +166,7 @@ IFOS_KERNEL void k_update_s_visc(SViscArgs a, const int *nt_dev, int DTINV)
     const float dhi = 1.0 / (double)g.DH;
     const float dthalbe = g.DT / 2.0;
     float vxx, vyx, vxy, vyy;
-    vel_derivs<N>(g, a.vx + c, a.vy + c, dhi, vxx, vyx, vxy, vyy);
+    velocity_derivs<N>(g.hc, a.vx + c, a.vy + c, g.pitch, dhi, vxx, vyx, vxy, vyy);
     cpml_stress(g, j, i, a.psi_vxx, a.psi_vyx, a.psi_vyy, a.psi_vxy, vxx, vyx, vxy, vyy);
 
     const long mc = c * a.cs;
