@@ -4,6 +4,7 @@ the snapshot store, adjoint loop (:2224-2419) with the zero-lag
 correlation, chain rule (:2471-2611) -- on device-resident
 Numerical-Recipes arrays, compared bit for bit with the golden vectors
 written from the reference's own kernels, elastic and viscoelastic."""
+import ctypes as C
 import json
 import os
 
@@ -11,8 +12,10 @@ import numpy as np
 import pytest
 
 import ifos_amd
+from ifos_amd import workloads
 from ifos_amd.dropin import DropinShot
 import oracle_lib as O
+from test_precision import params_of
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = ["el4_fs_v0", "visc8_fs_v1", "el2_nofs_rot", "visc6_fs_fx_v1"]
@@ -62,3 +65,39 @@ def test_dropin_gradient_matches_reference(gpu_lib, name):
     for k, n in enumerate(("vp", "vs", "rho")):
         np.testing.assert_array_equal(g[k], z["grad_" + n], err_msg=n)
     run.close()
+
+
+def test_c_client_forward_matches_oracle(gpu_lib, tmp_path):
+    """tests/c/dropin_forward.c: the reference's forward loop written in C
+    against include/ifos2d_dropin.h (built by __graft_entry__.build() as a
+    shared library and called in-process); seismograms and final wavefields
+    bit for bit against the oracle."""
+    so = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c", "libdropin_forward.so")
+    if not os.path.exists(so):
+        pytest.fail("tests/c/libdropin_forward.so is missing: run __graft_entry__.build()")
+    client = C.CDLL(so)
+    client.dropin_forward_run.argtypes = [C.c_char_p, C.c_char_p]
+    w = workloads.small_case(seed=71, NX=80, NY=56, NT=160, FDORDER=4, L=0, VELOCITY=1, DTINV=1, FW=10, stype=1)
+    cfg, s = w.cfg, w.shots[0]
+    src = np.ascontiguousarray(np.array(s.srcpos, np.float32).reshape(-1, 8))
+    rec = np.ascontiguousarray(np.array(s.recpos, np.int32).reshape(-1, 2))
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        f.write(np.array([cfg.NX, cfg.NY, cfg.NT, cfg.FDORDER, cfg.FW, cfg.FREE_SURF, cfg.VELOCITY, len(src), len(rec)],
+                         np.int32).tobytes())
+        f.write(np.array([cfg.DT, cfg.DH, cfg.VPPML, cfg.FPML, cfg.npower, cfg.k_max_PML], np.float32).tobytes())
+        f.write(src.tobytes())
+        f.write(rec.tobytes())
+        for k in ("vp", "vs", "rho"):
+            f.write(np.ascontiguousarray(w.true[k], np.float32).tobytes())
+    print("[c client] forward 80x56 FDORDER 4", flush=True)
+    assert client.dropin_forward_run(str(inp).encode(), str(out).encode()) == 0
+    got = np.fromfile(out, np.float32)
+    ntr, NT, n = len(rec), cfg.NT, cfg.NX * cfg.NY
+    vx = got[:ntr * NT].reshape(ntr, NT)
+    vy = got[ntr * NT:2 * ntr * NT].reshape(ntr, NT)
+    fin = got[2 * ntr * NT:].reshape(5, cfg.NY, cfg.NX)
+    ovx, ovy, ofin = O.forward(params_of(w), w.true, s.srcpos, s.recpos)
+    np.testing.assert_array_equal(vx, ovx)
+    np.testing.assert_array_equal(vy, ovy)
+    np.testing.assert_array_equal(fin, ofin)
