@@ -316,7 +316,8 @@ def main():
     ap.add_argument("--order", default="auto", choices=["auto", "tile", "slot"],
                     help="work order of several-shot launches (auto: the autotuner's pick)")
     ap.add_argument("--vel-rows", type=int, default=0, choices=[0, 1, 2, 4],
-                    help="velocity step cells per thread along y (0: the autotuner's pick, or 1 without it)")
+                    help="velocity step cells per thread along y (0: the autotuner's pick, or the engine's "
+                         "default without it)")
     ap.add_argument("--vel-lds", action="store_true",
                     help="velocity step with two rows per thread and the tile's stresses staged in LDS")
     ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"],

@@ -255,6 +255,11 @@ static int engine_create(const ifos_config *cfg, int device, int slots, ifos_eng
     E->nsnap = c.NT / c.DTINV;
     E->nbx = (c.NX + kTileX - 1) / kTileX;
     E->nby = (c.NY + kTileY - 1) / kTileY;
+    // two rows per thread in the velocity step once there are enough work
+    // items to fill the chip several times over (measured: faster on
+    // 2000x800 and on 16 x 1700x350, slower on 400x200 and on one 1700x350
+    // shot); the autotuner can change it
+    E->shape.rows_v = (long)E->nbx * E->nby * slots >= 2048 ? 2 : 1;
     ifos_holberg(c.FDORDER, c.MAXRELERROR, E->hc);
     const int L = c.L;
 

@@ -126,7 +126,8 @@ int  ifos_engine_get_work_order(ifos_engine *e, int *slot_major);
 /* Cells per thread along y in the velocity step: 1 (one cell per thread of
  * a 64x8 workgroup), 2 or 4 (adjacent rows per thread of a 64x4 or 64x2
  * workgroup, which share the y-stencil loads).  Results are the same bits
- * every way; the autotuner picks the fastest. */
+ * every way; the autotuner picks the fastest; an engine starts with 2 when it has
+ * at least 2048 (tile, shot) work items, else with 1. */
 int  ifos_engine_set_velocity_rows(ifos_engine *e, int rows);
 int  ifos_engine_get_velocity_rows(ifos_engine *e, int *rows);
 /* Velocity step with two rows per thread whose stencil values come from an
