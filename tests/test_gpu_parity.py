@@ -156,6 +156,32 @@ def test_stretched_pml_matches_oracle(gpu_lib, kmax, npower):
     eng.close()
 
 
+@pytest.mark.parametrize("adj", [2, 3])
+def test_adjoint_type_matches_oracle(gpu_lib, adj):
+    """ADJOINT_TYPE 2 (vy residuals only) and 3 (vx only)."""
+    w = workloads.small_case(seed=19 + adj, NX=70, NY=48, NT=150, FDORDER=6, L=2, VELOCITY=1, DTINV=3, FW=10)
+    print(f"[oracle gradient] ADJOINT_TYPE {adj}", flush=True)
+    cfg = w.cfg
+    cfg.ADJOINT_TYPE = adj
+    p = O.make_params(FL=[cfg.FL[i] for i in range(1, 8)],
+                      **{k: getattr(cfg, k) for k, _ in cfg._fields_ if k != "FL"})
+    shot = w.shots[0]
+    ovx, ovy, _ = O.forward(p, w.true, shot.srcpos, shot.recpos)
+    ovx[:, 0] = 0
+    ovy[:, 0] = 0
+    ref = O.gradient(p, w.start, shot.srcpos, shot.recpos, ovx, ovy)
+    eng = ifos_amd.Engine(cfg)
+    eng.set_model(**w.start)
+    eng.set_shot(shot.srcpos, shot.recpos)
+    mis = eng.shot_gradient(ovx, ovy)
+    g = eng.gradient()
+    assert_bitexact(g[0], ref["grad_vp"], "Vp gradient")
+    assert_bitexact(g[1], ref["grad_vs"], "Vs gradient")
+    assert_bitexact(g[2], ref["grad_rho"], "rho gradient")
+    assert abs(mis[0] - ref["misfit"][3]) <= 1e-12 * ref["misfit"][3]
+    eng.close()
+
+
 def test_gradient_matches_oracle_with_kill(gpu_lib):
     w = workloads.small_case(seed=5, NX=80, NY=50, NT=180, FDORDER=8, L=2, VELOCITY=1, DTINV=3)
     print("[oracle gradient] kill mask", flush=True)

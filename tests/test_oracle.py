@@ -153,6 +153,30 @@ def test_forward_matches_reference(fd, L, vel, fs, st):
 
 
 @needs_ref
+@pytest.mark.parametrize("adj", [2, 3])
+def test_adjoint_type_matches_reference(adj):
+    """ADJOINT_TYPE 2 (vy residuals only) and 3 (vx only): the injection of
+    src/update_v_PML.c:567-578 and the residual selection."""
+    w = workloads.small_case(seed=19 + adj, NX=48, NY=36, NT=120, FDORDER=4, L=2, VELOCITY=1, DTINV=3, FW=8)
+    p = params_of(w)
+    p.ADJOINT_TYPE = adj
+    s = w.shots[0]
+    ovx, ovy, _ = O.forward(p, w.true, s.srcpos, s.recpos)
+    ovx[:, 0] = 0
+    ovy[:, 0] = 0
+    g = O.gradient(p, w.start, s.srcpos, s.recpos, ovx, ovy)
+    dvx, _ = O.calc_res(p, g["seis_vx"], ovx)
+    dvy, _ = O.calc_res(p, g["seis_vy"], ovy)
+    r = O.ref_gradient(p, w.start, s.srcpos, s.recpos, dvx, dvy)
+    for k in ("grad_vp", "grad_vs", "grad_rho"):
+        np.testing.assert_array_equal(g[k], r[k], err_msg=k)
+    # the other component's residual must not reach the gradient
+    r1 = O.ref_gradient(p, w.start, s.srcpos, s.recpos, dvx * (adj == 3), dvy * (adj == 2))
+    for k in ("grad_vp", "grad_vs", "grad_rho"):
+        np.testing.assert_array_equal(r[k], r1[k], err_msg=k)
+
+
+@needs_ref
 @pytest.mark.parametrize("kmax,npower", [(4.0, 2.0), (2.5, 3.0)])
 def test_stretched_pml_matches_reference(kmax, npower):
     """A CPML coordinate stretch K > 1 (k_max_PML): forward and gradient."""
