@@ -250,6 +250,7 @@ StepGeom make_geom(int nx1, int nx2, int ny1, int ny2, int pitch, const float *h
 {
     if (nx1 != 1 || ny1 != 1) fatal("the device kernels update the whole grid: nx1 and ny1 must be 1");
     if (NPROCX != 1 || NPROCY != 1) fatal("one GPU holds the whole domain: NPROCX = NPROCY = 1 required");
+    if (BOUNDARY != 0) fatal("BOUNDARY must be 0: periodic sides need the wavefield exchanges of src/exchange_v.c");
     StepGeom g;
     g.NX = nx2; g.NY = ny2; g.pitch = pitch;
     g.FW = FW; g.FREE_SURF = FREE_SURF; g.BOUNDARY = BOUNDARY;

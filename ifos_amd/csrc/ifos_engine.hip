@@ -215,6 +215,10 @@ static int validate(const ifos_config *c)
     if (c->FW < 0 || 2 * c->FW > c->NX || 2 * c->FW > c->NY)
         return ifos_set_error("FW (%d) must fit twice into NX and NY", c->FW), -2;
     if (!(c->DT > 0) || !(c->DH > 0)) return ifos_set_error("DT and DH must be positive"), -2;
+    // BOUNDARY 1 makes the sides periodic through the wavefield exchanges
+    // of the domain decomposition (src/exchange_v.c, src/exchange_s.c), which
+    // the one-grid engine does not have
+    if (c->BOUNDARY != 0) return ifos_set_error("BOUNDARY must be 0 (periodic sides are not supported)"), -2;
     return 0;
 }
 
