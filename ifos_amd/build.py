@@ -47,24 +47,34 @@ def _run(cmd):
     return out
 
 
-def build(verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
+def sources():
+    """Every source file the library is linked from, relative to csrc/."""
+    return list(C_SOURCES) + list(CXX_SOURCES) + [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+
+
+def build(verbose=False, obj_dir=None, lib_path=None):
+    """Compile every source and link the library.  obj_dir / lib_path default
+    to the in-tree ifos_amd/_build and ifos_amd/lib/libifos2d_hip.so; every
+    object is compiled on every call (no timestamp shortcuts)."""
+    obj_dir = obj_dir or OBJ
+    lib_path = lib_path or LIB
+    os.makedirs(obj_dir, exist_ok=True)
+    os.makedirs(os.path.dirname(lib_path), exist_ok=True)
     objs = []
     jobs = []
     for src in C_SOURCES:
-        o = os.path.join(OBJ, os.path.basename(src).replace(".c", ".o"))
+        o = os.path.join(obj_dir, os.path.basename(src).replace(".c", ".o"))
         jobs.append(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     for src in CXX_SOURCES:
-        o = os.path.join(OBJ, os.path.basename(src).replace(".cpp", ".o"))
+        o = os.path.join(obj_dir, os.path.basename(src).replace(".cpp", ".o"))
         jobs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
-        o = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o"))
+        o = os.path.join(obj_dir, os.path.basename(src).replace(".hip", ".o"))
         jobs.append([HIPCC, *HIP_FLAGS, "-c", path, "-o", o])
         objs.append(o)
     procs = [(j, subprocess.Popen(j, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)) for j in jobs]
@@ -80,11 +90,11 @@ def build(verbose=False):
             print(out)
     if failed:
         raise RuntimeError("compilation failed")
-    tmp = LIB + ".tmp"
+    tmp = lib_path + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs,
           "-Wl,--no-undefined", "-lm", "-lpthread"])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib_path)
+    return lib_path
 
 
 if __name__ == "__main__":

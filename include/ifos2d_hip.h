@@ -152,7 +152,7 @@ int  ifos_engine_fields(ifos_engine *e, float *out5);
 /* Residuals/adjoint sources from observed traces [ntr][NT] (first sample
  * zeroed as src/inseis.c:75 delivers them), optional kill mask (ntr ints).
  * The L2 residual of src/calc_res.c (LNORM 2, the norm every workload here
- * uses); the other norms of LNORM 5-8 are not provided. */
+ * uses); the other norms of LNORM 5-8 are not provided.
  * obs_on_device != 0: obs pointers are device addresses.
  * misfit[0] = L2 (FP64 sum), misfit[1] = energy of the data (FP64 sum). */
 int  ifos_engine_residual(ifos_engine *e, const float *obs_vx, const float *obs_vy, const int *kill,
