@@ -26,9 +26,15 @@
  *                          check of the whole gradient (sign/scale).
  *  - FDORDER 10 and 12: the reference's kernels for these orders drop the
  *    CPML and use different scaling (src/update_v_PML.c:466-547,
- *    src/update_s_elastic_PML.c:442-535).  This restatement applies the same
- *    CPML stencil form as orders 2-8; "parity unpinned" against the
- *    reference for these two orders, pinned by polynomial exactness only.
+ *    src/update_s_elastic_PML.c:442-535), a defect of the reference.  This
+ *    restatement applies the orders-2-8 update (CPML, DT*rip*(...)/DH, the
+ *    full stress formula), pinned bit for bit at orders 2-8, to the order
+ *    10/12 difference sums.  Those sums (orc_probe_derivs) equal the
+ *    reference's own sums at orders 10 and 12 bit for bit (read off its
+ *    kernels with DT = DH = 1 and unit materials), are exact on polynomials
+ *    of degree <= FDORDER, and meet Holberg's phase-error bound over the
+ *    design band; the free surface (orc_probe_surface) equals the
+ *    reference's at every order 2-12 bit for bit.
  */
 #include <stdint.h>
 #include <math.h>
@@ -485,15 +491,13 @@ static void shot_free(shot_t *S)
         s_;                                                    \
     })
 
-/* ---- update_v_PML  (src/update_v_PML.c:29-596, orders 2-8) --------------- */
-static void update_v(shot_t *S, int nt, int sw, int nsrc, const float *srcpos, const real *sig,
-                     const real *sig1, int sig_stride)
+/* the four stress derivatives of the velocity update at (j,i), unscaled
+ * (src/update_v_PML.c:71-160 for orders 2-8; the same sums at orders 10/12,
+ * :466-547) */
+static inline void stress_derivs(shot_t *S, int j, int i, real *psxx_x, real *psxy_x, real *psxy_y, real *psyy_y)
 {
-    const int N = S->fdoh, NX = S->P.NX, NY = S->P.NY, FW = S->P.FW;
-    const float DT = S->P.DT, DH = S->P.DH;
-    real **sxx = S->sxx, **syy = S->syy, **sxy = S->sxy, **vx = S->vx, **vy = S->vy;
-    for (int j = 1; j <= NY; j++)
-        for (int i = 1; i <= NX; i++) {
+    const int N = S->fdoh;
+    real **sxx = S->sxx, **syy = S->syy, **sxy = S->sxy;
 #define A1(m) sxx[j][i + (m)]
 #define B1(m) sxx[j][i - (m) + 1]
 #define A2(m) sxy[j][i + (m)-1]
@@ -502,10 +506,10 @@ static void update_v(shot_t *S, int nt, int sw, int nsrc, const float *srcpos, c
 #define B3(m) sxy[j - (m)][i]
 #define A4(m) syy[j + (m)][i]
 #define B4(m) syy[j - (m) + 1][i]
-            real sxx_x = DSUM(A1, B1);
-            real sxy_x = DSUM(A2, B2);
-            real sxy_y = DSUM(A3, B3);
-            real syy_y = DSUM(A4, B4);
+    *psxx_x = DSUM(A1, B1);
+    *psxy_x = DSUM(A2, B2);
+    *psxy_y = DSUM(A3, B3);
+    *psyy_y = DSUM(A4, B4);
 #undef A1
 #undef B1
 #undef A2
@@ -514,6 +518,19 @@ static void update_v(shot_t *S, int nt, int sw, int nsrc, const float *srcpos, c
 #undef B3
 #undef A4
 #undef B4
+}
+
+/* ---- update_v_PML  (src/update_v_PML.c:29-596, orders 2-8) --------------- */
+static void update_v(shot_t *S, int nt, int sw, int nsrc, const float *srcpos, const real *sig,
+                     const real *sig1, int sig_stride)
+{
+    const int NX = S->P.NX, NY = S->P.NY, FW = S->P.FW;
+    const float DT = S->P.DT, DH = S->P.DH;
+    real **vx = S->vx, **vy = S->vy;
+    for (int j = 1; j <= NY; j++)
+        for (int i = 1; i <= NX; i++) {
+            real sxx_x, sxy_x, sxy_y, syy_y;
+            stress_derivs(S, j, i, &sxx_x, &sxy_x, &sxy_y, &syy_y);
             if ((!S->P.BOUNDARY) && (i <= FW)) {
                 S->psi_sxx_x[j][i] = S->b_xh[i] * S->psi_sxx_x[j][i] + S->a_xh[i] * sxx_x;
                 sxx_x = sxx_x / S->K_xh[i] + S->psi_sxx_x[j][i];
@@ -964,6 +981,68 @@ int orc_forward(const orc_params *p, const float *vp, const float *vs, const flo
                 for (int i = 1; i <= p->NX; i++)
                     final5[(size_t)k * p->NX * p->NY + (size_t)(j - 1) * p->NX + (i - 1)] = src[k][j][i];
     }
+    shot_free(&S);
+    return 0;
+}
+
+/* ---- single-operator probes (stencil and free-surface pinning) ---------- */
+/* Padded grid g[k] of the probe layout: rows -nd+1..NY+nd, columns
+ * -nd+1..NX+nd, x fastest. */
+static void probe_load(shot_t *S, real **dst, const real *src)
+{
+    const int w = S->P.NX + 2 * S->nd;
+    for (int j = LO; j <= HIY; j++)
+        for (int i = LO; i <= HIX; i++) dst[j][i] = src[(size_t)(j - LO) * w + (size_t)(i - LO)];
+}
+static void probe_store(shot_t *S, real **src, real *dst)
+{
+    const int w = S->P.NX + 2 * S->nd;
+    for (int j = LO; j <= HIY; j++)
+        for (int i = LO; i <= HIX; i++) dst[(size_t)(j - LO) * w + (size_t)(i - LO)] = src[j][i];
+}
+
+int orc_probe_derivs(const orc_params *p, const real *fields5, real *out8)
+{
+    if (p->FDORDER < 2 || p->FDORDER > 12 || (p->FDORDER & 1)) return -1;
+    const int NX = p->NX, NY = p->NY;
+    float *one = (float *)malloc(sizeof(float) * (size_t)NX * NY);
+    for (size_t k = 0; k < (size_t)NX * NY; k++) one[k] = 1.0f;
+    orc_params q = *p;
+    q.L = 0;
+    shot_t S;
+    shot_init(&S, &q, one, one, one, NULL, NULL);
+    const size_t pl = (size_t)(NX + 2 * S.nd) * (size_t)(NY + 2 * S.nd);
+    real **f[5] = {S.vx, S.vy, S.sxx, S.syy, S.sxy};
+    for (int k = 0; k < 5; k++) probe_load(&S, f[k], fields5 + (size_t)k * pl);
+    const size_t ip = (size_t)NX * NY;
+    for (int j = 1; j <= NY; j++)
+        for (int i = 1; i <= NX; i++) {
+            real d[8];
+            stress_derivs(&S, j, i, &d[0], &d[1], &d[2], &d[3]);
+            vel_derivs(&S, j, i, 1, &d[4], &d[5], &d[6], &d[7]);
+            for (int k = 0; k < 8; k++) out8[(size_t)k * ip + (size_t)(j - 1) * NX + (i - 1)] = d[k];
+        }
+    shot_free(&S);
+    free(one);
+    return 0;
+}
+
+int orc_probe_surface(const orc_params *p, const float *vp, const float *vs, const float *rho, const float *taus,
+                      const float *taup, real *fields8, real *prow)
+{
+    if (p->FDORDER < 2 || p->FDORDER > 12 || (p->FDORDER & 1)) return -1;
+    shot_t S;
+    shot_init(&S, p, vp, vs, rho, taus, taup);
+    const size_t pl = (size_t)(p->NX + 2 * S.nd) * (size_t)(p->NY + 2 * S.nd);
+    real **f[8] = {S.vx, S.vy, S.sxx, S.syy, S.sxy, S.ux, S.uy, S.uxy};
+    for (int k = 0; k < 8; k++) probe_load(&S, f[k], fields8 + (size_t)k * pl);
+    for (int l = 1; l <= p->L; l++)
+        for (int i = 1; i <= p->NX; i++) TEN(S.pp, 1, i, l) = prow[(size_t)(l - 1) * p->NX + (i - 1)];
+    if (p->L) surface_visc(&S);
+    else surface_elastic(&S);
+    for (int k = 0; k < 8; k++) probe_store(&S, f[k], fields8 + (size_t)k * pl);
+    for (int l = 1; l <= p->L; l++)
+        for (int i = 1; i <= p->NX; i++) prow[(size_t)(l - 1) * p->NX + (i - 1)] = TEN(S.pp, 1, i, l);
     shot_free(&S);
     return 0;
 }

@@ -35,7 +35,7 @@ extern "C" {
  * (src/globvar.h:11-150). */
 typedef struct orc_params {
     int NX, NY;            /* interior grid size (x fastest)              */
-    int FDORDER;           /* 2,4,6,8 (10,12: see ifos_oracle.c header)   */
+    int FDORDER;           /* 2-12 (10,12: see ifos_oracle.c header)      */
     int MAXRELERROR;       /* 0..4 row of the Holberg table               */
     int FW;                /* CPML width in cells                          */
     int FREE_SURF;         /* 1: stress-imaging free surface at j=1        */
@@ -90,6 +90,22 @@ int orc_forward(const orc_params *p, const float *vp, const float *vs,
                 const float *rho, const float *taus, const float *taup,
                 int nsrc, const float *srcpos, int ntr, const int *recpos,
                 real *seis_vx, real *seis_vy, real *final5);
+
+/* ---- single-operator probes (pinning the stencils and the free surface) -
+ * Probe grids are padded, rows -nd+1..NY+nd and columns -nd+1..NX+nd with
+ * nd = FDORDER/2+1, x fastest, (NY+2nd)*(NX+2nd) values each.
+ * orc_probe_derivs: fields5 = vx, vy, sxx, syy, sxy; out8 = [8][NY][NX] the
+ * unscaled staggered differences (grid spacing 1, no CPML) that the
+ * velocity update forms from the stresses, sxx_x, sxy_x, sxy_y, syy_y
+ * (src/update_v_PML.c), then those the stress update forms from the
+ * velocities, vxx, vyx, vxy, vyy (src/update_s_elastic_PML.c).  Orders 2-12.
+ * orc_probe_surface: one application of surface_elastic_PML (L = 0) or
+ * surface_PML (L > 0) to fields8 = vx, vy, sxx, syy, sxy, ux, uy, uxy (in
+ * and out), with zero CPML memory; prow = [L][NX], the memory variables
+ * p_l of row 1 (in and out). */
+int orc_probe_derivs(const orc_params *p, const real *fields5, real *out8);
+int orc_probe_surface(const orc_params *p, const float *vp, const float *vs, const float *rho, const float *taus,
+                      const float *taup, real *fields8, real *prow);
 
 /* Forward + residual + adjoint + zero-lag correlation + chain rule for one
  * shot.  obs_* are observed traces [ntr][NT] exactly as delivered by

@@ -15,11 +15,11 @@
  *
  * All arithmetic is FP32 in the reference's order: sums run over x in the
  * outer loop and y in the inner loop where the reference does so.  Pinned
- * against the reference's own compiled files (oracle/_ref) where those
- * build (calc_opt_step, lbfgs_core, calc_mat_change_test) and by closed
- * forms (tests/test_optimizer.py); PCG is pinned by numpy restatements of
- * the Polak-Ribiere formula only ("parity unpinned" against the compiled
- * reference, whose PCG reads and writes its state through files).
+ * against the reference's own compiled files (oracle/_ref): calc_opt_step,
+ * lbfgs_core, calc_mat_change_test, and PCG itself, driven through the
+ * state files it writes and reads over several iterations
+ * (tests/test_oracle_pins.py), bit for bit; and by closed forms
+ * (tests/test_optimizer.py).
  */
 #include <math.h>
 #include <stdlib.h>

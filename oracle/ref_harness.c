@@ -697,3 +697,171 @@ void ref_spat_filt(int nx, int ny, int size, int first_row, const char *prefix, 
     spat_filt(w, 1, 1);
     harness_grid_out(nx, ny, w, g);
 }
+
+/* ---------------------------------------------------------------------------
+ * Single-operator probes: the reference's own update_v_PML,
+ * update_s_elastic_PML, surface_elastic_PML and surface_PML applied once to
+ * given fields.  Probe grids are padded (rows -nd+1..NY+nd, columns
+ * -nd+1..NX+nd, x fastest), as orc_probe_* in ifos_oracle.h.
+ * ------------------------------------------------------------------------- */
+static void probe_in(float **a, const float *src, int nd)
+{
+    const int w = NX + 2 * nd;
+    for (int j = -nd + 1; j <= NY + nd; j++)
+        for (int i = -nd + 1; i <= NX + nd; i++) a[j][i] = src[(size_t)(j + nd - 1) * w + (size_t)(i + nd - 1)];
+}
+static void probe_out(float **a, float *dst, int nd)
+{
+    const int w = NX + 2 * nd;
+    for (int j = -nd + 1; j <= NY + nd; j++)
+        for (int i = -nd + 1; i <= NX + nd; i++) dst[(size_t)(j + nd - 1) * w + (size_t)(i + nd - 1)] = a[j][i];
+}
+static void probe_zero(float **a, int nd)
+{
+    for (int j = -nd + 1; j <= NY + nd; j++)
+        for (int i = -nd + 1; i <= NX + nd; i++) a[j][i] = 0.0f;
+}
+
+/* The staggered difference sums of the reference kernels at any FDORDER,
+ * read off single updates.  DT = DH = 1 and unit materials make every update
+ * return one sum, unscaled and unrounded beyond the sum itself:
+ *  - update_v_PML, vx = vy = 0, rho = 1 (rip = rjp = 1): with sxy = 0 it
+ *    leaves vx = sxx_x, vy = syy_y; with sxx = syy = 0, vx = sxy_y and
+ *    vy = sxy_x (the zero terms add exact zeros, also in the single sums of
+ *    orders 10/12, src/update_v_PML.c:466-547, whose DT*DT/DH factor is 1);
+ *  - update_s_elastic_PML with vs = 0, vp = rho = 1 (f = 0, g = 1): with
+ *    vy = 0 it leaves ux = vxx, with vx = 0 uy = vyy;
+ *  - the same with vs = vp = rho = 1 (uipjp = 1): with vx = 0 it leaves
+ *    sxy = vyx, with vy = 0 sxy = vxy.
+ * BOUNDARY = 1 and FREE_SURF = 1 keep the CPML off every cell but the bottom
+ * strip j > NY-FW (orders 2-8 apply it there; orders 10/12 have none), which
+ * the caller leaves out.  out8 as orc_probe_derivs. */
+void ref_probe_derivs(const int *ip, const float *fp, const float *fields5, float *out8)
+{
+    harness_setup(ip, fp);
+    DT = 1.0f; DH = 1.0f; BOUNDARY = 1; FREE_SURF = 1; L = 0;
+    const int nd = FDORDER / 2 + 1;
+    const size_t pl = (size_t)(NX + 2 * nd) * (size_t)(NY + 2 * nd), ipl = (size_t)NX * NY;
+    float *ones = (float *)malloc(sizeof(float) * ipl), *zeros = (float *)calloc(ipl, sizeof(float));
+    for (size_t k = 0; k < ipl; k++) ones[k] = 1.0f;
+    const float *VX = fields5, *VY = fields5 + pl, *SXX = fields5 + 2 * pl, *SYY = fields5 + 3 * pl,
+                *SXY = fields5 + 4 * pl;
+    for (int pass = 0; pass < 2; pass++) {
+        /* pass 0: vs = 0 (ux, uy read the normal derivatives); pass 1: vs = 1 */
+        run_t R;
+        run_init(&R, ones, pass ? ones : zeros, ones, NULL, NULL, NULL);
+        pml_t *P = &R.P;
+        for (int k = 0; k < 6; k++) {
+            run_zero(&R);
+            if (pass == 0 && k == 0) { probe_in(R.psxx, SXX, nd); probe_in(R.psyy, SYY, nd); }
+            else if (pass == 0 && k == 1) { probe_in(R.psxy, SXY, nd); }
+            else if (pass == 0 && k == 2) { probe_in(R.pvx, VX, nd); }
+            else if (pass == 0 && k == 3) { probe_in(R.pvy, VY, nd); }
+            else if (pass == 1 && k == 4) { probe_in(R.pvy, VY, nd); }
+            else if (pass == 1 && k == 5) { probe_in(R.pvx, VX, nd); }
+            else continue;
+            if (k <= 1)
+                update_v_PML(1, NX, 1, NY, 1, R.pvx, R.pvxp1, R.pvxm1, R.pvy, R.pvyp1, R.pvym1, R.uttx, R.utty,
+                             R.psxx, R.psyy, R.psxy, R.prip, R.prjp, NULL, NULL, NULL, 0, NULL, R.hc, 0, 0,
+                             P->K_x, P->a_x, P->b_x, P->K_xh, P->a_xh, P->b_xh, P->K_y, P->a_y, P->b_y, P->K_yh,
+                             P->a_yh, P->b_yh, R.psi_sxx_x, R.psi_syy_y, R.psi_sxy_y, R.psi_sxy_x);
+            else
+                update_s_elastic_PML(1, NX, 1, NY, R.pvx, R.pvy, R.ux, R.uy, R.uxy, R.uyx, R.psxx, R.psyy,
+                                     R.psxy, R.ppi, R.pu, R.puipjp, NULL, R.prho, R.hc, 0, P->K_x, P->a_x, P->b_x,
+                                     P->K_xh, P->a_xh, P->b_xh, P->K_y, P->a_y, P->b_y, P->K_yh, P->a_yh, P->b_yh,
+                                     R.psi_vxx, R.psi_vyy, R.psi_vxy, R.psi_vyx);
+            /* out8 order: sxx_x, sxy_x, sxy_y, syy_y, vxx, vyx, vxy, vyy */
+            float **src0 = NULL, **src1 = NULL;
+            int o0 = -1, o1 = -1;
+            if (k == 0) { src0 = R.pvx; o0 = 0; src1 = R.pvy; o1 = 3; }
+            if (k == 1) { src0 = R.pvx; o0 = 2; src1 = R.pvy; o1 = 1; }
+            if (k == 2) { src0 = R.ux; o0 = 4; }
+            if (k == 3) { src0 = R.uy; o0 = 7; }
+            if (k == 4) { src0 = R.psxy; o0 = 5; }
+            if (k == 5) { src0 = R.psxy; o0 = 6; }
+            copy_interior(src0, out8 + (size_t)o0 * ipl);
+            if (src1) copy_interior(src1, out8 + (size_t)o1 * ipl);
+        }
+    }
+    free(ones);
+    free(zeros);
+}
+
+/* One surface_elastic_PML (L = 0) or surface_PML (L > 0) call of the
+ * reference on the given fields (fields8 = vx, vy, sxx, syy, sxy, ux, uy,
+ * uxy, padded, in and out) with zero CPML memory; prow [L][NX] holds the
+ * memory variables p_l of row 1 (in and out). */
+void ref_probe_surface(const int *ip, const float *fp, const float *FLv, const float *vp, const float *vs,
+                       const float *rho, const float *taus, const float *taup, float *fields8, float *prow)
+{
+    harness_setup(ip, fp);
+    run_t R;
+    run_init(&R, vp, vs, rho, taus, taup, FLv);
+    run_zero(&R);
+    const int nd = R.nd;
+    const size_t pl = (size_t)(NX + 2 * nd) * (size_t)(NY + 2 * nd);
+    float **f[8] = {R.pvx, R.pvy, R.psxx, R.psyy, R.psxy, R.ux, R.uy, R.uxy};
+    for (int k = 0; k < 8; k++) probe_in(f[k], fields8 + (size_t)k * pl, nd);
+    for (int l = 1; l <= L; l++)
+        for (int i = 1; i <= NX; i++) R.pp[1][i][l] = prow[(size_t)(l - 1) * NX + (i - 1)];
+    pml_t *P = &R.P;
+    if (L)
+        surface_PML(1, R.pvx, R.pvy, R.psxx, R.psyy, R.psxy, R.psyz, R.pp, R.pq, R.ppi, R.pu, R.prho, R.ptaup,
+                    R.ptaus, R.etajm, R.peta, R.hc, P->K_x, P->a_x, P->b_x, R.psi_vxxs, R.ux, R.uy, R.uxy, R.uyz,
+                    R.psxz, R.uxz);
+    else
+        surface_elastic_PML(1, R.pvx, R.pvy, R.psxx, R.psyy, R.psxy, R.psyz, R.ppi, R.pu, R.prho, R.hc, P->K_x,
+                            P->a_x, P->b_x, R.psi_vxxs, R.ux, R.uy, R.uxy, R.uyz, R.psxz, R.uxz);
+    for (int k = 0; k < 8; k++) probe_out(f[k], fields8 + (size_t)k * pl, nd);
+    for (int l = 1; l <= L; l++)
+        for (int i = 1; i <= NX; i++) prow[(size_t)(l - 1) * NX + (i - 1)] = R.pp[1][i][l];
+}
+
+/* ---------------------------------------------------------------------------
+ * PCG() (src/PCG.c:27) called unmodified for iterations 1..niter on the
+ * gradients grads [niter][3][NY][NX] (Vp, Vs, rho), with every taper and
+ * filter off.  Its state between iterations goes through the files it writes
+ * and reads (<prefix>_p.old.0.0, _c.old.0.0, ... for each parameter).
+ * dirs [niter][3][NY][NX] receives waveconv after each call (the search
+ * direction), gradps the scaled gradients gradp.
+ * ------------------------------------------------------------------------- */
+void ref_pcg(int nx, int ny, int niter, int pcg_iter_start, const float *C3, const char *prefix,
+             const float *grads, float *dirs, float *gradps)
+{
+    harness_setup_opt(nx, ny);
+    IDX = IDY = 1;
+    SPATFILTER = 0; GRAD_FILTER = 0; FORWARD_ONLY = 0; ACOUSTIC = 0; VERBOSE = 0;
+    SWS_TAPER_GRAD_VERT = SWS_TAPER_GRAD_HOR = SWS_TAPER_GRAD_SOURCES = SWS_TAPER_FILE = 0;
+    snprintf(JACOBIAN, STRING_SIZE, "%s", prefix);
+    const size_t pl = (size_t)nx * ny;
+    float **w[3], **gp[3];
+    for (int k = 0; k < 3; k++) {
+        w[k] = matrix(1, ny, 1, nx);
+        gp[k] = matrix(1, ny, 1, nx);
+    }
+    float **tc = matrix(1, ny, 1, nx);
+    float **sp = matrix(1, 8, 1, 1);
+    int **rp = imatrix(1, 3, 1, 1);
+    for (int it = 1; it <= niter; it++) {
+        for (int k = 0; k < 3; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++)
+                    w[k][j][i] = grads[((size_t)(it - 1) * 3 + k) * pl + (size_t)(j - 1) * nx + (i - 1)];
+        PCG(w[0], tc, 1, sp, rp, 1, it, C3[0], gp[0], -1, w[1], C3[1], gp[1], w[2], C3[2], gp[2], 0.0f, 0.0f,
+            pcg_iter_start);
+        for (int k = 0; k < 3; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) {
+                    size_t e = ((size_t)(it - 1) * 3 + k) * pl + (size_t)(j - 1) * nx + (i - 1);
+                    dirs[e] = w[k][j][i];
+                    gradps[e] = gp[k][j][i];
+                }
+    }
+    for (int k = 0; k < 3; k++) {
+        free_matrix(w[k], 1, ny, 1, nx);
+        free_matrix(gp[k], 1, ny, 1, nx);
+    }
+    free_matrix(tc, 1, ny, 1, nx);
+    free_matrix(sp, 1, 8, 1, 1);
+    free_imatrix(rp, 1, 3, 1, 1);
+}

@@ -429,3 +429,59 @@ def spat_filt(g, size, first_row, which="oracle"):
     out = np.empty((ny, nx), np.float32)
     oracle().orc_spat_filt(nx, ny, size, first_row, _ptr(_f32(g)), _ptr(out))
     return out
+
+
+# --------------------------------------------------- single-operator probes --
+def probe_shape(p):
+    nd = p.FDORDER // 2 + 1
+    return nd, (p.NY + 2 * nd, p.NX + 2 * nd)
+
+
+def probe_derivs(p, fields5, which="oracle", fp64=False):
+    """The eight unscaled staggered difference sums (sxx_x, sxy_x, sxy_y,
+    syy_y of the velocity update; vxx, vyx, vxy, vyy of the stress update) of
+    padded fields vx, vy, sxx, syy, sxy; [8][NY][NX]."""
+    nd, shp = probe_shape(p)
+    if which == "oracle":
+        dt = np.float64 if fp64 else np.float32
+        ty = C.c_double if fp64 else C.c_float
+        f = np.ascontiguousarray(fields5, dtype=dt).reshape(5, *shp)
+        out = np.zeros((8, p.NY, p.NX), dt)
+        rc = oracle(fp64).orc_probe_derivs(C.byref(p), _ptr(f, ty), _ptr(out, ty))
+        assert rc == 0
+        return out
+    f = _f32(fields5).reshape(5, *shp)
+    out = np.zeros((8, p.NY, p.NX), np.float32)
+    ip, fp, _ = _ip_fp(p)
+    ref().ref_probe_derivs(ip, fp, _ptr(f), _ptr(out))
+    return out
+
+
+def probe_surface(p, model, fields8, prow, which="oracle"):
+    nd, shp = probe_shape(p)
+    vp, vs, rho = (_f32(model[k]) for k in ("vp", "vs", "rho"))
+    taus = _f32(model["taus"]) if p.L else None
+    taup = _f32(model["taup"]) if p.L else None
+    f = _f32(fields8).reshape(8, *shp).copy()
+    pr = _f32(prow).reshape(max(p.L, 1), p.NX).copy()
+    if which == "oracle":
+        rc = oracle().orc_probe_surface(C.byref(p), _ptr(vp), _ptr(vs), _ptr(rho), _ptr(taus), _ptr(taup),
+                                        _ptr(f), _ptr(pr))
+        assert rc == 0
+    else:
+        ip, fp, fl = _ip_fp(p)
+        ref().ref_probe_surface(ip, fp, fl, _ptr(vp), _ptr(vs), _ptr(rho), _ptr(taus), _ptr(taup), _ptr(f), _ptr(pr))
+    return f, pr
+
+
+def ref_pcg(grads, C3, start, prefix):
+    """The reference's PCG() over iterations 1..len(grads); grads
+    [niter][3][NY][NX].  Returns (directions, scaled gradients), same shape."""
+    g = _f32(grads)
+    n, _, ny, nx = g.shape
+    dirs = np.zeros_like(g)
+    gps = np.zeros_like(g)
+    c3 = _f32(np.asarray(C3))
+    ref().ref_pcg(C.c_int(nx), C.c_int(ny), C.c_int(n), C.c_int(start), _ptr(c3), prefix.encode(), _ptr(g),
+                  _ptr(dirs), _ptr(gps))
+    return dirs, gps

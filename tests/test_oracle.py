@@ -153,6 +153,23 @@ def test_forward_matches_reference(fd, L, vel, fs, st):
 
 
 @needs_ref
+@pytest.mark.parametrize("fd,L,vel,fs", list(itertools.product([2, 4, 6, 8], [1, 3], [0, 1], [0, 1])))
+def test_forward_matches_reference_one_and_three_mechanisms(fd, L, vel, fs):
+    """L=1 and L=3 relaxation mechanisms (FL 10/40/70 Hz): the reference's
+    update_s_visc_PML / surface_PML loop over any L (src/update_s_visc_PML.c:28,
+    src/surface_PML.c:25), the memory-variable sums and the GSLS
+    coefficients of src/prepare_update_s.c:21 for each."""
+    w = workloads.small_case(seed=fd * 10 + L + 5, NX=44, NY=32, NT=90, FDORDER=fd, L=L, VELOCITY=vel, FREE_SURF=fs,
+                             FW=8, DTINV=3)
+    p = params_of(w)
+    s = w.shots[0]
+    a = O.forward(p, w.true, s.srcpos, s.recpos)
+    b = O.forward(p, w.true, s.srcpos, s.recpos, which="ref")
+    for x, y, n in zip(a, b, ("vx", "vy", "fields")):
+        np.testing.assert_array_equal(x, y, err_msg=n)
+
+
+@needs_ref
 @pytest.mark.parametrize("adj", [2, 3])
 def test_adjoint_type_matches_reference(adj):
     """ADJOINT_TYPE 2 (vy residuals only) and 3 (vx only): the injection of
@@ -201,7 +218,7 @@ def test_stretched_pml_matches_reference(kmax, npower):
 
 
 @needs_ref
-@pytest.mark.parametrize("fd,L,vel", list(itertools.product([4, 8], [0, 2], [0, 1])))
+@pytest.mark.parametrize("fd,L,vel", list(itertools.product([4, 8], [0, 1, 2, 3], [0, 1])))
 def test_gradient_matches_reference(fd, L, vel):
     w = workloads.small_case(seed=3 + fd + L, NX=48, NY=36, NT=120, FDORDER=fd, L=L, VELOCITY=vel, DTINV=3, FW=8)
     p = params_of(w)
