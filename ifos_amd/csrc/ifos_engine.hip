@@ -865,6 +865,13 @@ extern "C" int ifos_engine_residual(ifos_engine *E, const float *obs_vx, const f
     const int ntr = E->ntr;
     const size_t n = (size_t)ntr * c.NT;
     const float *ovx = obs_vx, *ovy = obs_vy;
+    // an active component (ADJOINT_TYPE 1: both, 2: vy, 3: vx) needs its
+    // observed traces; a missing one would leave the residual reading the
+    // engine's copy from an earlier call
+    if (ntr > 0 && !obs_vx && (c.ADJOINT_TYPE == 1 || c.ADJOINT_TYPE == 3))
+        return ifos_set_error("observed vx traces missing (ADJOINT_TYPE %d uses them)", c.ADJOINT_TYPE), -2;
+    if (ntr > 0 && !obs_vy && (c.ADJOINT_TYPE == 1 || c.ADJOINT_TYPE == 2))
+        return ifos_set_error("observed vy traces missing (ADJOINT_TYPE %d uses them)", c.ADJOINT_TYPE), -2;
     if (!obs_on_device) {
         if (obs_vx) HIP_TRY(hipMemcpyAsync(seis_buf(E, 4), obs_vx, n * sizeof(float), hipMemcpyHostToDevice, E->stream));
         if (obs_vy) HIP_TRY(hipMemcpyAsync(seis_buf(E, 5), obs_vy, n * sizeof(float), hipMemcpyHostToDevice, E->stream));

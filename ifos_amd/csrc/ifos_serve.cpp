@@ -104,7 +104,12 @@ struct ifos_server {
                 st.engines_reused++;
                 return p.get();
             }
-        // evict idle engines, least recently used first, until r fits
+        // evict idle engines, least recently used first, until r fits; when
+        // even evicting every idle engine would not make room, keep them all
+        size_t idle = 0;
+        for (auto &p : pool)
+            if (!p->busy && p->eng) idle += p->bytes;
+        if (in_use - idle + r.bytes > budget) return nullptr;
         while (in_use + r.bytes > budget) {
             Resident *lru = nullptr;
             for (auto &p : pool)
