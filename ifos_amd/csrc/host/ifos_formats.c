@@ -346,3 +346,183 @@ int ifos_trkill_offsets(int ntr_glob, const int *recpos, int nsrc_glob, const fl
     }
     return 0;
 }
+
+/* ---- observed traces as inseis reads them (src/inseis.c:70-83) ---------- */
+
+int ifos_su_read_observed(const char *path, int ntr, int ns, float *data)
+{
+    int got = 0;
+    const int rc = ifos_su_read(path, ntr, ns, data, &got);
+    if (rc) return rc;
+    if (got != ntr) return ifos_set_error("'%s' holds %d traces, expected %d", path, got, ntr), -2;
+    /* section[t][1] = 0, section[t][k] = sample k of the file for k = 2..ns */
+    for (int t = 0; t < ntr; t++) data[(size_t)t * ns] = 0.0f;
+    return 0;
+}
+
+/* ---- source and receiver positions (src/sources.c:27, src/receiver.c:27) ---- */
+
+static int iround_f(float x) { return (int)floor(x + 0.5); }
+
+int ifos_sources_read(const char *path, int SOURCE_TYPE, int max_src, float *srcpos, int *nsrc_out, float *fc_max)
+{
+    FILE *fp = fopen(path, "r");
+    if (!fp) return ifos_set_error(" Source file could not be opened !"), -2;
+    int nsrc = 0;
+    if (fscanf(fp, "%d", &nsrc) != 1 || nsrc < 0) {
+        fclose(fp);
+        return ifos_set_error("'%s': no source count on the first line", path), -2;
+    }
+    if (nsrc > max_src) {
+        fclose(fp);
+        return ifos_set_error("'%s' lists %d sources, room for %d", path, nsrc, max_src), -2;
+    }
+    rewind(fp);
+    char line[256];
+    if (!fgets(line, 255, fp)) line[0] = 0; /* the count line */
+    float fc = 0.0f;
+    for (int l = 0; l < nsrc; l++) {
+        float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (!fgets(line, 255, fp)) line[0] = 0;
+        /* columns: x, z, y, tshift, fc, amp, azimuth, type (z is read and
+         * dropped); missing trailing columns take the defaults of the
+         * reference's fall-through switch */
+        int n = sscanf(line, "%f%f%f%f%f%f%f%f", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7]);
+        if (n < 0) n = 0;
+        if (n < 6) {
+            fclose(fp);
+            return ifos_set_error("Missing parameter in SOURCE_FILE! (source %d of '%s')", l + 1, path), -2;
+        }
+        if (n < 7) v[6] = 0.0f;
+        if (n < 8) v[7] = (float)SOURCE_TYPE;
+        float *r = srcpos + (size_t)l * IFOS_SRC_STRIDE;
+        r[0] = v[0];        /* x (m) */
+        r[1] = v[2];        /* y (m) */
+        r[2] = 0.0f;
+        r[3] = v[3];        /* tshift */
+        r[4] = v[4];        /* fc */
+        r[5] = v[5];        /* amp */
+        r[6] = v[6];        /* azimuth */
+        r[7] = v[7];        /* type */
+        fc = r[4];
+    }
+    fclose(fp);
+    for (int l = 0; l < nsrc; l++)
+        if (srcpos[(size_t)l * IFOS_SRC_STRIDE + 4] > fc) fc = srcpos[(size_t)l * IFOS_SRC_STRIDE + 4];
+    *nsrc_out = nsrc;
+    if (fc_max) *fc_max = fc;
+    return 0;
+}
+
+int ifos_sources_to_grid(int nsrc, const float *srcpos_m, float DH, int NX, int NY, float *srcpos, int *nsrc_out)
+{
+    int k = 0;
+    for (int l = 0; l < nsrc; l++) {
+        const float *s = srcpos_m + (size_t)l * IFOS_SRC_STRIDE;
+        const int ix = iround_f(s[0] / DH), iy = iround_f(s[1] / DH);
+        /* one rank covering the grid: POS = (0,0), IENDX = NX, IENDY = NY;
+         * C division and remainder truncate towards zero */
+        if ((ix - 1) / NX != 0 || (iy - 1) / NY != 0) continue;
+        float *d = srcpos + (size_t)k * IFOS_SRC_STRIDE;
+        d[0] = (float)(((ix - 1) % NX) + 1);
+        d[1] = (float)(((iy - 1) % NY) + 1);
+        d[2] = 0.0f;
+        for (int c = 3; c < 8; c++) d[c] = s[c];
+        k++;
+    }
+    *nsrc_out = k;
+    return 0;
+}
+
+int ifos_receivers_read(const char *path, float DH, float refrec_x, float refrec_y, int max_tr, int *recpos,
+                        int *ntr_out)
+{
+    FILE *fp = fopen(path, "r");
+    if (!fp) return ifos_set_error(" Receiver file could not be opened !"), -2;
+    int n = 0, c;
+    while ((c = fgetc(fp)) != EOF)
+        if (c == '\n') ++n;
+    rewind(fp);
+    int *pos = (int *)malloc(sizeof(int) * 2 * (size_t)(n > 0 ? n : 1));
+    if (!pos) {
+        fclose(fp);
+        return ifos_set_error("out of host memory"), -1;
+    }
+    for (int t = 0; t < n; t++) {
+        float x = 0.0f, y = 0.0f;
+        if (fscanf(fp, "%f%f\n", &x, &y) != 2) {
+            free(pos);
+            fclose(fp);
+            return ifos_set_error("'%s': receiver line %d unreadable", path, t + 1), -2;
+        }
+        pos[2 * t] = iround_f((x + refrec_x) / DH);
+        pos[2 * t + 1] = iround_f((y + refrec_y) / DH);
+    }
+    fclose(fp);
+    /* a receiver on the grid point of an earlier one is dropped; entries are
+     * kept while their x index is positive (the reference's duplicate
+     * marker is a negative x index) */
+    int flag = 0;
+    for (int a = 0; a < n - 1; a++)
+        for (int b = a + 1; b < n; b++)
+            if (pos[2 * a] == pos[2 * b] && pos[2 * a + 1] == pos[2 * b + 1]) pos[2 * b] = -(++flag);
+    int k = 0;
+    for (int t = 0; t < n; t++)
+        if (pos[2 * t] > 0) {
+            if (k == max_tr) {
+                free(pos);
+                return ifos_set_error("'%s' lists more than %d receivers", path, max_tr), -2;
+            }
+            recpos[2 * k] = pos[2 * t];
+            recpos[2 * k + 1] = pos[2 * t + 1];
+            k++;
+        }
+    free(pos);
+    *ntr_out = k;
+    return 0;
+}
+
+int ifos_receivers_line(float DH, float XREC1, float YREC1, float XREC2, float YREC2, int NGEOPH, int max_tr,
+                        int *recpos, int *ntr_out)
+{
+    if (NGEOPH < 1) return ifos_set_error("NGEOPH must be positive"), -2;
+    const int x1 = iround_f(XREC1 / DH), y1 = iround_f(YREC1 / DH), x2 = iround_f(XREC2 / DH),
+              y2 = iround_f(YREC2 / DH);
+    int ntr;
+    /* nzrec1 = nzrec2 = 1 in the reference: horizontal unless the line is
+     * steeper than 45 degrees (src/receiver.c:172-204); integer slopes */
+    if (abs(y2 - y1) <= abs(x2 - x1)) {
+        if (x2 == x1) return ifos_set_error("receiver line of one point"), -2;
+        ntr = (x2 - x1) / NGEOPH + 1;
+        if (ntr > max_tr) return ifos_set_error("%d receivers, room for %d", ntr, max_tr), -2;
+        for (int x = x1; x <= x2; x += NGEOPH) {
+            const int itr = (x - x1) / NGEOPH;
+            recpos[2 * itr] = x;
+            recpos[2 * itr + 1] = y1 + ((y2 - y1) / (x2 - x1) * (x - x1));
+        }
+    } else {
+        ntr = (y2 - y1) / NGEOPH + 1;
+        if (ntr > max_tr) return ifos_set_error("%d receivers, room for %d", ntr, max_tr), -2;
+        for (int y = y1; y <= y2; y += NGEOPH) {
+            const int itr = (y - y1) / NGEOPH;
+            recpos[2 * itr] = x1 + ((x2 - x1) / (y2 - y1) * (y - y1));
+            recpos[2 * itr + 1] = y;
+        }
+    }
+    *ntr_out = ntr > 0 ? ntr : 0;
+    return 0;
+}
+
+/* ---- gradient and model dumps (src/matrix_operations.c:24) ------------- */
+
+int ifos_matrix_dump(const char *path_name, int NX, int NY, const float *m)
+{
+    size_t n = strlen(path_name);
+    char *p = (char *)malloc(n + 5);
+    if (!p) return ifos_set_error("out of host memory"), -1;
+    memcpy(p, path_name, n);
+    memcpy(p + n, ".bin", 5);
+    const int rc = ifos_model_write(p, NX, NY, m);
+    free(p);
+    return rc;
+}

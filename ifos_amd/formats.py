@@ -99,3 +99,70 @@ def trkill_offsets(recpos, srcpos, DH, lower, upper, table=None):
                                  sp.ctypes.data_as(C.POINTER(C.c_float)), DH, lower, upper,
                                  t.ctypes.data_as(C.POINTER(C.c_int))))
     return t
+
+
+def read_su_observed(path, ntr, ns):
+    """Observed traces as inseis reads them: sample 1 of each trace zeroed."""
+    L = _L()
+    L.ifos_su_read_observed.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    buf = np.zeros((ntr, ns), np.float32)
+    _check(L.ifos_su_read_observed(path.encode(), ntr, ns, buf.ctypes.data_as(C.POINTER(C.c_float))))
+    return buf
+
+
+def read_sources(path, source_type=1, max_src=100000):
+    """SOURCE_FILE rows [nsrc][8] (x, y in metres, 0, tshift, fc, amp,
+    azimuth, type) and the largest fc."""
+    L = _L()
+    L.ifos_sources_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_float)]
+    buf = np.zeros((max_src, 8), np.float32)
+    n, fc = C.c_int(), C.c_float()
+    _check(L.ifos_sources_read(path.encode(), source_type, max_src, buf.ctypes.data_as(C.POINTER(C.c_float)),
+                               C.byref(n), C.byref(fc)))
+    return buf[:n.value].copy(), fc.value
+
+
+def sources_to_grid(srcpos_m, DH, NX, NY):
+    """Grid-index source rows (splitsrc on one rank)."""
+    L = _L()
+    L.ifos_sources_to_grid.argtypes = [C.c_int, C.POINTER(C.c_float), C.c_float, C.c_int, C.c_int,
+                                       C.POINTER(C.c_float), C.POINTER(C.c_int)]
+    sp = np.ascontiguousarray(srcpos_m, np.float32).reshape(-1, 8)
+    out = np.zeros_like(sp)
+    n = C.c_int()
+    _check(L.ifos_sources_to_grid(len(sp), sp.ctypes.data_as(C.POINTER(C.c_float)), DH, NX, NY,
+                                  out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(n)))
+    return out[:n.value].copy()
+
+
+def read_receivers(path, DH, refrec=(0.0, 0.0), max_tr=100000):
+    """REC_FILE receiver grid indices [ntr][2] (READREC 1)."""
+    L = _L()
+    L.ifos_receivers_read.argtypes = [C.c_char_p, C.c_float, C.c_float, C.c_float, C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]
+    buf = np.zeros((max_tr, 2), np.int32)
+    n = C.c_int()
+    _check(L.ifos_receivers_read(path.encode(), DH, refrec[0], refrec[1], max_tr,
+                                 buf.ctypes.data_as(C.POINTER(C.c_int)), C.byref(n)))
+    return buf[:n.value].copy()
+
+
+def receiver_line(DH, XREC1, YREC1, XREC2, YREC2, NGEOPH, max_tr=100000):
+    """A straight receiver line (READREC 0), grid indices [ntr][2]."""
+    L = _L()
+    L.ifos_receivers_line.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    buf = np.zeros((max_tr, 2), np.int32)
+    n = C.c_int()
+    _check(L.ifos_receivers_line(DH, XREC1, YREC1, XREC2, YREC2, NGEOPH, max_tr,
+                                 buf.ctypes.data_as(C.POINTER(C.c_int)), C.byref(n)))
+    return buf[:n.value].copy()
+
+
+def dump_matrix(path_name, m):
+    """write_matrix_disk: m [NY][NX] to <path_name>.bin, x-outer."""
+    L = _L()
+    L.ifos_matrix_dump.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    m = np.ascontiguousarray(m, np.float32)
+    _check(L.ifos_matrix_dump(path_name.encode(), m.shape[1], m.shape[0], m.ctypes.data_as(C.POINTER(C.c_float))))

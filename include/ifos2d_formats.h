@@ -61,6 +61,41 @@ int ifos_su_write(const char *path, int ntr, int ns, const float *data, const in
  * sample count differs from ns. */
 int ifos_su_read(const char *path, int max_tr, int ns, float *data, int *ntr_out);
 
+/* replaces inseis() src/inseis.c:24 (SEIS_FORMAT 1): the ntr traces of an SU
+ * file as observed data -- sample 1 of every trace is set to zero, samples
+ * 2..ns are the file's (src/inseis.c:75-83); fails unless the file holds
+ * exactly ntr traces of ns samples. */
+int ifos_su_read_observed(const char *path, int ntr, int ns, float *data);
+
+/* replaces sources() src/sources.c:27 with SRCREC 1: SOURCE_FILE's first
+ * line holds the count; each further line x, z, y, tshift, fc, amp
+ * [, azimuth [, type]] (z dropped, azimuth 0 and type SOURCE_TYPE when
+ * absent; fewer than six values fail as there).  srcpos [max_src][8] rows
+ * x (m), y (m), 0, tshift, fc, amp, azimuth, type; *fc_max = the largest fc
+ * (the reference's TS = 1/fc_max). */
+int ifos_sources_read(const char *path, int SOURCE_TYPE, int max_src, float *srcpos, int *nsrc_out, float *fc_max);
+/* replaces splitsrc() src/splitsrc.c:25 on one rank: grid indices
+ * iround(x/DH), iround(y/DH) in the rows' first two columns (srcpos rows of
+ * IFOS_SRC_STRIDE, the form ifos_engine_set_shot takes); sources whose
+ * indices fall outside the rank's truncating-division test are dropped. */
+int ifos_sources_to_grid(int nsrc, const float *srcpos_m, float DH, int NX, int NY, float *srcpos, int *nsrc_out);
+/* replaces receiver() src/receiver.c:27 with READREC 1: one "x y" pair
+ * (metres) per line of REC_FILE, grid indices iround((x + REFREC[1])/DH),
+ * iround((y + REFREC[2])/DH); a receiver on the grid point of an earlier
+ * one is dropped.  recpos [max_tr][2]. */
+int ifos_receivers_read(const char *path, float DH, float refrec_x, float refrec_y, int max_tr, int *recpos,
+                        int *ntr_out);
+/* receiver() with READREC 0: the straight line from (XREC1, YREC1) to
+ * (XREC2, YREC2) every NGEOPH grid points, horizontal unless steeper than
+ * 45 degrees, with the reference's integer slope (src/receiver.c:159-204). */
+int ifos_receivers_line(float DH, float XREC1, float YREC1, float XREC2, float YREC2, int NGEOPH, int max_tr,
+                        int *recpos, int *ntr_out);
+
+/* replaces write_matrix_disk() src/matrix_operations.c:24 on one rank: the
+ * gradient or model m [NY][NX] to "<path_name>.bin", x-outer (the model
+ * file layout; the reference's per-rank file merged by mergemod). */
+int ifos_matrix_dump(const char *path_name, int NX, int NY, const float *m);
+
 /* Trace kill tables [ntr_glob][nsrc_glob] (1 = trace not used), as
  * calc_res builds kill_tmp (src/calc_res.c:66-126).  A shot's kill mask for
  * ifos_engine_residual is its column: kill[r] = table[r][shot].

@@ -865,3 +865,79 @@ void ref_pcg(int nx, int ny, int niter, int pcg_iter_start, const float *C3, con
     free_matrix(sp, 1, 8, 1, 1);
     free_imatrix(rp, 1, 3, 1, 1);
 }
+
+/* ---------------------------------------------------------------------------
+ * File readers of the reference, called unmodified: sources() (SRCREC 1),
+ * splitsrc() on one rank, receiver() (READREC 1 and 0), inseis().
+ * ------------------------------------------------------------------------- */
+int ref_sources(const char *path, int source_type, int max_src, float *out, int *nsrc_out, float *fc_max)
+{
+    harness_setup_opt(1, 1);
+    SRCREC = 1;
+    SOURCE_TYPE = source_type;
+    PLANE_WAVE_DEPTH = 0.0f;
+    RUN_MULTIPLE_SHOTS = 1;
+    snprintf(SOURCE_FILE, STRING_SIZE, "%s", path);
+    int n = 0;
+    float **sp = sources(&n);
+    if (n > max_src) return -1;
+    for (int l = 1; l <= n; l++)
+        for (int k = 1; k <= 8; k++) out[(size_t)(l - 1) * 8 + k - 1] = sp[k][l];
+    *nsrc_out = n;
+    *fc_max = 1.0f / TS;
+    return 0;
+}
+
+int ref_splitsrc(int nx, int ny, float dh, int nsrc, const float *srcpos_m, float *out, int *nloc)
+{
+    harness_setup_opt(nx, ny);
+    DH = dh;
+    IENDX = nx;
+    IENDY = ny;
+    float **sp = matrix(1, 8, 1, nsrc);
+    for (int l = 1; l <= nsrc; l++)
+        for (int k = 1; k <= 8; k++) sp[k][l] = srcpos_m[(size_t)(l - 1) * 8 + k - 1];
+    int n = 0;
+    float **loc = splitsrc(sp, &n, nsrc);
+    for (int l = 1; l <= n; l++)
+        for (int k = 1; k <= 8; k++) out[(size_t)(l - 1) * 8 + k - 1] = loc[k][l];
+    *nloc = n;
+    return 0;
+}
+
+int ref_receivers(int readrec, const char *path, float dh, float rx, float ry, float x1, float y1, float x2, float y2,
+                  int ngeoph, int max_tr, int *out, int *ntr_out)
+{
+    harness_setup_opt(1, 1);
+    DH = dh;
+    READREC = readrec;
+    REFREC[1] = rx;
+    REFREC[2] = ry;
+    REFREC[3] = 0.0f;
+    XREC1 = x1; YREC1 = y1; XREC2 = x2; YREC2 = y2;
+    NGEOPH = ngeoph;
+    VERBOSE = 0;
+    snprintf(REC_FILE, STRING_SIZE, "%s", path ? path : "");
+    int n = 0;
+    int **rp = receiver(&n, NULL, 0);
+    if (n > max_tr) return -1;
+    for (int t = 1; t <= n; t++) {
+        out[2 * (t - 1)] = rp[1][t];
+        out[2 * (t - 1) + 1] = rp[2][t];
+    }
+    *ntr_out = n;
+    return 0;
+}
+
+/* inseis(..., sws 1) on <prefix>_vx.su.shot<shot>: out [ntr][ns] */
+void ref_inseis(const char *prefix, int shot, int ntr, int ns, float *out)
+{
+    harness_setup_opt(1, 1);
+    snprintf(DATA_DIR, STRING_SIZE, "%s", prefix);
+    /* one spare column: the reference writes section[t][ns+1] */
+    float **sec = matrix(1, ntr, 1, ns + 1);
+    inseis(NULL, shot, sec, ntr, ns, 1, 0);
+    for (int t = 1; t <= ntr; t++)
+        for (int k = 1; k <= ns; k++) out[(size_t)(t - 1) * ns + k - 1] = sec[t][k];
+    free_matrix(sec, 1, ntr, 1, ns + 1);
+}

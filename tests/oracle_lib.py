@@ -485,3 +485,36 @@ def ref_pcg(grads, C3, start, prefix):
     ref().ref_pcg(C.c_int(nx), C.c_int(ny), C.c_int(n), C.c_int(start), _ptr(c3), prefix.encode(), _ptr(g),
                   _ptr(dirs), _ptr(gps))
     return dirs, gps
+
+
+# ------------------------------------------------------------- file readers --
+def ref_sources(path, source_type=1, max_src=1000):
+    out = np.zeros((max_src, 8), np.float32)
+    n, fc = C.c_int(), C.c_float()
+    rc = ref().ref_sources(path.encode(), source_type, max_src, _ptr(out), C.byref(n), C.byref(fc))
+    assert rc == 0
+    return out[:n.value].copy(), fc.value
+
+
+def ref_splitsrc(srcpos_m, DH, NX, NY):
+    sp = _f32(srcpos_m).reshape(-1, 8)
+    out = np.zeros_like(sp)
+    n = C.c_int()
+    ref().ref_splitsrc(NX, NY, C.c_float(DH), len(sp), _ptr(sp), _ptr(out), C.byref(n))
+    return out[:n.value].copy()
+
+
+def ref_receivers(readrec, path, DH, refrec=(0.0, 0.0), line=(0.0, 0.0, 0.0, 0.0), ngeoph=1, max_tr=10000):
+    out = np.zeros((max_tr, 2), np.int32)
+    n = C.c_int()
+    rc = ref().ref_receivers(readrec, path.encode() if path else None, C.c_float(DH), C.c_float(refrec[0]),
+                             C.c_float(refrec[1]), *[C.c_float(v) for v in line], ngeoph, max_tr,
+                             _ptr(out, C.c_int), C.byref(n))
+    assert rc == 0
+    return out[:n.value].copy()
+
+
+def ref_inseis(prefix, shot, ntr, ns):
+    out = np.zeros((ntr, ns), np.float32)
+    ref().ref_inseis(prefix.encode(), shot, ntr, ns, _ptr(out))
+    return out

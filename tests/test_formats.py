@@ -3,6 +3,7 @@ the quasi-JSON parameter file (the reference's own par/in_and_out/
 IFOS2D_FW.json, committed as tests/golden/IFOS2D_FW.json), binary model
 files and SU seismograms, checked against numpy readings of the bytes."""
 import os
+import tempfile
 
 import numpy as np
 import pytest
@@ -124,3 +125,100 @@ def test_trkill_offsets_match_reference():
     base[0, :] = 1
     both = F.trkill_offsets(rec, src, DH, lo, hi, table=base)
     np.testing.assert_array_equal(both, np.maximum(base, got))
+
+
+# ------------------------------------------- source / receiver files, inseis --
+needs_ref_files = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
+
+
+def write_source_file(path, rows):
+    with open(path, "w") as f:
+        f.write(f"{len(rows)}\n")
+        for r in rows:
+            f.write(" ".join(f"{v:g}" for v in r) + "\n")
+
+
+SOURCE_ROWS = [(100.0, 0.0, 20.0, 0.0, 30.0, 1.0),                      # 6 columns: azimuth 0, type default
+               (210.5, 3.0, 9.75, 0.01, 25.0, 2.0e3, 45.0),               # 7 columns: type default
+               (400.0, 0.0, 40.0, 0.0, 35.0, 1.0, 30.0, 4.0),             # rotated force
+               (3.0, 0.0, 3.0, 0.002, 15.0, 1.5, 0.0, 2.0),
+               (-40.0, 0.0, 10.0, 0.0, 20.0, 1.0, 0.0, 1.0)]               # left of the grid
+
+
+@needs_ref_files
+@pytest.mark.parametrize("stype", [1, 3])
+def test_source_file_matches_reference(tmp_path, stype):
+    p = str(tmp_path / "src.dat")
+    write_source_file(p, SOURCE_ROWS)
+    mine, fc = F.read_sources(p, source_type=stype)
+    ref, fc_ref = O.ref_sources(p, source_type=stype)
+    np.testing.assert_array_equal(mine, ref)
+    assert fc == fc_ref == 35.0
+    assert mine[0, 6] == 0.0 and mine[0, 7] == stype and mine[1, 7] == stype and mine[2, 7] == 4.0
+    # x, y from columns 1 and 3 (column 2 is z)
+    assert mine[1, 0] == 210.5 and mine[1, 1] == 9.75
+
+
+@needs_ref_files
+@pytest.mark.parametrize("DH", [1.0, 2.0, 3.3])
+def test_sources_to_grid_matches_splitsrc(tmp_path, DH):
+    p = str(tmp_path / "src.dat")
+    write_source_file(p, SOURCE_ROWS)
+    rows, _ = F.read_sources(p)
+    for NX, NY in ((200, 30), (50, 12)):
+        np.testing.assert_array_equal(F.sources_to_grid(rows, DH, NX, NY), O.ref_splitsrc(rows, DH, NX, NY))
+    g = F.sources_to_grid(rows, 1.0, 500, 100)
+    assert g[0, 0] == 100 and g[0, 1] == 20
+
+
+def test_source_file_missing_columns(tmp_path):
+    p = str(tmp_path / "src.dat")
+    write_source_file(p, [(1.0, 0.0, 2.0, 0.0, 10.0)])
+    with pytest.raises(ifos_amd.IfosError):
+        F.read_sources(p)
+
+
+@needs_ref_files
+@pytest.mark.parametrize("DH,ref", [(1.0, (0.0, 0.0)), (2.5, (3.0, -1.0))])
+def test_receiver_file_matches_reference(tmp_path, DH, ref):
+    p = str(tmp_path / "rec.dat")
+    pts = [(10.0, 5.0), (12.4, 5.0), (12.6, 5.2), (12.0, 5.0), (30.25, 7.75), (10.0, 5.0), (0.1, 5.0)]
+    with open(p, "w") as f:
+        for x, y in pts:
+            f.write(f"{x} {y}\n")
+    mine = F.read_receivers(p, DH, refrec=ref)
+    np.testing.assert_array_equal(mine, O.ref_receivers(1, p, DH, refrec=ref))
+    assert len(mine) < len(pts)  # duplicates on one grid point dropped
+
+
+@needs_ref_files
+@pytest.mark.parametrize("line,ngeoph", [((10.0, 6.0, 210.0, 6.0), 2), ((10.0, 4.0, 90.0, 30.0), 3),
+                                          ((20.0, 5.0, 26.0, 95.0), 4)])
+def test_receiver_line_matches_reference(line, ngeoph):
+    mine = F.receiver_line(1.0, *line, ngeoph)
+    np.testing.assert_array_equal(mine, O.ref_receivers(0, None, 1.0, line=line, ngeoph=ngeoph))
+    assert len(mine) > 3
+
+
+@needs_ref_files
+def test_observed_su_matches_inseis():
+    rng = np.random.default_rng(4)
+    ntr, ns = 9, 57
+    data = rng.standard_normal((ntr, ns)).astype(np.float32)
+    rec = np.stack([np.arange(ntr) + 5, np.full(ntr, 3)], 1).astype(np.int32)
+    # the reference's DATA_DIR buffer holds 74 bytes: a short prefix
+    with tempfile.TemporaryDirectory(prefix="su", dir="/tmp") as d:
+        prefix = os.path.join(d, "o")
+        F.write_su(prefix + "_vx.su.shot2", data, rec, 2.0, 1e-3, 10.0, 4.0)
+        mine = F.read_su_observed(prefix + "_vx.su.shot2", ntr, ns)
+        ref = O.ref_inseis(prefix, 2, ntr, ns)
+    np.testing.assert_array_equal(mine, ref)
+    assert np.all(mine[:, 0] == 0) and np.array_equal(mine[:, 1:], data[:, 1:])
+
+
+def test_matrix_dump_is_model_layout(tmp_path):
+    m = np.arange(12, dtype=np.float32).reshape(3, 4)
+    F.dump_matrix(str(tmp_path / "grad"), m)
+    raw = np.fromfile(str(tmp_path / "grad.bin"), np.float32)
+    np.testing.assert_array_equal(raw, m.T.ravel())  # x outer, y inner
+    np.testing.assert_array_equal(F.read_model(str(tmp_path / "grad.bin"), 4, 3), m)
