@@ -131,6 +131,17 @@ def test_trkill_offsets_match_reference():
 needs_ref_files = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
 
 
+@pytest.fixture
+def short_tmp():
+    """A directory with a short path: the reference keeps file names in
+    char[74] buffers (STRING_SIZE, src/fd.h:22) and aborts the process when
+    the truncated name does not open, and pytest's tmp_path under xdist is
+    longer than that."""
+    import pathlib
+    with tempfile.TemporaryDirectory(prefix="f", dir="/tmp") as d:
+        yield pathlib.Path(d)
+
+
 def write_source_file(path, rows):
     with open(path, "w") as f:
         f.write(f"{len(rows)}\n")
@@ -147,8 +158,8 @@ SOURCE_ROWS = [(100.0, 0.0, 20.0, 0.0, 30.0, 1.0),                      # 6 colu
 
 @needs_ref_files
 @pytest.mark.parametrize("stype", [1, 3])
-def test_source_file_matches_reference(tmp_path, stype):
-    p = str(tmp_path / "src.dat")
+def test_source_file_matches_reference(short_tmp, stype):
+    p = str(short_tmp / "src.dat")
     write_source_file(p, SOURCE_ROWS)
     mine, fc = F.read_sources(p, source_type=stype)
     ref, fc_ref = O.ref_sources(p, source_type=stype)
@@ -180,8 +191,8 @@ def test_source_file_missing_columns(tmp_path):
 
 @needs_ref_files
 @pytest.mark.parametrize("DH,ref", [(1.0, (0.0, 0.0)), (2.5, (3.0, -1.0))])
-def test_receiver_file_matches_reference(tmp_path, DH, ref):
-    p = str(tmp_path / "rec.dat")
+def test_receiver_file_matches_reference(short_tmp, DH, ref):
+    p = str(short_tmp / "rec.dat")
     pts = [(10.0, 5.0), (12.4, 5.0), (12.6, 5.2), (12.0, 5.0), (30.25, 7.75), (10.0, 5.0), (0.1, 5.0)]
     with open(p, "w") as f:
         for x, y in pts:
