@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ifos2d_opt.h"
@@ -259,17 +260,23 @@ extern "C" void ifos_opt_destroy(ifos_opt *o)
     delete o;
 }
 
+// FP64 sum of a*b (mode 0) or a*(a-b) (mode 1) on stream st, with the
+// kRedBlocks partials and the result slot in partial[0 .. kRedBlocks]
+static int dot_on(hipStream_t st, double *partial, const float *a, const float *b, long n, int mode, double *out)
+{
+    if (n < 0) return ifos_set_error("negative length"), -2;
+    hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, st, a, b, n, mode, partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedThreads), 0, st, partial, kRedBlocks, partial + kRedBlocks);
+    OPT_TRY(hipGetLastError());
+    OPT_TRY(hipMemcpyAsync(out, partial + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, st));
+    OPT_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
 static int dot_mode(ifos_opt *o, const float *a, const float *b, long n, int mode, double *out)
 {
     OPT_TRY(hipSetDevice(o->device));
-    if (n < 0) return ifos_set_error("negative length"), -2;
-    hipLaunchKernelGGL(k_dot_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, o->stream, a, b, n, mode, o->partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedThreads), 0, o->stream, o->partial, kRedBlocks,
-                       o->partial + kRedBlocks);
-    OPT_TRY(hipGetLastError());
-    OPT_TRY(hipMemcpyAsync(out, o->partial + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, o->stream));
-    OPT_TRY(hipStreamSynchronize(o->stream));
-    return 0;
+    return dot_on(o->stream, o->partial, a, b, n, mode, out);
 }
 
 extern "C" int ifos_opt_dot(ifos_opt *o, const float *a, const float *b, long n, double *out)
@@ -292,35 +299,85 @@ extern "C" int ifos_opt_max(ifos_opt *o, const float *a, long n, int absolute, f
     return 0;
 }
 
-extern "C" int ifos_opt_pcg(ifos_opt *o, long n, const float *grad, float C, float *p_last, float *c_last, int iter,
-                            int pcg_iter_start, float *dir, double *beta_out)
+// PCG direction on stream st with reduction scratch partial[kRedBlocks + 1]
+static int pcg_on(hipStream_t st, double *partial, long n, const float *grad, float C, float *p_last, float *c_last,
+                  int iter, int pcg_iter_start, float *dir, double *beta_out)
 {
-    OPT_TRY(hipSetDevice(o->device));
     int use1 = 1, use2 = 1;
     if ((iter - pcg_iter_start) < 2) {
         use2 = 0;
         if ((iter - pcg_iter_start) < 1) use1 = 0;
     }
-    hipLaunchKernelGGL(k_pcg_scale, dim3(blocks_for(n, 256)), dim3(256), 0, o->stream, grad, C, dir, n);
+    hipLaunchKernelGGL(k_pcg_scale, dim3(blocks_for(n, 256)), dim3(256), 0, st, grad, C, dir, n);
     OPT_TRY(hipGetLastError());
     const bool conj = (iter > 1) && use1;
     float beta = 0.0f;
     double b64 = 0.0;
     if (conj) {
         double num = 0.0, den = 0.0;
-        ENGINE_OK(dot_mode(o, dir, p_last, n, 1, &num));
-        ENGINE_OK(dot_mode(o, p_last, p_last, n, 0, &den));
+        ENGINE_OK(dot_on(st, partial, dir, p_last, n, 1, &num));
+        ENGINE_OK(dot_on(st, partial, p_last, p_last, n, 0, &den));
         if (den != 0.0) b64 = num / den;
         if (b64 < 0.0) b64 = 0.0;
         beta = (float)b64;
     }
     const int form = !conj ? 0 : (iter == 2 ? 1 : ((iter > 2 && use2) ? 2 : 0));
-    hipLaunchKernelGGL(k_pcg_direction, dim3(blocks_for(n, 256)), dim3(256), 0, o->stream, dir, p_last, c_last, n,
-                       beta, form, conj ? 1 : 0);
+    hipLaunchKernelGGL(k_pcg_direction, dim3(blocks_for(n, 256)), dim3(256), 0, st, dir, p_last, c_last, n, beta,
+                       form, conj ? 1 : 0);
     OPT_TRY(hipGetLastError());
-    OPT_TRY(hipStreamSynchronize(o->stream));
+    OPT_TRY(hipStreamSynchronize(st));
     if (beta_out) *beta_out = b64;
     return 0;
+}
+
+extern "C" int ifos_opt_pcg(ifos_opt *o, long n, const float *grad, float C, float *p_last, float *c_last, int iter,
+                            int pcg_iter_start, float *dir, double *beta_out)
+{
+    OPT_TRY(hipSetDevice(o->device));
+    return pcg_on(o->stream, o->partial, n, grad, C, p_last, c_last, iter, pcg_iter_start, dir, beta_out);
+}
+
+// ---------------------------------------------------------------------------
+// ifos_opt_pcg_noalloc: the PCG direction with its reduction scratch in a
+// file-scope device array instead of an ifos_opt context, for callers that
+// cannot allocate device memory.  Each GPU holds its own copy of the array
+// for the life of the process; host code never passes the array's name
+// (a host-side stand-in, not a device address) to a kernel, only the device
+// address hipGetSymbolAddress reports for the current device.  Every entry
+// of the scratch is written before it is read (each block writes its
+// partial, then the sum writes the result slot), so nothing is cleared.
+constexpr int kNoAllocScratch = kRedBlocks + 8;  // doubles: partials + result slot
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+constexpr int kNoAllocMaxDevices = 64;
+constexpr long kNoAllocMaxCells = 2147483647L;  // element-wise grids stay below 2^23 blocks
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kNoAllocMaxDevices];
+
+static int noalloc_scratch(double **out)
+{
+    int dev = -1;
+    OPT_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return ifos_set_error("device %d beyond the %d supported", dev, kNoAllocMaxDevices), -2;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        double *p = nullptr;
+        OPT_TRY(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+        if (!p) return ifos_set_error("no device address for the reduction scratch on device %d", dev), -1;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" int ifos_opt_pcg_noalloc(long n, const float *grad, float C, float *p_last, float *c_last, int iter,
+                                    int pcg_iter_start, float *dir, double *beta_out)
+{
+    if (n < 0 || n > kNoAllocMaxCells)
+        return ifos_set_error("ifos_opt_pcg_noalloc: %ld cells, the limit is %ld", n, kNoAllocMaxCells), -2;
+    double *partial = nullptr;
+    ENGINE_OK(noalloc_scratch(&partial));
+    return pcg_on(nullptr, partial, n, grad, C, p_last, c_last, iter, pcg_iter_start, dir, beta_out);
 }
 
 extern "C" int ifos_opt_step_length(ifos_opt *o, const float *L2t, const float *epst, float *opteps)

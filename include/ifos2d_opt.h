@@ -45,6 +45,17 @@ int ifos_opt_max(ifos_opt *o, const float *a, long n, int absolute, float *out);
 int ifos_opt_pcg(ifos_opt *o, long n, const float *grad, float C, float *p_last, float *c_last, int iter,
                  int pcg_iter_start, float *dir, double *beta);
 
+/* No counterpart in the reference.  ifos_opt_pcg for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and results, with no
+ * ifos_opt context.  It runs on the current device's null stream and keeps
+ * its FP64 reduction partials in a static device array of 1032 doubles
+ * (8256 bytes) that each GPU holds until the process ends; it calls no
+ * allocating routine.  n is limited to 2^31 - 1 cells (larger n: error -2).
+ * Calls that overlap on one GPU share that array and race; serialise them. */
+int ifos_opt_pcg_noalloc(long n, const float *grad, float C, float *p_last, float *c_last, int iter,
+                         int pcg_iter_start, float *dir, double *beta);
+
 /* replaces calc_opt_step() src/calc_opt_step.c:29 (host arrays: the
  * reference's L2t[1..3], epst1[1..3]): the parabola through the three
  * points, solved in FP64 on the device, with the same clamps. */
