@@ -91,11 +91,29 @@ def serving(args):
     from ifos_amd import workloads
     from ifos_amd.serve import FORWARD, GRADIENT, Server
     rng = np.random.default_rng(args.seed)
-    n = max(1, int(round(args.rate * args.duration)))
+    full = args.preset == "full"
+    if full:
+        # the serving workload's own ranges: 20 requests/s, grids 300x150 to
+        # 3000x1200 with dh 1-10 m, 1-64 shots, FDORDER 4 or 8, elastic or
+        # L=2, NT 1000-6000, layered models, a 288 GB device
+        args.rate, args.max_nx, args.min_nt, args.max_nt, args.max_shots = 20.0, 3000, 1000, 6000, 64
+        n = args.requests or 24
+    else:
+        n = args.requests or max(1, int(round(args.rate * args.duration)))
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n))
 
     def survey(k):
         nx = int(rng.integers(300, args.max_nx + 1)) // 8 * 8
+        if full:
+            ny = max(150, min(1200, int(nx * rng.uniform(0.4, 0.5)) // 8 * 8))
+            dh = float(rng.uniform(1.0, 10.0))
+            nt = int(rng.integers(args.min_nt, args.max_nt + 1)) // 4 * 4
+            L = int(rng.choice([0, 2]))
+            w = workloads.layered_visc(seed=1000 + k, NX=nx, NY=ny, DH=dh, NT=nt, DTINV=4, nshots=1,
+                                       nrec=max(4, (nx - 60) // 5), FDORDER=int(rng.choice([4, 8])), L=L, fc=10.0)
+            if L == 0:
+                w.cfg.L = 0
+            return w
         ny = int(nx * rng.uniform(0.4, 0.5)) // 8 * 8
         return workloads.small_case(seed=1000 + k, NX=nx, NY=ny,
                                     NT=int(rng.integers(args.min_nt, args.max_nt + 1)) // 4 * 4,
@@ -128,22 +146,27 @@ def serving(args):
             time.sleep(min(0.002, at - (time.perf_counter() - t0)))
         ids.append(srv.submit(kind, w.cfg, w.start, shots, obs=obs, snap=args.snap, model_id=id(w)))
     done = []
-    for rid, (at, w, kind, shots, obs) in zip(ids, reqs):
-        srv.wait(rid)
+    for k, (rid, (at, w, kind, shots, obs)) in enumerate(zip(ids, reqs)):
+        while not srv.wait(rid, timeout=30.0):
+            log(f"[serve] waiting for request {k + 1}/{n} ({time.perf_counter() - t0:.0f} s)")
         r = srv.result(rid)
-        done.append((kind, len(shots), r["times"], 2.0 * w.cfg.NX * w.cfg.NY * w.cfg.NT * len(shots)))
+        done.append((kind, len(shots), r["times"], 2.0 * w.cfg.NX * w.cfg.NY * w.cfg.NT * len(shots), r["shared"]))
+        log(f"[serve] request {k + 1}/{n} done: {'gradient' if kind == GRADIENT else 'forward'}, "
+            f"{w.cfg.NX}x{w.cfg.NY} NT {w.cfg.NT}, {len(shots)} shots, latency {r['times'][2] - r['times'][0]:.2f} s")
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
-    lat = np.array([t[2] - t[0] for _, _, t, _ in done])
-    wait = np.array([t[1] - t[0] for _, _, t, _ in done])
-    gshots = sum(ns for kind, ns, _, _ in done if kind == GRADIENT)
-    fshots = sum(ns for kind, ns, _, _ in done if kind == FORWARD)
-    cells = sum(c for kind, _, _, c in done if kind == GRADIENT) + sum(c / 2 for kind, _, _, c in done if kind == FORWARD)
+    lat = np.array([t[2] - t[0] for _, _, t, _, _ in done])
+    wait = np.array([t[1] - t[0] for _, _, t, _, _ in done])
+    gshots = sum(ns for kind, ns, _, _, _ in done if kind == GRADIENT)
+    fshots = sum(ns for kind, ns, _, _, _ in done if kind == FORWARD)
+    cells = sum(c for kind, _, _, c, _ in done if kind == GRADIENT) + \
+        sum(c / 2 for kind, _, _, c, _ in done if kind == FORWARD)
     out = {"metric": "shot-gradients/s & Gcell-updates/s under Poisson arrivals (serving)",
            "value": round(gshots / wall, 4), "unit": "shot-gradients/s", "n_gpus": 1, "higher_is_better": True,
            "data": "synthetic (seeded layered/anomaly models, zero observed traces)",
-           "config": {"workload": "serving", "requests": n, "rate_per_s": args.rate, "duration_s": args.duration,
+           "config": {"workload": "serving" + ("_full_ranges" if full else ""), "requests": n,
+                      "rate_per_s": args.rate, "arrival_span_s": round(float(arrivals[-1]), 3),
                       "surveys": args.surveys or n,
                       "grid_nx": [300, args.max_nx], "nt": [args.min_nt, args.max_nt], "shots": [1, args.max_shots],
                       "workers": args.workers, "max_slots": args.max_slots, "budget_gb": args.budget_gb,
@@ -153,7 +176,8 @@ def serving(args):
            "queue_wait_s": {"p50": round(float(np.percentile(wait, 50)), 4),
                             "p99": round(float(np.percentile(wait, 99)), 4)},
            "forward_shots_per_s": round(fshots / wall, 4), "gcell_updates_per_s": round(cells / wall / 1e9, 3),
-           "wall_s": round(wall, 2), "server": st}
+           "wall_s": round(wall, 2), "requests_in_shared_launches": sum(1 for d in done if d[4] > 0),
+           "server": st}
     print(json.dumps(out))
     return 0
 
@@ -302,6 +326,86 @@ def post_kernels(args):
     return 0
 
 
+def precision_errors(args):
+    """Errors of the 16-bit snapshot runs against the FP64 build of the
+    scheme (ifos_fp64_*, forward recomputed from checkpoints) at the
+    workload's full size.  Observed traces come from FP32 forward runs on the
+    true model (untimed); the FP64, BF16 and FP16 runs then compute the
+    gradient of the same shots on the start model.  Reported per parameter:
+    relative L2 error of the summed gradient, and the largest relative L2
+    error of any shot's seismograms (the FP32 forward against the FP64
+    forward)."""
+    import ifos_amd
+    w = build_workload(args.workload)
+    cfg = w.cfg
+    shots = w.shots[:args.shots] if args.shots else w.shots
+    log(f"[precision] workload {w.name}: {cfg.NX}x{cfg.NY} L={cfg.L} FDORDER={cfg.FDORDER} NT={cfg.NT} "
+        f"DTINV={cfg.DTINV}, {len(shots)} shots")
+    eng = ifos_amd.Engine(cfg)
+    eng.set_model(**w.true)
+    obs = []
+    for k, s in enumerate(shots):
+        eng.set_shot(s.srcpos, s.recpos)
+        eng.forward(snapshots=False)
+        vx, vy = eng.seismograms()
+        vx[:, 0] = 0.0
+        vy[:, 0] = 0.0
+        obs.append((vx, vy))
+        log(f"[precision] observed data {k + 1}/{len(shots)}")
+    eng.close()
+
+    def kill_of(s):
+        return s.kill if (cfg.TRKILL and s.kill is not None) else None
+
+    e64 = ifos_amd.Fp64Engine(cfg, segment_steps=args.segment)
+    e64.set_model(**w.start)
+    seis64 = []
+    t0 = time.perf_counter()
+    for k, s in enumerate(shots):
+        e64.set_shot(s.srcpos, s.recpos)
+        e64.shot_gradient(obs[k][0], obs[k][1], kill_of(s))
+        seis64.append(e64.seismograms())
+        log(f"[precision] fp64 shot {k + 1}/{len(shots)} ({time.perf_counter() - t0:.1f} s)")
+    t64 = time.perf_counter() - t0
+    g64 = e64.gradient(total=True)
+    seg = e64.segment_steps
+    e64.close()
+
+    def rel(a, b):
+        return float(np.linalg.norm((a.astype(np.float64) - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+    res = {}
+    for prec in ("bf16", "fp16"):
+        eng = ifos_amd.Engine(cfg)
+        eng.set_model(**w.start)
+        eng.set_snapshot_precision(prec)
+        eng.reset_gradient()
+        seis_err = 0.0
+        t0 = time.perf_counter()
+        for k, s in enumerate(shots):
+            eng.set_shot(s.srcpos, s.recpos)
+            eng.shot_gradient(obs[k][0], obs[k][1], kill_of(s))
+            vx, vy = eng.seismograms()
+            seis_err = max(seis_err, rel(vx, seis64[k][0]), rel(vy, seis64[k][1]))
+            log(f"[precision] {prec} shot {k + 1}/{len(shots)} ({time.perf_counter() - t0:.1f} s)")
+        t = time.perf_counter() - t0
+        g = eng.gradient(total=True)
+        eng.close()
+        res[prec] = {"grad_rel_l2": {n: rel(g[i], g64[i]) for i, n in enumerate(("vp", "vs", "rho"))},
+                     "seismogram_rel_l2_max": seis_err, "shot_gradients_per_s": round(len(shots) / t, 4)}
+        log(f"[precision] {prec}: {res[prec]}")
+    out = {"metric": "relative L2 error vs the FP64 build of the scheme (16-bit snapshots, FP32 stencils)",
+           "value": max(res["bf16"]["grad_rel_l2"].values()), "unit": "max relative L2 gradient error (bf16)",
+           "n_gpus": 1, "higher_is_better": False, "dtype": "fp32 stencils, 16-bit snapshots; yardstick fp64",
+           "data": "synthetic (seeded layered model with faults; observed traces from FP32 forward runs on the "
+                   "true model)",
+           "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "FDORDER": cfg.FDORDER, "L": cfg.L,
+                      "NT": cfg.NT, "DTINV": cfg.DTINV, "shots": len(shots), "fp64_segment_steps": seg},
+           "errors": res, "fp64_shot_gradients_per_s": round(len(shots) / t64, 4)}
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -333,6 +437,15 @@ def main():
                     help="time whole FWI iterations of the workload (gradient, PCG, step search, update)")
     ap.add_argument("--dropin", action="store_true",
                     help="one shot gradient through the drop-in layer (the reference's call sequence)")
+    ap.add_argument("--precision-errors", action="store_true",
+                    help="BF16 and FP16 snapshot errors against the FP64 build at the workload's full size")
+    ap.add_argument("--shots", type=int, default=0, help="precision errors: first N shots (0: all)")
+    ap.add_argument("--segment", type=int, default=0,
+                    help="precision errors: FP64 checkpoint segment in steps (0: DTINV*ceil(sqrt(NT/DTINV)))")
+    ap.add_argument("--preset", default="", choices=["", "full"],
+                    help="serving: 'full' draws from the whole serving workload (20/s, 300x150-3000x1200, 1-64 "
+                         "shots, NT 1000-6000)")
+    ap.add_argument("--requests", type=int, default=0, help="serving: number of requests (0: rate x duration)")
     ap.add_argument("--rate", type=float, default=4.0, help="serving: requests per second")
     ap.add_argument("--duration", type=float, default=15.0, help="serving: seconds of arrivals")
     ap.add_argument("--max-nx", type=int, default=1200)
@@ -359,6 +472,8 @@ def main():
         return fwi_iteration(args)
     if args.post:
         return post_kernels(args)
+    if args.precision_errors:
+        return precision_errors(args)
     import ifos_amd
     w = build_workload(args.workload)
     cfg = w.cfg

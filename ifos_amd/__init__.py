@@ -235,6 +235,15 @@ class Engine:
         _check(lib().ifos_engine_get_velocity_rows(self.h, C.byref(v)))
         return v.value
 
+    def set_fused(self, on):
+        """One kernel per time step (velocity and stress) instead of two; same bits."""
+        _check(lib().ifos_engine_set_fused(self.h, int(bool(on))))
+
+    def get_fused(self):
+        v = C.c_int()
+        _check(lib().ifos_engine_get_fused(self.h, C.byref(v)))
+        return v.value
+
     def set_velocity_lds(self, on):
         _check(lib().ifos_engine_set_velocity_lds(self.h, int(bool(on))))
 
@@ -302,6 +311,12 @@ class Engine:
     def reset_gradient(self):
         _check(lib().ifos_engine_reset_gradient(self.h))
 
+    def slot_gradient(self, slot):
+        """The gradient of one slot's shot from the last adjoint run (before the shot sum)."""
+        g = np.zeros((3, self.cfg.NY, self.cfg.NX), np.float32)
+        _check(lib().ifos_engine_slot_gradient(self.h, int(slot), _fp(g[0]), _fp(g[1]), _fp(g[2])))
+        return g
+
     def time_kernel(self, kernel, reps=20):
         ms = C.c_double()
         _check(lib().ifos_engine_time_kernel(self.h, kernel, reps, C.byref(ms)))
@@ -354,3 +369,89 @@ SNAP_PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
 def snapshot_bytes(config, prec=0):
     """Device bytes of one shot's snapshot storage at that precision."""
     return int(lib().ifos_snapshot_bytes(C.byref(config), SNAP_PRECISIONS.get(prec, prec)))
+
+
+class Fp64Engine:
+    """The scheme in FP64 on the device with checkpointed forward
+    recomputation (include/ifos2d_hip.h, ifos_fp64_*): the yardstick that
+    16-bit snapshot runs are measured against.  Same call order as Engine:
+    set_model -> set_shot -> shot_gradient -> gradient."""
+
+    def __init__(self, config, device=0, segment_steps=0):
+        L = lib()
+        L.ifos_fp64_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.ifos_fp64_destroy.argtypes = [C.c_void_p]
+        L.ifos_fp64_destroy.restype = None
+        for name in ("ifos_fp64_set_model", "ifos_fp64_set_shot", "ifos_fp64_shot_gradient", "ifos_fp64_seismograms",
+                     "ifos_fp64_gradient", "ifos_fp64_reset_gradient", "ifos_fp64_segment_steps"):
+            getattr(L, name).restype = C.c_int
+        L.ifos_fp64_set_model.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        L.ifos_fp64_set_shot.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+        L.ifos_fp64_shot_gradient.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+        L.ifos_fp64_seismograms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ifos_fp64_gradient.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ifos_fp64_reset_gradient.argtypes = [C.c_void_p]
+        L.ifos_fp64_segment_steps.argtypes = [C.c_void_p]
+        self.cfg = config
+        h = C.c_void_p()
+        _check(L.ifos_fp64_create(C.byref(config), device, int(segment_steps), C.byref(h)))
+        self.h = h
+        self.ntr = 0
+
+    def close(self):
+        if self.h:
+            lib().ifos_fp64_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def segment_steps(self):
+        return lib().ifos_fp64_segment_steps(self.h)
+
+    def set_model(self, vp, vs, rho, taus=None, taup=None):
+        arrs = [None if a is None else _f32(a) for a in (vp, vs, rho, taus, taup)]
+        _check(lib().ifos_fp64_set_model(self.h, *[None if a is None else a.ctypes.data_as(C.c_void_p)
+                                                   for a in arrs]))
+
+    def set_shot(self, srcpos, recpos):
+        srcpos = _f32(np.asarray(srcpos, np.float32).reshape(-1, SRC_STRIDE))
+        recpos = np.ascontiguousarray(np.asarray(recpos, np.int32).reshape(-1, 2))
+        self.ntr = recpos.shape[0]
+        _check(lib().ifos_fp64_set_shot(self.h, srcpos.shape[0], srcpos.ctypes.data_as(C.c_void_p), self.ntr,
+                                        recpos.ctypes.data_as(C.c_void_p)))
+
+    def shot_gradient(self, obs_vx, obs_vy, kill=None):
+        ovx, ovy = _f32(obs_vx), _f32(obs_vy)
+        k = None if kill is None else np.ascontiguousarray(kill, np.int32)
+        mis = np.zeros(2, np.float64)
+        _check(lib().ifos_fp64_shot_gradient(self.h, ovx.ctypes.data_as(C.c_void_p), ovy.ctypes.data_as(C.c_void_p),
+                                             None if k is None else k.ctypes.data_as(C.c_void_p),
+                                             mis.ctypes.data_as(C.c_void_p)))
+        return mis
+
+    def seismograms(self):
+        vx = np.zeros((self.ntr, self.cfg.NT), np.float64)
+        vy = np.zeros_like(vx)
+        _check(lib().ifos_fp64_seismograms(self.h, vx.ctypes.data_as(C.c_void_p), vy.ctypes.data_as(C.c_void_p)))
+        return vx, vy
+
+    def gradient(self, total=False):
+        g = np.zeros((3, self.cfg.NY, self.cfg.NX), np.float64)
+        _check(lib().ifos_fp64_gradient(self.h, int(total), *[g[k].ctypes.data_as(C.c_void_p) for k in range(3)]))
+        return g
+
+    def reset_gradient(self):
+        _check(lib().ifos_fp64_reset_gradient(self.h))
+
+
+def fp64_bytes(config, segment_steps=0, ntr=0, nsrc=0):
+    """Device bytes of an Fp64Engine for that configuration."""
+    L = lib()
+    L.ifos_fp64_bytes.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    L.ifos_fp64_bytes.restype = C.c_size_t
+    return int(L.ifos_fp64_bytes(C.byref(config), segment_steps, ntr, nsrc))

@@ -110,7 +110,10 @@ struct ifos_engine {
     float *slab = nullptr;
     float *vx, *vy, *sxx, *syy, *sxy, *ux, *uy, *uxy, *vxp1, *vyp1;
     float *r = nullptr, *p = nullptr, *q = nullptr;
-    long ws = 0;          // floats per slot: 10 wavefields + 3L memory variables
+    float *vx2, *vy2, *sxx2, *syy2, *sxy2;  // second set of the one-kernel step
+    long ws = 0;          // floats per slot: 15 wavefields + 3L memory variables
+    bool fused = false;   // one kernel per time step (k_fused_step)
+    int last_set = 0;     // wavefield set holding the state after the last run
     // shared material grids
     float *mslab = nullptr;
     float *rip, *rjp, *uipjp, *f, *g, *fipjp, *rho, *vs, *vp, *taus, *taup;
@@ -126,6 +129,7 @@ struct ifos_engine {
     float *pml = nullptr;   // 12 profiles
     float *psi = nullptr;   // per slot: 5 x-strips then 4 y-strips
     float *psi_sxx_x, *psi_sxy_x, *psi_vxx, *psi_vyx, *psi_vxxs, *psi_syy_y, *psi_sxy_y, *psi_vyy, *psi_vxy;
+    float *psi2_sxx_x, *psi2_sxy_x, *psi2_syy_y, *psi2_sxy_y;  // second set of the one-kernel step
     long pss = 0;           // floats per slot
 
     // control
@@ -151,6 +155,7 @@ struct ifos_engine {
     int *tiles = nullptr;
     size_t tiles_cap = 0;
     int off_src_start = 0, off_src_idx = 0, off_rec_start = 0, off_rec_idx = 0;
+    int off_srcx_start = 0, off_srcx_idx = 0, off_recx_start = 0, off_recx_idx = 0;  // tile + rim lists
     LaunchShape shape;
     // energy preconditioner (src/eprecond.c, src/eprecond1.c): Ws and Wr per
     // slot, the 1/We sum over shots, the mode-3 geometric factor
@@ -184,6 +189,7 @@ struct ifos_engine {
     size_t snap_bytes = 0;
     int snap_prec = 0;        // 0 FP32, 1 BF16, 2 FP16
     bool snap_valid = false;  // the last forward stored snapshots for the current model and shot
+    bool slot_grad_valid = false;  // wcbuf holds each slot's gradient of the last adjoint run
 
     // graphs
     hipGraphExec_t gexec[2] = {nullptr, nullptr};  // forward(with snapshots), adjoint
@@ -267,8 +273,9 @@ static int engine_create(const ifos_config *cfg, int device, int slots, ifos_eng
     ifos_holberg(c.FDORDER, c.MAXRELERROR, E->hc);
     const int L = c.L;
 
-    // per slot: 10 wavefields + 3L memory planes
-    const int nwave = 10 + 3 * L;
+    // per slot: 10 wavefields + 3L memory planes + the second set of
+    // vx, vy, sxx, syy, sxy for the one-kernel step
+    const int nwave = 15 + 3 * L;
     E->ws = (long)nwave * E->field;
     const size_t wbytes = sizeof(float) * (size_t)E->ws * slots;
     if (dmalloc(&E->slab, wbytes) != hipSuccess) {
@@ -289,6 +296,7 @@ static int engine_create(const ifos_config *cfg, int device, int slots, ifos_eng
         E->p = next(); for (int l = 1; l < L; l++) next();
         E->q = next(); for (int l = 1; l < L; l++) next();
     }
+    E->vx2 = next(); E->vy2 = next(); E->sxx2 = next(); E->syy2 = next(); E->sxy2 = next();
     // shared: 15 material grids + 3L coefficient planes
     const int nmat = 15 + 3 * L;
     if (dmalloc(&E->mslab, sizeof(float) * E->field * nmat) != hipSuccess) {
@@ -330,13 +338,15 @@ static int engine_create(const ifos_config *cfg, int device, int slots, ifos_eng
     if (dmalloc(&E->pml, sizeof(float) * prof.size()) != hipSuccess) return fail(-3);
     hipMemcpyAsync(E->pml, prof.data(), sizeof(float) * prof.size(), hipMemcpyHostToDevice, E->stream);
     const size_t xs = (size_t)c.NY * fw2, ys = (size_t)fw2 * c.NX;
-    E->pss = (long)(5 * xs + 4 * ys);
+    E->pss = (long)(7 * xs + 6 * ys);
     if (dmalloc(&E->psi, sizeof(float) * E->pss * slots) != hipSuccess) return fail(-3);
     hipMemsetAsync(E->psi, 0, sizeof(float) * E->pss * slots, E->stream);
     E->psi_sxx_x = E->psi; E->psi_sxy_x = E->psi + xs; E->psi_vxx = E->psi + 2 * xs;
     E->psi_vyx = E->psi + 3 * xs; E->psi_vxxs = E->psi + 4 * xs;
     float *yb = E->psi + 5 * xs;
     E->psi_syy_y = yb; E->psi_sxy_y = yb + ys; E->psi_vyy = yb + 2 * ys; E->psi_vxy = yb + 3 * ys;
+    float *b2 = yb + 4 * ys;
+    E->psi2_sxx_x = b2; E->psi2_sxy_x = b2 + xs; E->psi2_syy_y = b2 + 2 * xs; E->psi2_sxy_y = b2 + 2 * xs + ys;
 
     if (dmalloc(&E->ctl, sizeof(int) * 4) != hipSuccess) return fail(-3);
     hipMemsetAsync(E->ctl, 0, sizeof(int) * 4, E->stream);
@@ -483,6 +493,7 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
     }
     HIP_TRY(hipStreamSynchronize(E->stream));
     E->snap_valid = false;
+    E->slot_grad_valid = false;
     if (nsrc > E->cap_src) {
         dfree(E->src_i); dfree(E->src_d); dfree(E->sig);
         E->src_i = nullptr; E->src_d = nullptr; E->sig = nullptr;
@@ -579,6 +590,36 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
     bucket(so, [&](int q) { return (int)srcpos[(size_t)q * IFOS_SRC_STRIDE]; },
            [&](int q) { return (int)srcpos[(size_t)q * IFOS_SRC_STRIDE + 1]; }, s_start, s_idx);
     bucket(to, [&](int q) { return recpos[2 * q]; }, [&](int q) { return recpos[2 * q + 1]; }, r_start, r_idx);
+    // the same per tile for the tile and its N-cell rim (k_fused_step): a
+    // point is listed for every tile whose rim reaches it, in point order
+    auto bucket_rim = [&](const std::vector<int> &off, auto x_of, auto y_of, std::vector<int> &start,
+                          std::vector<int> &idx) {
+        const int R = E->N;
+        start.assign((size_t)nshots * (nblk + 1), 0);
+        std::vector<std::vector<int>> per(nblk);
+        idx.clear();
+        for (int k = 0; k < nshots; k++) {
+            for (auto &v : per) v.clear();
+            for (int q = off[k]; q < off[k + 1]; q++) {
+                const int x = x_of(q), y = y_of(q);
+                const int bx0 = std::max(0, (x - 1 - R) / kTileX), bx1 = std::min(E->nbx - 1, (x - 1 + R) / kTileX);
+                const int by0 = std::max(0, (y - 1 - R) / kTileY), by1 = std::min(E->nby - 1, (y - 1 + R) / kTileY);
+                for (int by = by0; by <= by1; by++)
+                    for (int bx = bx0; bx <= bx1; bx++) per[by * E->nbx + bx].push_back(q);
+            }
+            int *st = start.data() + (size_t)k * (nblk + 1);
+            st[0] = (int)idx.size();
+            for (int b = 0; b < nblk; b++) {
+                idx.insert(idx.end(), per[b].begin(), per[b].end());
+                st[b + 1] = (int)idx.size();
+            }
+        }
+        if (idx.empty()) idx.push_back(0);
+    };
+    std::vector<int> sx_start, sx_idx, rx_start, rx_idx;
+    bucket_rim(so, [&](int q) { return (int)srcpos[(size_t)q * IFOS_SRC_STRIDE]; },
+               [&](int q) { return (int)srcpos[(size_t)q * IFOS_SRC_STRIDE + 1]; }, sx_start, sx_idx);
+    bucket_rim(to, [&](int q) { return recpos[2 * q]; }, [&](int q) { return recpos[2 * q + 1]; }, rx_start, rx_idx);
     std::vector<int> blob;
     E->off_src_start = 0;
     blob.insert(blob.end(), s_start.begin(), s_start.end());
@@ -588,6 +629,14 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
     blob.insert(blob.end(), r_start.begin(), r_start.end());
     E->off_rec_idx = (int)blob.size();
     blob.insert(blob.end(), r_idx.begin(), r_idx.end());
+    E->off_srcx_start = (int)blob.size();
+    blob.insert(blob.end(), sx_start.begin(), sx_start.end());
+    E->off_srcx_idx = (int)blob.size();
+    blob.insert(blob.end(), sx_idx.begin(), sx_idx.end());
+    E->off_recx_start = (int)blob.size();
+    blob.insert(blob.end(), rx_start.begin(), rx_start.end());
+    E->off_recx_idx = (int)blob.size();
+    blob.insert(blob.end(), rx_idx.begin(), rx_idx.end());
     if (blob.size() > E->tiles_cap) {
         dfree(E->tiles);
         E->tiles = nullptr;
@@ -659,6 +708,9 @@ static StepArgs step_args(ifos_engine *E, int sw, bool snapshots)
     a.psi_vxx = E->psi_vxx; a.psi_vyx = E->psi_vyx; a.psi_vyy = E->psi_vyy; a.psi_vxy = E->psi_vxy;
     a.psi_vxxs = E->psi_vxxs;
     a.pss = E->pss;
+    a.vx2 = E->vx2; a.vy2 = E->vy2; a.sxx2 = E->sxx2; a.syy2 = E->syy2; a.sxy2 = E->sxy2;
+    a.psi2_sxx_x = E->psi2_sxx_x; a.psi2_sxy_x = E->psi2_sxy_x; a.psi2_syy_y = E->psi2_syy_y;
+    a.psi2_sxy_y = E->psi2_sxy_y;
     TileList src;
     src.start = E->tiles + E->off_src_start; src.idx = E->tiles + E->off_src_idx;
     src.ix = E->src_i; src.iy = E->src_i + E->cap_src; src.type = E->src_i + 2 * E->cap_src;
@@ -672,6 +724,10 @@ static StepArgs step_args(ifos_engine *E, int sw, bool snapshots)
     a.inj = sw == 0 ? src : rec;
     a.exp = src;
     a.rec = rec;
+    TileList srcx = src, recx = rec;
+    srcx.start = E->tiles + E->off_srcx_start; srcx.idx = E->tiles + E->off_srcx_idx;
+    recx.start = E->tiles + E->off_recx_start; recx.idx = E->tiles + E->off_recx_idx;
+    a.injx = sw == 0 ? srcx : recx;
     a.explosive = E->explosive;
     a.snap_prec = E->snap_prec;
     if (snapshots || sw == 1) {
@@ -730,12 +786,14 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
     int unit = c.DTINV;
     while (unit < 32 && NT % (unit * 2) == 0) unit *= 2;
     const int kind = sw;
+    // the one-kernel step takes one slot per work item
+    LaunchShape ls = E->shape;
+    if (E->fused) ls.spb = 1;
     const int key[6] = {E->nsrc, E->ntr,
-                        (snapshots ? 1 : 0) + 2 * (int)(E->shape.cpml_weight * 4.0f) + 64 * E->shape.rows_v +
-                            1024 * E->shape.lds_v,
+                        (snapshots ? 1 : 0) + 2 * (int)(ls.cpml_weight * 4.0f) + 64 * ls.rows_v +
+                            1024 * ls.lds_v + 4096 * (E->fused ? 1 : 0),
                         E->nslots,
-                        ((E->shape.order * 8 + E->shape.spb) * 4096 + E->shape.grid_v) * 4096 + E->shape.grid, unit};
-    const LaunchShape ls = E->shape;
+                        ((ls.order * 8 + ls.spb) * 4096 + ls.grid_v) * 4096 + ls.grid, unit};
     StepArgs a = shaped_args(step_args(E, sw, snapshots), ls);
     const StepArgs *da = nullptr;
     if (E->use_graphs) {
@@ -747,8 +805,12 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
             std::lock_guard<std::mutex> capture_guard(device_lock());
             HIP_TRY(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
             for (int k = 0; k < unit; k++) {
-                launch_vel_step(a, da, ls, k, E->stream);
-                launch_str_step(a, da, ls, k, E->stream);
+                if (E->fused) {
+                    launch_fused_step(a, da, ls, k, E->stream);
+                } else {
+                    launch_vel_step(a, da, ls, k, E->stream);
+                    launch_str_step(a, da, ls, k, E->stream);
+                }
             }
             launch_tick(E->ctl, unit, E->stream);
             HIP_TRY(hipStreamEndCapture(E->stream, &graph));
@@ -767,10 +829,15 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
         a.base = nullptr;  // the step number is passed directly
         ENG_TRY(upload_args(E, a, 3, &da));
         for (int nt = 1; nt <= NT; nt++) {
-            launch_vel_step(a, da, ls, nt, E->stream);
-            launch_str_step(a, da, ls, nt, E->stream);
+            if (E->fused) {
+                launch_fused_step(a, da, ls, nt, E->stream);
+            } else {
+                launch_vel_step(a, da, ls, nt, E->stream);
+                launch_str_step(a, da, ls, nt, E->stream);
+            }
         }
     }
+    E->last_set = E->fused ? (NT & 1) : 0;
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -825,6 +892,7 @@ extern "C" int ifos_engine_forward(ifos_engine *E, int store_snapshots)
     if (!E->tiles) return ifos_set_error("set_shot must be called before forward"), -2;
     if (store_snapshots) ENG_TRY(ensure_snapshots(E));
     E->snap_valid = false;
+    E->slot_grad_valid = false;
     ENG_TRY(zero_state(E));
     // energies of this shot's wavefields start at zero (src/IFOS2D.c:1623-1631)
     if (E->ep_mode && store_snapshots)
@@ -848,7 +916,8 @@ extern "C" int ifos_engine_seismograms(ifos_engine *E, float *vx, float *vy)
 extern "C" int ifos_engine_fields(ifos_engine *E, float *out5)
 {
     HIP_TRY(hipSetDevice(E->device));
-    float *src[5] = {E->vx, E->vy, E->sxx, E->syy, E->sxy};
+    float *set0[5] = {E->vx, E->vy, E->sxx, E->syy, E->sxy}, *set1[5] = {E->vx2, E->vy2, E->sxx2, E->syy2, E->sxy2};
+    float **src = E->last_set ? set1 : set0;
     for (int k = 0; k < 5; k++)
         HIP_TRY(hipMemcpy2DAsync(out5 + (size_t)k * E->plane, sizeof(float) * E->c.NX, src[k] + pad_index(E, 1, 1),
                                  sizeof(float) * E->pitch, sizeof(float) * E->c.NX, E->c.NY, hipMemcpyDeviceToHost,
@@ -1233,6 +1302,7 @@ extern "C" int ifos_engine_adjoint(ifos_engine *E)
     if (!E->snap || !E->snap_valid)
         return ifos_set_error("adjoint needs a forward run with snapshots on the current model and shot"), -2;
     if (!E->tiles || !E->model_set) return ifos_set_error("adjoint needs set_model and set_shot"), -2;
+    E->slot_grad_valid = false;
     ENG_TRY(zero_state(E));
     HIP_TRY(hipMemsetAsync(E->wcbuf, 0, sizeof(float) * 3 * E->plane * E->nslots, E->stream));
     ENG_TRY(run_loop(E, 1, false));
@@ -1251,7 +1321,11 @@ extern "C" int ifos_engine_adjoint(ifos_engine *E)
     ga.nslots = E->nslots;
     ga.g_vp = E->g_vp; ga.g_vs = E->g_vs; ga.g_rho = E->g_rho;
     ga.s_vp = E->s_vp; ga.s_vs = E->s_vs; ga.s_rho = E->s_rho;
+    // each slot's gradient replaces its correlations (no longer needed)
+    ga.slot_g = E->wcbuf;
+    ga.slot_gs = 3 * E->plane;
     launch_gradient(ga, E->stream);
+    E->slot_grad_valid = true;
     return check_nan(E);
 }
 
@@ -1272,6 +1346,35 @@ extern "C" int ifos_engine_gradient(ifos_engine *E, int which, float *g_vp, floa
     if (g_vs) HIP_TRY(hipMemcpyAsync(g_vs, s1, pb, hipMemcpyDeviceToHost, E->stream));
     if (g_rho) HIP_TRY(hipMemcpyAsync(g_rho, s2, pb, hipMemcpyDeviceToHost, E->stream));
     HIP_TRY(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+extern "C" int ifos_engine_slot_gradient(ifos_engine *E, int slot, float *g_vp, float *g_vs, float *g_rho)
+{
+    HIP_TRY(hipSetDevice(E->device));
+    if (slot < 0 || slot >= E->nslots) return ifos_set_error("slot %d: the engine has %d shots set", slot, E->nslots), -2;
+    if (!E->slot_grad_valid) return ifos_set_error("no adjoint run since the last forward run"), -2;
+    const size_t pb = sizeof(float) * E->plane;
+    const float *b = E->wcbuf + (long)slot * 3 * E->plane;
+    if (g_vp) HIP_TRY(hipMemcpyAsync(g_vp, b, pb, hipMemcpyDeviceToHost, E->stream));
+    if (g_vs) HIP_TRY(hipMemcpyAsync(g_vs, b + E->plane, pb, hipMemcpyDeviceToHost, E->stream));
+    if (g_rho) HIP_TRY(hipMemcpyAsync(g_rho, b + 2 * E->plane, pb, hipMemcpyDeviceToHost, E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+extern "C" int ifos_engine_set_fused(ifos_engine *E, int on)
+{
+    HIP_TRY(hipSetDevice(E->device));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    if ((on != 0) != E->fused) drop_graphs(E);
+    E->fused = on != 0;
+    return 0;
+}
+
+extern "C" int ifos_engine_get_fused(ifos_engine *E, int *on)
+{
+    *on = E->fused ? 1 : 0;
     return 0;
 }
 
@@ -1309,10 +1412,10 @@ extern "C" size_t ifos_engine_bytes(const ifos_config *c, int slots, int snap_pr
     const size_t field = pitch * (size_t)(c->NY + 2 * H + 1);
     const size_t plane = (size_t)c->NX * c->NY;
     const int fw2 = c->FW > 0 ? 2 * c->FW : 1;
-    size_t b = sizeof(float) * field * (size_t)(10 + 3 * c->L) * slots;   // wavefields
+    size_t b = sizeof(float) * field * (size_t)(15 + 3 * c->L) * slots;   // wavefields, both sets
     b += sizeof(float) * field * (size_t)(15 + 3 * c->L);                 // materials
     b += sizeof(float) * plane * (11 + 3 * (size_t)slots);                // interior + correlation
-    b += sizeof(float) * (size_t)(5 * c->NY * fw2 + 4 * fw2 * c->NX) * slots;  // CPML memory
+    b += sizeof(float) * (size_t)(7 * c->NY * fw2 + 6 * fw2 * c->NX) * slots;  // CPML memory
     b += sizeof(float) * (size_t)6 * ntr_total * c->NT;                   // seismograms etc.
     b += (sizeof(float) * c->NT + 3 * sizeof(int) + 2 * sizeof(double)) * (size_t)nsrc_total;
     b += (3 * sizeof(int) + 2 * sizeof(double)) * (size_t)ntr_total;
@@ -1459,6 +1562,12 @@ extern "C" double ifos_engine_kernel_bytes_per_cell(ifos_engine *E, int kernel)
         if (L == 0) return 4.0 * (2 + 3 + 3 + 3);  // read vx,vy,sxx,syy,sxy,f,g,uipjp; write sxx,syy,sxy
         // read vx,vy, sxx,syy,sxy, 3L memory variables, 6 moduli/tau; write 3 stresses, 3L memory variables
         return 4.0 * (2 + 3 + 3 * L + 6 + 3 + 3 * L);
+    case 2:
+        // one-kernel step: read vx,vy,sxx,syy,sxy,rip,rjp and the stress
+        // materials (3 elastic, 6 viscoelastic), 3L memory variables; write
+        // vx,vy,sxx,syy,sxy and 3L memory variables
+        if (L == 0) return 4.0 * (5 + 2 + 3 + 5);
+        return 4.0 * (5 + 2 + 6 + 3 * L + 5 + 3 * L);
     default: return 0.0;
     }
 }
@@ -1477,9 +1586,18 @@ extern "C" int ifos_engine_time_kernel(ifos_engine *E, int kernel, int reps, dou
     hipEvent_t t0, t1;
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
+    LaunchShape ls1 = E->shape;
+    ls1.spb = 1;
+    if (kernel == 2) {
+        a = shaped_args(step_args(E, 0, false), ls1);
+        a.base = nullptr;
+        a.nan_flag = nullptr;
+        ENG_TRY(upload_args(E, a, 3, &da));
+    }
     auto one = [&]() {
         if (kernel == 0) launch_vel_step(a, da, E->shape, 2, E->stream);
-        else launch_str_step(a, da, E->shape, 2, E->stream);
+        else if (kernel == 1) launch_str_step(a, da, E->shape, 2, E->stream);
+        else launch_fused_step(a, da, ls1, 2, E->stream);
     };
     one();
     HIP_TRY(hipEventRecord(t0, E->stream));
@@ -1576,10 +1694,38 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
                 pick.grid = grid_s;
             }
         }
+    // the one-kernel step (one shot per workgroup): every CPML cost weight,
+    // work order and grid, timed back to back
+    bool fused = false;
+    for (float cw : {0.0f, 1.0f, 2.0f})
+        for (int order = 0; order < (E->nslots > 1 ? 2 : 1); order++)
+            for (int per_cu : {0, 2, 3, 4}) {
+                LaunchShape ls = pick;
+                ls.spb = 1;
+                ls.order = order;
+                ls.cpml_weight = cw;
+                ls.grid = per_cu ? (per_cu * ncu + 7) / 8 * 8 : 0;
+                const StepArgs a = shaped_args(a0, ls);
+                ENG_TRY(upload_args(E, a, 3, &da));
+                launch_fused_step(a, da, ls, 2, E->stream);
+                HIP_TRY(hipEventRecord(ev[0], E->stream));
+                for (int k = 0; k < reps; k++) launch_fused_step(a, da, ls, 2, E->stream);
+                HIP_TRY(hipEventRecord(ev[1], E->stream));
+                HIP_TRY(hipEventSynchronize(ev[1]));
+                float mf = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&mf, ev[0], ev[1]));
+                mf /= reps;
+                if (mf < best) {
+                    best = mf;
+                    pick = ls;
+                    fused = true;
+                }
+            }
     for (auto &x : ev) hipEventDestroy(x);
     hipEventDestroy(t0);
     hipEventDestroy(t1);
     E->shape = pick;
+    E->fused = fused;
     drop_graphs(E);
     ENG_TRY(zero_state(E));
     HIP_TRY(hipStreamSynchronize(E->stream));

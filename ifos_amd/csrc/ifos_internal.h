@@ -197,6 +197,12 @@ struct GradArgs {
     // holds Vp, Vs and rho coefficients [3][n] at tf + s * tfs (null: none)
     const float *tf;
     long tfs;
+    // each slot's own gradient after the preconditioning and tapers, before
+    // the sum (null: not kept): Vp, Vs, rho at slot_g + s * slot_gs + {0, n, 2n}.
+    // It may be the correlation buffer itself: a thread reads its cell's
+    // correlations of a slot before it writes that cell's gradients there
+    float *slot_g;
+    long slot_gs;
 };
 
 // adjoint sources from seismograms (src/calc_res.c, LNORM 2)

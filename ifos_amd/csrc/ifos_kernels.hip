@@ -473,6 +473,12 @@ IFOS_KERNEL void k_gradient(GradArgs a)
             svs *= t[a.n + e];
             srho *= t[2L * a.n + e];
         }
+        if (a.slot_g) {
+            float *sg = a.slot_g + (long)s * a.slot_gs + e;
+            sg[0] = svp;
+            sg[a.n] = svs;
+            sg[2L * a.n] = srho;
+        }
         gvp = s ? gvp + svp : svp;
         gvs = s ? gvs + svs : svs;
         grho = s ? grho + srho : srho;

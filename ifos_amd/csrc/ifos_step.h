@@ -43,7 +43,14 @@ struct StepArgs {
     float *psi_sxx_x, *psi_sxy_x, *psi_syy_y, *psi_sxy_y;
     float *psi_vxx, *psi_vyx, *psi_vyy, *psi_vxy, *psi_vxxs;
     long pss;
+    // second wavefield set of the one-kernel step (k_fused_step): step nt
+    // reads set (nt - 1) & 1 and writes set nt & 1 (set 0 is the one above)
+    float *vx2, *vy2, *sxx2, *syy2, *sxy2;
+    float *psi2_sxx_x, *psi2_sxy_x, *psi2_syy_y, *psi2_sxy_y;
     TileList inj, exp, rec;
+    // the injection points (sources forward, receivers adjoint) of each tile
+    // and of the N-cell rim around it, for k_fused_step
+    TileList injx;
     const int *explosive;      // [slot]: 1 when the slot's sources act as explosive (forward)
     // forward snapshots rho_x, rho_y, x, y, u, each [nsnap][NY][NX] per slot:
     // FP32 in snap (snap_prec 0), or 16-bit codes in snap16 (1: BF16,
@@ -100,6 +107,8 @@ struct LaunchShape {
 void launch_vel_step(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s);
 void launch_str_step(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s);
 void launch_tick(int *base, int by, hipStream_t s);
+// one kernel per time step (velocity and stress), spb 1; ls.grid as for the stress step
+void launch_fused_step(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s);
 // the argument block as the kernels of launch shape ls must see it (slot
 // groups and the division constants filled in): upload this one
 StepArgs shaped_args(const StepArgs &a, const LaunchShape &ls);

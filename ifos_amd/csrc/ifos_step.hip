@@ -800,6 +800,358 @@ __global__ void k_tick(int *base, int by)
 }
 
 // ---------------------------------------------------------------------------
+// One time step in one kernel.  A workgroup owns a 64x8 tile.  It first
+// stages the old stresses it needs in LDS, computes the new velocities of the
+// tile and of an N-cell rim around it (the rim's values are what the
+// neighbouring tiles compute for their own cells, from the same inputs and
+// the same arithmetic, so they are the same bits), then updates the tile's
+// stresses from those velocities.  Wavefields and the velocity-side CPML
+// memory are read from one buffer and written to the other (the buffers swap
+// every step), because a tile reads its neighbours' old values while they
+// write their new ones.  Per cell and step this moves one read and one write
+// of vx, vy, sxx, syy, sxy instead of the two kernels' two reads of the
+// stresses and of the velocities.
+template <int N>
+struct FusedLds {
+    static constexpr int VX = kTileX + 2 * N, VY = kTileY + 2 * N;  // velocity region
+    static constexpr int AX = kTileX + 4 * N - 1, SY = kTileY + 4 * N - 1;
+    float sxx[VY][AX];   // rows j0+1-N .., columns i0+2-2N ..
+    float syy[SY][VX];   // rows j0+2-2N .., columns i0+1-N ..
+    float sxy[SY][AX];   // rows j0+1-2N .., columns i0+1-2N ..
+    float vx[VY][VX];    // new velocities, rows j0+1-N .., columns i0+1-N ..
+    float vy[VY][VX];
+    float acc[2][kTileY][kTileX];  // accelerations of the tile's cells (forward snapshot steps)
+};
+
+// velocity-side CPML (the arithmetic of cpml_velocity) reading the memory of
+// the previous step from the *_r arrays and, when `own`, writing the new
+// memory to the *_w arrays
+__device__ __forceinline__ void cpml_velocity_rw(const StepGeom &g, int j, int i, bool own, const float *rxx,
+                                                 const float *rxy, const float *ryy, const float *ryx, float *wxx,
+                                                 float *wxy, float *wyy, float *wyx, float &sxx_x, float &sxy_x,
+                                                 float &sxy_y, float &syy_y)
+{
+    const Cpml &pm = g.pml;
+    auto side_x = [&](int h) {
+        const long o = (long)(j - 1) * g.px + h;
+        const float p1 = cmem(pm.b_xh)[h] * gmem(rxx)[o] + cmem(pm.a_xh)[h] * sxx_x;
+        sxx_x = divK(sxx_x, pm.K_xh, h, g.K_one) + p1;
+        const float p2 = cmem(pm.b_x)[h] * gmem(rxy)[o] + cmem(pm.a_x)[h] * sxy_x;
+        sxy_x = divK(sxy_x, pm.K_x, h, g.K_one) + p2;
+        if (own) {
+            gmem(wxx)[o] = p1;
+            gmem(wxy)[o] = p2;
+        }
+    };
+    auto side_y = [&](int h) {
+        const long o = (long)h * g.py + (i - 1);
+        const float p1 = cmem(pm.b_yh)[h] * gmem(ryy)[o] + cmem(pm.a_yh)[h] * syy_y;
+        syy_y = divK(syy_y, pm.K_yh, h, g.K_one) + p1;
+        const float p2 = cmem(pm.b_y)[h] * gmem(ryx)[o] + cmem(pm.a_y)[h] * sxy_y;
+        sxy_y = divK(sxy_y, pm.K_y, h, g.K_one) + p2;
+        if (own) {
+            gmem(wyy)[o] = p1;
+            gmem(wyx)[o] = p2;
+        }
+    };
+    if (!g.BOUNDARY && i <= g.FW) side_x(i - 1);
+    if (!g.BOUNDARY && i >= g.NX - g.FW + 1) side_x(i - g.NX + 2 * g.FW - 1);
+    if (!g.FREE_SURF && j <= g.FW) side_y(j - 1);
+    if (j >= g.NY - g.FW + 1) side_y(j - g.NY + 2 * g.FW - 1);
+}
+
+template <int N, int L>
+__global__ __launch_bounds__(512) void k_fused_step(const StepArgs *__restrict__ pa, int k, int persistent,
+                                                     int flags)
+{
+    using T = FusedLds<N>;
+    constexpr int VX = T::VX, VY = T::VY, AX = T::AX, SY = T::SY;
+    const StepArgs &a = *pa;
+    __shared__ T st;
+    __shared__ float s_syy[kTileY][kTileX];
+    __shared__ float s_sxy[kTileY][kTileX];
+    __shared__ float s_red[3][kTileY];
+    const StepGeom &g = a.g;
+    const ItemMap m = item_map(a);
+    const int nt = step_number(a, k);
+    const int tid = threadIdx.y * kTileX + threadIdx.x, tx = threadIdx.x, ty = threadIdx.y;
+    const int P = g.pitch, NX = g.NX, NY = g.NY, H = N + 1;
+    // buffers: step nt reads set (nt - 1) & 1 and writes set nt & 1
+    const bool odd = (nt & 1) != 0;
+    const float *rvx = odd ? a.vx : a.vx2, *rvy = odd ? a.vy : a.vy2;
+    const float *rsxx = odd ? a.sxx : a.sxx2, *rsyy = odd ? a.syy : a.syy2, *rsxy = odd ? a.sxy : a.sxy2;
+    float *wvx = odd ? a.vx2 : a.vx, *wvy = odd ? a.vy2 : a.vy;
+    float *wsxx = odd ? a.sxx2 : a.sxx, *wsyy = odd ? a.syy2 : a.syy, *wsxy = odd ? a.sxy2 : a.sxy;
+    const float *rp_xx = odd ? a.psi_sxx_x : a.psi2_sxx_x, *rp_xy = odd ? a.psi_sxy_x : a.psi2_sxy_x;
+    const float *rp_yy = odd ? a.psi_syy_y : a.psi2_syy_y, *rp_yx = odd ? a.psi_sxy_y : a.psi2_sxy_y;
+    float *wp_xx = odd ? a.psi2_sxx_x : a.psi_sxx_x, *wp_xy = odd ? a.psi2_sxy_x : a.psi_sxy_x;
+    float *wp_yy = odd ? a.psi2_syy_y : a.psi_syy_y, *wp_yx = odd ? a.psi2_sxy_y : a.psi_sxy_y;
+    const bool snap_step = (flags & kSnapStep) != 0, corr_step = (flags & kCorrStep) != 0;
+    const bool fwd = (flags & kForward) != 0;
+    const bool rates = snap_step && a.VELOCITY == 1;
+    int first, end, stride;
+    work_range(m, m.ntiles * m.ngroups, persistent != 0, first, end, stride);
+    for (int it = first; it < end; it += stride) {
+        int blk, s;
+        item_of(m, it, 1, blk, s);
+        blk = __builtin_amdgcn_readfirstlane(blk);
+        s = __builtin_amdgcn_readfirstlane(s);
+        const int by = (int)fdiv(blk, m.nbx_mul, m.nbx_sh), bx = blk - by * m.nbx;
+        const int i0 = bx * kTileX, j0 = by * kTileY;
+        const long wo = (long)s * a.ws, po = (long)s * a.pss;
+        // ---- stage the old stresses (zero outside the padded grid, where
+        // they only feed velocities of cells outside the model)
+        auto stage = [&](const float *src, float *dst, int rows, int cols, int jr, int ic) {
+            const auto sg = gmem(src) + wo;
+            for (int q = tid; q < rows * cols; q += kTileX * kTileY) {
+                const int r = q / cols, cc = q - r * cols;
+                const int gj = jr + r, gi = ic + cc;
+                const bool in = gj >= 1 - H && gj <= NY + H && gi >= -15 && gi <= P - 16;
+                dst[q] = in ? sg[(long)gj * P + gi] : 0.0f;
+            }
+        };
+        __syncthreads();  // the previous item's readers are done with the LDS
+        stage(rsxx, &st.sxx[0][0], VY, AX, j0 + 1 - N, i0 + 2 - 2 * N);
+        stage(rsyy, &st.syy[0][0], SY, VX, j0 + 2 - 2 * N, i0 + 1 - N);
+        stage(rsxy, &st.sxy[0][0], SY, AX, j0 + 1 - 2 * N, i0 + 1 - 2 * N);
+        __syncthreads();
+        // ---- velocities of the tile and its rim
+        const int so = __builtin_amdgcn_readfirstlane(s * (a.nbx * a.nby + 1) + blk);  // uniform
+        const int q0 = cmem(a.injx.start)[so], q1 = cmem(a.injx.start)[so + 1];
+        for (int q = tid; q < VX * VY; q += kTileX * kTileY) {
+            const int ry = q / VX, rx = q - ry * VX;
+            const int gj = j0 + 1 - N + ry, gi = i0 + 1 - N + rx;
+            float vx = 0.0f, vy = 0.0f;
+            if (gj >= 1 && gj <= NY && gi >= 1 && gi <= NX) {
+                const bool own = rx >= N && rx < N + kTileX && ry >= N && ry < N + kTileY;
+                const float *X = &st.sxx[ry][rx + N - 1];
+                const float *Y = &st.syy[ry + N - 1][rx];
+                const float *S = &st.sxy[ry + N][rx + N];
+                float sxx_x = g.hc[1] * (X[1] - X[0]);
+                float sxy_x = g.hc[1] * (S[0] - S[-1]);
+                float sxy_y = g.hc[1] * (S[0] - S[-AX]);
+                float syy_y = g.hc[1] * (Y[VX] - Y[0]);
+#pragma unroll
+                for (int mm = 2; mm <= N; mm++) {
+                    sxx_x = sxx_x + g.hc[mm] * (X[mm] - X[1 - mm]);
+                    sxy_x = sxy_x + g.hc[mm] * (S[mm - 1] - S[-mm]);
+                    sxy_y = sxy_y + g.hc[mm] * (S[(mm - 1) * AX] - S[-mm * AX]);
+                    syy_y = syy_y + g.hc[mm] * (Y[mm * VX] - Y[(1 - mm) * VX]);
+                }
+                cpml_velocity_rw(g, gj, gi, own, rp_xx + po, rp_xy + po, rp_yy + po, rp_yx + po, wp_xx + po,
+                                 wp_xy + po, wp_yy + po, wp_yx + po, sxx_x, sxy_x, sxy_y, syy_y);
+                const long c = (long)gj * P + gi;
+                const float vx0 = gmem(rvx)[wo + c], vy0 = gmem(rvy)[wo + c];
+                const float rip = gmem(a.rip)[c], rjp = gmem(a.rjp)[c];
+                if (own) {
+                    if (fwd) {
+                        if (snap_step) {
+                            st.acc[0][ry - N][rx - N] = vel_rate(rip, sxx_x + sxy_y, g.DH);
+                            st.acc[1][ry - N][rx - N] = vel_rate(rjp, sxy_x + syy_y, g.DH);
+                        }
+                    } else if (flags & kAdjInt) {
+                        gmem(a.vxp1)[wo + c] += vx0 * g.DT;
+                        gmem(a.vyp1)[wo + c] += vy0 * g.DT;
+                    } else if (corr_step) {
+                        gmem(a.vxp1)[wo + c] = vx0;
+                        gmem(a.vyp1)[wo + c] = vy0;
+                    }
+                }
+                vx = vel_new(vx0, g.DT, rip, sxx_x + sxy_y, g.DH);
+                vy = vel_new(vy0, g.DT, rjp, sxy_x + syy_y, g.DH);
+                // point forces / residual injection at this cell, in list order
+                for (int qq = q0; qq < q1; qq++) {
+                    const int l = cmem(a.injx.idx)[qq];
+                    if (cmem(a.injx.ix)[l] != gi || cmem(a.injx.iy)[l] != gj) continue;
+                    const float s0 = cmem(a.injx.sig)[(long)l * a.NT + nt - 1];
+                    const float s1 = cmem(a.injx.sig1)[(long)l * a.NT + nt - 1];
+                    if (a.sw == 0) {
+                        const int type = cmem(a.injx.type)[l];
+                        if (type == 2) vx += (g.DT * rip * s1) / (g.DH * g.DH);
+                        if (type == 3) vy += (g.DT * rjp * s1) / (g.DH * g.DH);
+                        if (type == 4) {
+                            vx += (g.DT * rip * cmem(a.injx.sin_azi)[l] * s0) / (g.DH * g.DH);
+                            vy += (g.DT * rjp * cmem(a.injx.cos_azi)[l] * s0) / (g.DH * g.DH);
+                        }
+                    } else {
+                        if (a.ADJOINT_TYPE == 1) { vx += s0; vy += s1; }
+                        if (a.ADJOINT_TYPE == 2) vy += s1;
+                        if (a.ADJOINT_TYPE == 3) vx += s0;
+                    }
+                }
+                if (own) {
+                    gmem(wvx)[wo + c] = vx;
+                    gmem(wvy)[wo + c] = vy;
+                }
+            }
+            st.vx[ry][rx] = vx;
+            st.vy[ry][rx] = vy;
+        }
+        __syncthreads();
+        // ---- the tile's cells: one per thread
+        const int i = i0 + 1 + tx, j = j0 + 1 + ty;
+        const bool live = i <= NX && j <= NY;
+        const bool top = by == 0 && g.FREE_SURF;
+        const long c = (long)j * P + i, e = (long)(j - 1) * NX + (i - 1);
+        const float nvx = st.vx[ty + N][tx + N], nvy = st.vy[ty + N][tx + N];
+        if (fwd && snap_step) {
+            const float val[2] = {st.acc[0][ty][tx], st.acc[1][ty][tx]};
+            snap_store<2>(a, s, 0, (nt - 1) / a.DTINV, e, blk, live, val, s_red);
+        }
+        if ((flags & kEnergy) && live) {
+            const auto W = gmem(a.wacc) + (long)s * a.wacs;
+            W[e] += ((nvx * nvx) + (nvy * nvy));
+        }
+        int expl = 0, ex0 = 0, ex1 = 0, rc0 = 0, rc1 = 0;
+        if (fwd) {
+            expl = cmem(a.explosive)[s];
+            ex0 = cmem(a.exp.start)[so];
+            ex1 = cmem(a.exp.start)[so + 1];
+            rc0 = cmem(a.rec.start)[so];
+            rc1 = cmem(a.rec.start)[so + 1];
+        }
+        float sxx = 0.0f, syy = 0.0f, sxy = 0.0f, ux = 0.0f, uy = 0.0f, uxy = 0.0f;
+        float rr[L > 0 ? L : 1], pp[L > 0 ? L : 1], qq[L > 0 ? L : 1];
+        StressMat<L> M;
+        M.fe = M.ge = M.fipjp = 0.0f;
+        M.vc = ViscCell{};
+        const float *V = &st.vx[ty + N][tx + N], *Wv = &st.vy[ty + N][tx + N];
+        if (live) {
+            if (L == 0) {
+                M.fe = gmem(a.f)[c];
+                M.ge = gmem(a.gl)[c];
+                M.fipjp = gmem(a.uipjp)[c];
+            } else {
+                visc_coeffs(L, g.DT, gmem(a.mu)[c], gmem(a.pi)[c], gmem(a.muipjp)[c], gmem(a.taus)[c],
+                            gmem(a.taup)[c], gmem(a.tausipjp)[c], M.vc);
+            }
+            const float scale = L ? (float)(1.0 / (double)g.DH) : g.DT / g.DH;
+            float vxx = g.hc[1] * (V[0] - V[-1]);
+            float vyx = g.hc[1] * (Wv[1] - Wv[0]);
+            float vxy = g.hc[1] * (V[VX] - V[0]);
+            float vyy = g.hc[1] * (Wv[0] - Wv[-VX]);
+#pragma unroll
+            for (int mm = 2; mm <= N; mm++) {
+                vxx = vxx + g.hc[mm] * (V[mm - 1] - V[-mm]);
+                vyx = vyx + g.hc[mm] * (Wv[mm] - Wv[1 - mm]);
+                vxy = vxy + g.hc[mm] * (V[mm * VX] - V[(1 - mm) * VX]);
+                vyy = vyy + g.hc[mm] * (Wv[(mm - 1) * VX] - Wv[-mm * VX]);
+            }
+            vxx = vxx * scale;
+            vyx = vyx * scale;
+            vxy = vxy * scale;
+            vyy = vyy * scale;
+            cpml_stress(g, j, i, a.psi_vxx + po, a.psi_vyx + po, a.psi_vyy + po, a.psi_vxy + po, vxx, vyx, vxy, vyy);
+            sxx = st.sxx[ty + N][tx + 2 * N - 1];
+            syy = st.syy[ty + 2 * N - 1][tx + N];
+            sxy = st.sxy[ty + 2 * N][tx + 2 * N];
+            if (L == 0) {
+                if (rates) rates_elastic(M.fe, M.ge, M.fipjp, g.DT, vxx, vyx, vxy, vyy, ux, uy, uxy);
+                stress_elastic(M.fe, M.ge, M.fipjp, vxx, vyx, vxy, vyy, sxx, syy, sxy);
+            } else {
+#pragma unroll
+                for (int l = 0; l < L; l++) {
+                    rr[l] = gmem(a.r)[wo + c + (long)l * a.ls];
+                    pp[l] = gmem(a.p)[wo + c + (long)l * a.ls];
+                    qq[l] = gmem(a.q)[wo + c + (long)l * a.ls];
+                }
+                stress_visc<(L > 0 ? L : 1)>(M.vc, a.eta, a.bip, a.bjm, a.cip, a.cjm, g.DT, vxx, vyx, vxy, vyy, sxx,
+                                           syy, sxy, rr, pp, qq, rates, ux, uy, uxy);
+            }
+            if (expl) {
+                for (int q = ex0; q < ex1; q++) {
+                    const int l = cmem(a.exp.idx)[q];
+                    if (cmem(a.exp.ix)[l] != i || cmem(a.exp.iy)[l] != j) continue;
+                    const float amp = explosive_amp(a.exp.sig + (long)l * a.NT - 1, nt, a.NT, g.DH);
+                    sxx = add_half(sxx, amp);
+                    syy = add_half(syy, amp);
+                }
+            }
+        }
+        if (top) {
+            s_syy[ty][tx] = live ? syy : 0.0f;
+            s_sxy[ty][tx] = live ? sxy : 0.0f;
+            __syncthreads();
+            if (live && j == 1) {
+                syy = 0.0f;
+                if (rates) uy = 0.0f;
+                if (L) {
+#pragma unroll
+                    for (int l = 0; l < L; l++) qq[l] = 0.0f;
+                }
+#pragma unroll
+                for (int mm = 1; mm <= N; mm++) {
+                    gmem(wsyy)[wo + c - (long)mm * P] = -s_syy[mm][tx];
+                    gmem(wsxy)[wo + c - (long)mm * P] = -s_sxy[mm - 1][tx];
+                }
+                // surface_derivs on the new velocities of row 1 (from LDS)
+                float vxx = 0.0f, vyy = 0.0f;
+#pragma unroll
+                for (int mm = 1; mm <= N; mm++) {
+                    vxx += g.hc[mm] * (V[mm - 1] - V[-mm]);
+                    vyy += g.hc[mm] * (Wv[(mm - 1) * VX] - Wv[-mm * VX]);
+                }
+                const float dh24 = 1.0 / (double)g.DH;
+                vxx *= dh24;
+                vyy *= dh24;
+                const Cpml &pm = g.pml;
+                auto surf = [&](int h) {
+                    auto ps = gmem(a.psi_vxxs + po) + h;
+                    *ps = cmem(pm.b_x)[h] * *ps + cmem(pm.a_x)[h] * vxx;
+                    vxx = divK(vxx, pm.K_x, h, g.K_one) + *ps;
+                };
+                if (!g.BOUNDARY && i <= g.FW) surf(i - 1);
+                if (!g.BOUNDARY && i >= g.NX - g.FW + 1) surf(i - g.NX + 2 * g.FW - 1);
+                if (L == 0) {
+                    surface_elastic_update((float)((double)M.fe * 2.0), M.ge, g.DT, vxx, vyy, sxx, rates, ux);
+                } else {
+                    surface_visc_update<(L > 0 ? L : 1)>(M.vc.mu, M.vc.pi, M.vc.taus, M.vc.taup, a.eta, g.DT, vxx,
+                                                         vyy, sxx, pp, rates, ux);
+                }
+            }
+        }
+        if (snap_step) {
+            const float val[3] = {a.VELOCITY ? ux : sxx, a.VELOCITY ? uy : syy, a.VELOCITY ? uxy : sxy};
+            snap_store<3>(a, s, 2, (nt - 1) / a.DTINV, e, blk, live, val, s_red);
+        }
+        if (!live) continue;
+        gmem(wsxx)[wo + c] = sxx;
+        gmem(wsyy)[wo + c] = syy;
+        gmem(wsxy)[wo + c] = sxy;
+        if (L) {
+#pragma unroll
+            for (int l = 0; l < L; l++) {
+                gmem(a.r)[wo + c + (long)l * a.ls] = rr[l];
+                gmem(a.p)[wo + c + (long)l * a.ls] = pp[l];
+                gmem(a.q)[wo + c + (long)l * a.ls] = qq[l];
+            }
+        }
+        if (corr_step) {
+            const long ks = (a.NT - nt) / a.DTINV, wco = (long)s * a.wcs;
+            float wr = gmem(a.wc_rho)[wco + e], w = gmem(a.wc)[wco + e], wu = gmem(a.wc_u)[wco + e];
+            correlate_cell(gmem(a.vxp1)[wo + c], gmem(a.vyp1)[wo + c], sxx, syy, sxy, snap_load(a, s, 0, ks, e, blk),
+                           snap_load(a, s, 1, ks, e, blk), snap_load(a, s, 2, ks, e, blk),
+                           snap_load(a, s, 3, ks, e, blk), snap_load(a, s, 4, ks, e, blk), gmem(a.muss)[e],
+                           gmem(a.lamss)[e], gmem(a.vs_i)[e] > 0.0f, wr, w, wu);
+            gmem(a.wc_rho)[wco + e] = wr;
+            gmem(a.wc)[wco + e] = w;
+            gmem(a.wc_u)[wco + e] = wu;
+        }
+        if (fwd) {
+            for (int q = rc0; q < rc1; q++) {
+                const int t = cmem(a.rec.idx)[q];
+                if (cmem(a.rec.ix)[t] != i || cmem(a.rec.iy)[t] != j) continue;
+                gmem(a.seis_vx)[(long)t * a.NT + nt - 1] = nvx;
+                gmem(a.seis_vy)[(long)t * a.NT + nt - 1] = nvy;
+            }
+        }
+        if (a.nan_flag && j == NY / 2 && i == NX / 2) {
+            if (isnan(nvy) && *gmem(a.nan_flag) == 0) *gmem(a.nan_flag) = nt;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // multiplier and shift of fdiv for divisor d >= 1 (the round-up method for
 // unsigned division by an invariant: l = ceil(log2 d),
 // mul = floor(2^32 (2^l - d) / d) + 1, shift l - 1)
@@ -945,6 +1297,41 @@ void launch_str_step(const StepArgs &a, const StepArgs *da, const LaunchShape &l
     case 4: str_n<4>(a, da, ls, k, s); break;
     case 5: str_n<5>(a, da, ls, k, s); break;
     case 6: str_n<6>(a, da, ls, k, s); break;
+    default: break;
+    }
+}
+
+template <int N, int L>
+static void fused_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
+{
+    const int gr = grid_of(a, ls, ls.grid);
+    hipLaunchKernelGGL((k_fused_step<N, L>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid > 0 ? 1 : 0,
+                       step_flags(a, k));
+}
+
+template <int N>
+static void fused_n(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
+{
+    switch (a.L) {
+    case 0: fused_launch<N, 0>(a, da, ls, k, s); break;
+    case 1: fused_launch<N, 1>(a, da, ls, k, s); break;
+    case 2: fused_launch<N, 2>(a, da, ls, k, s); break;
+    case 3: fused_launch<N, 3>(a, da, ls, k, s); break;
+    case 4: fused_launch<N, 4>(a, da, ls, k, s); break;
+    case 5: fused_launch<N, 5>(a, da, ls, k, s); break;
+    default: break;
+    }
+}
+
+void launch_fused_step(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
+{
+    switch (a.g.N) {
+    case 1: fused_n<1>(a, da, ls, k, s); break;
+    case 2: fused_n<2>(a, da, ls, k, s); break;
+    case 3: fused_n<3>(a, da, ls, k, s); break;
+    case 4: fused_n<4>(a, da, ls, k, s); break;
+    case 5: fused_n<5>(a, da, ls, k, s); break;
+    case 6: fused_n<6>(a, da, ls, k, s); break;
     default: break;
     }
 }

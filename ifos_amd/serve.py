@@ -25,13 +25,14 @@ class _Result(C.Structure):
     _fields_ = [("seis_vx", C.POINTER(C.c_float)), ("seis_vy", C.POINTER(C.c_float)),
                 ("g_vp", C.POINTER(C.c_float)), ("g_vs", C.POINTER(C.c_float)), ("g_rho", C.POINTER(C.c_float)),
                 ("misfit", C.POINTER(C.c_double)), ("times", C.c_double * 3), ("slots", C.c_int),
-                ("reused", C.c_int)]
+                ("reused", C.c_int), ("shared", C.c_int)]
 
 
 class Stats(C.Structure):
     _fields_ = [(n, C.c_longlong) for n in ("submitted", "finished", "failed", "engines_created", "engines_reused",
                                             "engines_evicted")] + \
-               [(n, C.c_size_t) for n in ("budget", "bytes_in_use", "peak_bytes")]
+               [(n, C.c_size_t) for n in ("budget", "bytes_in_use", "peak_bytes")] + \
+               [(n, C.c_longlong) for n in ("launches", "shared_launches")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -128,6 +129,7 @@ class Server:
         out["times"] = tuple(res.times)
         out["slots"] = res.slots
         out["reused"] = bool(res.reused)
+        out["shared"] = res.shared
         return out
 
     def stats(self):

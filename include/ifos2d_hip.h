@@ -138,10 +138,22 @@ int  ifos_engine_get_velocity_rows(ifos_engine *e, int *rows);
  * launch against 15.0 us for two rows from global memory. */
 int  ifos_engine_set_velocity_lds(ifos_engine *e, int on);
 int  ifos_engine_get_velocity_lds(ifos_engine *e, int *on);
+/* One kernel per time step instead of the velocity and stress kernels: each
+ * workgroup stages the old stresses of its 64x8 tile and a 2*FDORDER/2-cell
+ * margin in LDS, forms the new velocities of the tile and of an
+ * FDORDER/2-cell rim (what the neighbouring tiles form for those cells, the
+ * same bits), then the tile's new stresses.  The wavefields and the
+ * velocity-side CPML memory alternate between two sets from step to step,
+ * because tiles read their neighbours' old values.  Same bits as the
+ * two-kernel step; one shot per workgroup on this path. */
+int  ifos_engine_set_fused(ifos_engine *e, int on);
+int  ifos_engine_get_fused(ifos_engine *e, int *on);
 /* Time every candidate launch shape (and, with several slots, both work
  * orders) on one forward step of the current model and shots (reps launches
- * of each kernel), keep the fastest, zero the stepped wavefields; reports
- * the pick (grid: the stress step's) and its time per step. */
+ * of each kernel), the two-kernel step and the one-kernel step
+ * (ifos_engine_set_fused), keep the fastest, zero the stepped wavefields;
+ * reports the pick (grid: the stress step's or the one kernel's) and its
+ * time per step. */
 int  ifos_engine_autotune(ifos_engine *e, int reps, int *spb, int *grid, double *ms_per_step);
 /* Forward run (src/IFOS2D.c:1651-1894); snapshots every DTINV steps if asked. */
 int  ifos_engine_forward(ifos_engine *e, int store_snapshots);
@@ -168,6 +180,13 @@ int  ifos_engine_shot_gradient(ifos_engine *e, const float *obs_vx, const float 
  * Arrays [NY][NX]: Vp, Vs, rho gradients. */
 int  ifos_engine_gradient(ifos_engine *e, int which, float *g_vp, float *g_vs, float *g_rho);
 int  ifos_engine_reset_gradient(ifos_engine *e);
+/* The gradient of the shot in `slot` alone, from the last adjoint run (after
+ * its energy preconditioning and tapers, before the sum): what a one-slot
+ * engine's which = 0 gradient would be for that shot, bit for bit.  Kept
+ * until the next forward run or set_shots.  A caller that sums shots of
+ * several requests run in one launch adds these per request in shot order,
+ * as the reference's loop adds shots (src/IFOS2D.c:2703-2713). */
+int  ifos_engine_slot_gradient(ifos_engine *e, int slot, float *g_vp, float *g_vs, float *g_rho);
 /* Energy preconditioning of the gradient (EPRECOND, EPSILON_WE,
  * EPRECOND_PER_SHOT of src/IFOS2D.c): mode 1 weights with
  * sqrt(Ws*Wr), mode 3 with sqrt(Ws) times the receiver Green's-function
@@ -247,6 +266,41 @@ int  ifos_engine_time_kernel(ifos_engine *e, int kernel, int reps, double *ms_av
 double ifos_engine_kernel_bytes_per_cell(ifos_engine *e, int kernel);
 /* Synchronise and report the first NaN step seen by the stability probe (0: none). */
 int  ifos_engine_nan_step(ifos_engine *e);
+
+/* ------------------------------------------------------------------------ */
+/* FP64 build of the same scheme (no reference counterpart; the reference   */
+/* is FP32): the yardstick that reduced-precision results are measured      */
+/* against.  Fields, materials, residuals, correlation and gradients are    */
+/* double; the setup scalars (Holberg coefficients, CPML profiles, source   */
+/* wavelets, GSLS factors) are the reference's FP32 values.  The forward    */
+/* state is kept every segment_steps steps (a multiple of DTINV; 0 picks    */
+/* DTINV*ceil(sqrt(NT/DTINV))) and the adjoint rebuilds one segment of      */
+/* snapshots at a time, so memory is NT/C states plus C/DTINV snapshot      */
+/* slots instead of NT/DTINV slots.  Energy preconditioning, tapers and     */
+/* time filtering are not provided here.                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct ifos_fp64 ifos_fp64;
+int  ifos_fp64_create(const ifos_config *cfg, int device, int segment_steps, ifos_fp64 **out);
+void ifos_fp64_destroy(ifos_fp64 *e);
+/* Device bytes of such an engine (0 for an invalid configuration). */
+size_t ifos_fp64_bytes(const ifos_config *cfg, int segment_steps, int ntr, int nsrc);
+/* The segment length in use. */
+int  ifos_fp64_segment_steps(ifos_fp64 *e);
+/* Models as for ifos_engine_set_model (host [NY][NX] FP32). */
+int  ifos_fp64_set_model(ifos_fp64 *e, const float *vp, const float *vs, const float *rho, const float *taus,
+                         const float *taup);
+/* One shot, arguments as for ifos_engine_set_shot. */
+int  ifos_fp64_set_shot(ifos_fp64 *e, int nsrc, const float *srcpos, int ntr, const int *recpos);
+/* Forward, residual (LNORM 2) against host observed traces [ntr][NT],
+ * adjoint with the zero-lag correlation, chain rule; the shot's gradient is
+ * added to the running sums.  misfit[0] = L2, misfit[1] = data energy. */
+int  ifos_fp64_shot_gradient(ifos_fp64 *e, const float *obs_vx, const float *obs_vy, const int *kill,
+                             double *misfit);
+/* Seismograms of the last shot, [ntr][NT] each. */
+int  ifos_fp64_seismograms(ifos_fp64 *e, double *vx, double *vy);
+/* which = 0: the last shot's gradients, 1: the sums since the reset. */
+int  ifos_fp64_gradient(ifos_fp64 *e, int which, double *g_vp, double *g_vs, double *g_rho);
+int  ifos_fp64_reset_gradient(ifos_fp64 *e);
 
 #ifdef __cplusplus
 }

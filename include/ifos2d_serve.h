@@ -10,15 +10,21 @@
  *    requests of the same configuration, snapshot precision and slot count
  *    (and keep their model when the model id matches); idle engines are
  *    evicted, least recently used first, when a new request needs the room.
- *  - Continuous batching: worker threads each drive their own engine and
- *    stream, so requests admitted while others run overlap on the device; a
- *    request's shots advance several at a time in the engine's slots.
+ *  - Continuous batching: work is admitted as launches of up to max_slots
+ *    shots on one engine.  A launch takes the oldest waiting request's next
+ *    shots and fills its free slots with the next shots of later requests of
+ *    the same configuration, kind, snapshot precision and model id (!= 0),
+ *    so requests that arrive while others run join the next launch of their
+ *    kind.  Worker threads each drive their own engine and stream, so
+ *    launches of different requests overlap on the device.
  *  - Order: first come first served, except that requests behind one that
  *    does not fit may start while it waits; after 8 such overtakes the
  *    waiting request is served before any other.
  *
  * Results are bit-identical to running the request alone on one engine:
- * the kernels are deterministic and a request's shots are summed in order.
+ * the kernels are deterministic, each shot's gradient is taken from its own
+ * slot (ifos_engine_slot_gradient), and a request's shots are added to its
+ * sum in shot order in FP32, as a one-slot engine adds them.
  * Errors: 0 on success, negative on failure, message in ifos_last_error().
  */
 #ifndef IFOS2D_SERVE_H
@@ -59,14 +65,18 @@ typedef struct ifos_result {
     float *g_vp, *g_vs, *g_rho;           /* gradient: [NY][NX], shot sum     */
     double *misfit;                       /* gradient: [nshots][2] L2, energy */
     double times[3];                      /* submitted, started, finished (s) */
-    int slots;                            /* shots per launch it ran with     */
-    int reused;                           /* 1: it ran on a resident engine   */
+    int slots;                            /* most of its shots in one launch  */
+    int reused;                           /* 1: it started on a resident engine */
+    int shared;                           /* its launches that also carried
+                                             shots of other requests         */
 } ifos_result;
 
 typedef struct ifos_server_stats {
     long long submitted, finished, failed;
     long long engines_created, engines_reused, engines_evicted;
     size_t budget, bytes_in_use, peak_bytes;
+    long long launches, shared_launches;  /* engine launches; those carrying
+                                             shots of several requests      */
 } ifos_server_stats;
 
 typedef struct ifos_server ifos_server;

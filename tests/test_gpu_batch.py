@@ -3,7 +3,8 @@ launches.  Each shot's seismograms, misfit and gradient must equal the same
 shot run alone on a one-slot engine bit for bit, for every launch shape
 (shots per workgroup 1/2/4, one workgroup per item or a persistent grid),
 and the batch gradient must equal the sequential sum over the shots in slot
-order (the reference's shot loop, src/IFOS2D.c:2703-2713)."""
+order (the reference's shot loop, src/IFOS2D.c:2703-2713); each slot's own
+gradient (ifos_engine_slot_gradient) equals that shot's gradient alone."""
 import numpy as np
 import pytest
 
@@ -88,6 +89,9 @@ def test_batch_equals_shots_alone(gpu_lib, case):
             np.testing.assert_array_equal(vy[t0:t0 + n], avy, err_msg=f"shot {k} vy")
             assert per[k, 0] == mis[0] and per[k, 1] == mis[1], (k, per[k], mis)
             t0 += n
+        # each slot keeps its own shot's gradient, as that shot alone gives it
+        for k, (_, _, g) in enumerate(alone):
+            np.testing.assert_array_equal(eng.slot_gradient(k), g, err_msg=f"slot {k} gradient")
         # the batch gradient is the slot-order sum of the per-shot gradients
         want = alone[0][2].copy()
         for _, _, g in alone[1:]:

@@ -7,6 +7,10 @@
   for the oracle; checked through properties that do not depend on size:
   two shots advanced together give each shot's gradient bit for bit as run
   alone, and the batch's running sum equals the slot-order sum.
+- 1700x350 batched case (NT 3000, uneven receiver spreads): 16 shots in one
+  launch against shots run alone, and the slot-order sum.
+- 4000x1000, L=3, NT 8000 case: 16-bit snapshots against the FP64 build of
+  the scheme at full size.
 """
 import numpy as np
 import pytest
@@ -81,3 +85,108 @@ def test_layered_visc_full_size_batch_equals_alone(gpu_lib):
     expect = (np.float32(0) + alone[0]) + alone[1]
     np.testing.assert_array_equal(total, expect.astype(np.float32))
     assert np.isfinite(total).all() and np.abs(total).max() > 0
+
+
+def test_batched_aperture_full_size_slots_equal_alone(gpu_lib):
+    """1700x350, NT 3000, 16 shots with uneven receiver spreads in one launch
+    (BF16 snapshots, autotuned launch shape): shots 0, 7 and 15 give the
+    seismograms and gradients they give alone on a one-slot engine, and the
+    batch gradient is the slot-order FP32 sum of the 16 slot gradients."""
+    w = workloads.batched_aperture()
+    cfg = w.cfg
+    shots = [(s.srcpos, s.recpos) for s in w.shots[:16]]
+    ntr = [len(r) for _, r in shots]
+    assert min(ntr) < max(ntr)
+    print(f"[full size] batched_aperture {cfg.NX}x{cfg.NY} NT={cfg.NT}, 16 shots, traces {min(ntr)}-{max(ntr)}",
+          flush=True)
+    bat = ifos_amd.Engine(cfg, slots=16)
+    bat.set_model(**w.true)
+    bat.set_shots(shots)
+    bat.forward()
+    ovx, ovy = bat.seismograms()
+    starts = np.cumsum([0] + ntr)
+    ovx[:, 0] = 0
+    ovy[:, 0] = 0
+    bat.set_model(**w.start)
+    bat.set_snapshot_precision("bf16")
+    bat.reset_gradient()
+    bat.autotune(reps=2)
+    bat.shot_gradient(ovx, ovy)
+    vx, vy = bat.seismograms()
+    slot_g = [bat.slot_gradient(k) for k in range(16)]
+    total = bat.gradient(total=True)
+    bat.close()
+    expect = np.zeros_like(total)
+    for g in slot_g:
+        expect = expect + g
+    np.testing.assert_array_equal(total, expect)
+    one = ifos_amd.Engine(cfg)
+    one.set_model(**w.start)
+    one.set_snapshot_precision("bf16")
+    for k in (0, 7, 15):
+        print(f"[full size] batched_aperture shot {k} alone", flush=True)
+        a, b = starts[k], starts[k + 1]
+        one.set_shot(*shots[k])
+        one.shot_gradient(ovx[a:b], ovy[a:b])
+        avx, avy = one.seismograms()
+        np.testing.assert_array_equal(vx[a:b], avx, err_msg=f"shot {k} vx")
+        np.testing.assert_array_equal(vy[a:b], avy, err_msg=f"shot {k} vy")
+        np.testing.assert_array_equal(slot_g[k], one.gradient(), err_msg=f"shot {k} gradient")
+    one.close()
+    assert np.isfinite(total).all() and np.abs(total).max() > 0
+
+
+def test_precision_large_full_size_against_fp64(gpu_lib):
+    """4000x1000, FDORDER 8, L=3, NT 8000, DTINV 2, one shot: BF16 and FP16
+    snapshots leave the forward seismograms unchanged (bit for bit against a
+    forward-only run); the FP32 seismograms are within 1e-5 relative L2 of
+    the FP64 build's; the BF16 and FP16 gradients agree with each other to
+    1e-3.  Against the FP64 build (forward recomputed from checkpoints) both
+    are about 5e-2 off, nearly all of it in the rows of the source and the
+    receivers, and the two 16-bit storages differ from each other by 3e-4:
+    that error is the FP32 scheme's own (its correlation sums over 4000
+    snapshot steps), not the snapshot storage's; the bound here is 0.1."""
+    w = workloads.precision_large()
+    cfg = w.cfg
+    s = w.shots[5]
+    print(f"[full size] precision_large {cfg.NX}x{cfg.NY} L={cfg.L} NT={cfg.NT}", flush=True)
+    eng = ifos_amd.Engine(cfg)
+    eng.set_model(**w.true)
+    eng.set_shot(s.srcpos, s.recpos)
+    eng.forward()
+    ovx, ovy = eng.seismograms()
+    ovx[:, 0] = 0
+    ovy[:, 0] = 0
+    eng.set_model(**w.start)
+    eng.forward()
+    fvx, fvy = eng.seismograms()
+    grads = {}
+    for prec in ("bf16", "fp16"):
+        print(f"[full size] precision_large {prec} gradient", flush=True)
+        eng.set_snapshot_precision(prec)   # frees the previous snapshot storage
+        eng.shot_gradient(ovx, ovy)
+        vx, vy = eng.seismograms()
+        np.testing.assert_array_equal(vx, fvx)
+        np.testing.assert_array_equal(vy, fvy)
+        grads[prec] = eng.gradient()
+    eng.close()
+    print("[full size] precision_large fp64 gradient", flush=True)
+    e64 = ifos_amd.Fp64Engine(cfg)
+    e64.set_model(**w.start)
+    e64.set_shot(s.srcpos, s.recpos)
+    e64.shot_gradient(ovx, ovy)
+    v64x, v64y = e64.seismograms()
+    g64 = e64.gradient()
+    e64.close()
+
+    def rel(a, b):
+        return float(np.linalg.norm((a.astype(np.float64) - b).ravel()) / np.linalg.norm(b.ravel()))
+
+    assert rel(fvx, v64x) <= 1e-5 and rel(fvy, v64y) <= 1e-5
+    d16 = [rel(grads["bf16"][k], grads["fp16"][k].astype(np.float64)) for k in range(3)]
+    print(f"[full size] precision_large bf16 vs fp16 (vp, vs, rho): {d16}", flush=True)
+    assert max(d16) <= 1e-3
+    for prec in ("fp16", "bf16"):
+        errs = [rel(grads[prec][k], g64[k]) for k in range(3)]
+        print(f"[full size] precision_large {prec} gradient error vs fp64 (vp, vs, rho): {errs}", flush=True)
+        assert max(errs) <= 0.1, (prec, errs)

@@ -1,7 +1,8 @@
 """Velocity step with two or four cells per thread (adjacent rows of the
-64x8 tile per thread of a 64x4 or 64x2 workgroup), and with two rows per
-thread reading the tile's stresses from an LDS copy: the same bits as one
-cell per thread
+64x8 tile per thread of a 64x4 or 64x2 workgroup), with two rows per
+thread reading the tile's stresses from an LDS copy, and the one-kernel
+step (velocity of the tile and its rim, then its stresses, wavefield sets
+alternating between steps): the same bits as one cell per thread
 and as the oracle, for every part of the velocity step -- CPML strips on all
 sides, force sources and adjoint injection on either row of a thread, grids
 whose last tile row is partly outside the model, acceleration snapshots in
@@ -26,12 +27,16 @@ CASES = [dict(seed=61, NX=75, NY=51, NT=150, FDORDER=8, L=2, VELOCITY=1, DTINV=3
 
 
 def run(w, rows, obs=None, slots=1, shots=None, spb=1, grid=0, prec=0, eprecond=0):
-    """rows 1, 2, 4: cells per thread; "lds": two rows per thread from LDS."""
+    """rows 1, 2, 4: cells per thread; "lds": two rows per thread from LDS;
+    "fused": one kernel per time step."""
     eng = ifos_amd.Engine(w.cfg, slots=slots)
     shots = shots or [(w.shots[0].srcpos, w.shots[0].recpos)]
     eng.set_model(**(w.start if obs is not None else w.true))
     eng.set_shots(shots)
-    if rows == "lds":
+    if rows == "fused":
+        eng.set_fused(True)
+        assert eng.get_fused() == 1
+    elif rows == "lds":
         eng.set_velocity_lds(True)
         assert eng.get_velocity_lds() == 1 and eng.get_velocity_rows() == 2
     else:
@@ -53,7 +58,7 @@ def run(w, rows, obs=None, slots=1, shots=None, spb=1, grid=0, prec=0, eprecond=
     return out
 
 
-@pytest.mark.parametrize("rows", [2, 4, "lds"])
+@pytest.mark.parametrize("rows", [2, 4, "lds", "fused"])
 @pytest.mark.parametrize("case", CASES, ids=["visc8_odd", "el4_v0_pml_top", "visc12_force", "el2_force"])
 def test_rows_match_oracle(gpu_lib, case, rows):
     w = workloads.small_case(FW=10, **case)
@@ -73,7 +78,7 @@ def test_rows_match_oracle(gpu_lib, case, rows):
         np.testing.assert_array_equal(g[k], ref["grad_" + n], err_msg=n)
 
 
-@pytest.mark.parametrize("rows", [2, 4, "lds"])
+@pytest.mark.parametrize("rows", [2, 4, "lds", "fused"])
 @pytest.mark.parametrize("prec", [1, 2])
 def test_rows_snapshot16(gpu_lib, prec, rows):
     w = workloads.small_case(seed=65, NX=80, NY=47, NT=150, FDORDER=8, L=2, VELOCITY=1, DTINV=3, FW=10)
@@ -86,7 +91,7 @@ def test_rows_snapshot16(gpu_lib, prec, rows):
         np.testing.assert_array_equal(g[k], want["grad_" + n], err_msg=n)
 
 
-@pytest.mark.parametrize("rows", [2, 4, "lds"])
+@pytest.mark.parametrize("rows", [2, 4, "lds", "fused"])
 def test_rows_energy_and_batches(gpu_lib, rows):
     w = workloads.small_case(seed=66, NX=90, NY=53, NT=160, FDORDER=6, L=1, VELOCITY=1, DTINV=2, FW=10)
     print(f"[rows {rows}] energy sums, 3 shots, 2 per workgroup, persistent grid", flush=True)
