@@ -57,27 +57,73 @@ def build_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def host_info():
+    """Sockets, cores and threads of the host, from /proc/cpuinfo."""
+    sockets, cores, model = set(), set(), ""
+    cpus = 0
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "processor":
+                cpus += 1
+            elif k == "physical id":
+                phys = v
+                sockets.add(v)
+            elif k == "core id":
+                core = v
+                cores.add((phys, core))
+            elif k == "model name" and not model:
+                model = v
+    except OSError:
+        pass
+    return {"cpu": model, "sockets": len(sockets) or None, "physical_cores": len(cores) or None,
+            "logical_cpus": cpus or os.cpu_count(), "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OMP_PLACES": os.environ.get("OMP_PLACES"), "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "mpi_ranks": "none (one process, one thread)"}
+
+
 def cpu_baseline(w, steps=300):
-    """Reference kernels (oracle/_ref) on the host: `steps` forward time steps
-    of the same grid, scaled to shot-gradients/s as 1 / (2 * NT * t_step)."""
+    """The reference's own kernels and inline loops (oracle/_ref, compiled
+    from its sources) on one host core: one shot gradient of the same grid
+    with all receivers -- forward run with the snapshot store, adjoint run
+    with residual injection and the zero-lag correlation, chain rule -- over
+    a reduced number of time steps (`steps` in all: steps / 2 forward and
+    steps / 2 adjoint, rounded to the snapshot stride), scaled to the full
+    NT: shot-gradients/s = 1 / (t * NT / NT_sample).  The reference has no
+    threads; MPI is not used."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     cfg = w.cfg
+    nt = max(cfg.DTINV, (steps // 2) // cfg.DTINV * cfg.DTINV)
     fields = {k: getattr(cfg, k) for k, _ in cfg._fields_ if k != "FL"}
-    fields["NT"] = steps
+    fields["NT"] = nt
     p = O.make_params(FL=[cfg.FL[i] for i in range(1, 8)], **fields)
     shot = w.shots[0]
-    kind = "reference" if O.ref_available() else "port"
-    which = "ref" if kind == "reference" else "oracle"
-    t0 = time.perf_counter()
-    O.forward(p, w.start, shot.srcpos, shot.recpos[:8], which=which)
-    t = time.perf_counter() - t0
-    t_step = t / steps
-    rate = 1.0 / (2 * cfg.NT * t_step)
-    return dict(value=rate, unit="shot-gradients/s", cores=1, kind=kind,
-                sample=f"{steps} forward time steps of the {cfg.NX}x{cfg.NY} L={cfg.L} FDORDER={cfg.FDORDER} grid "
-                       f"({t:.1f} s, {cfg.NX * cfg.NY * steps / t / 1e6:.2f} Mcell-updates/s), scaled to one "
-                       f"forward+adjoint of NT={cfg.NT}; snapshot and correlation work not counted")
+    ntr = len(shot.recpos)
+    rng = np.random.default_rng(0)
+    diff_vx = (1e-3 * rng.standard_normal((ntr, nt))).astype(np.float32)
+    diff_vy = (1e-3 * rng.standard_normal((ntr, nt))).astype(np.float32)
+    if O.ref_available():
+        kind = "reference"
+        t0 = time.perf_counter()
+        O.ref_gradient(p, w.start, shot.srcpos, shot.recpos, diff_vx, diff_vy)
+        t = time.perf_counter() - t0
+        what = "the reference's kernels and main's snapshot/correlation/chain-rule loops (oracle/_ref)"
+    else:
+        kind = "port"
+        obs = np.zeros((ntr, nt), np.float32)
+        t0 = time.perf_counter()
+        O.gradient(p, w.start, shot.srcpos, shot.recpos, obs, obs)
+        t = time.perf_counter() - t0
+        what = "the oracle's restatement (oracle/liboracle.so)"
+    rate = 1.0 / (t * cfg.NT / nt)
+    return dict(value=rate, unit="shot-gradients/s", cores=1, kind=kind, host=host_info(),
+                sample=f"one shot gradient of the {cfg.NX}x{cfg.NY} L={cfg.L} FDORDER={cfg.FDORDER} grid with all "
+                       f"{ntr} receivers over NT={nt} (forward with snapshots every {cfg.DTINV} steps, adjoint with "
+                       f"the correlation, chain rule) by {what}: {t:.1f} s, "
+                       f"{2 * cfg.NX * cfg.NY * nt / t / 1e6:.2f} Mcell-updates/s, scaled to NT={cfg.NT}")
 
 
 def serving(args):
@@ -406,6 +452,103 @@ def precision_errors(args):
     return 0
 
 
+def batched_vs_plain(args):
+    """The same shots' gradients on the one-shot-at-a-time FP32 path and on
+    the batched path (args.batch shots per launch, args.snap snapshots),
+    each autotuned, in one JSON
+    line with their shot-gradients/s, the ratio, the algorithmic bytes per
+    shot-gradient of each, and the batched gradient's error against the
+    FP32 path's.  Observed traces come from untimed forward runs on the true
+    model."""
+    import ifos_amd
+    w = build_workload(args.workload)
+    cfg = w.cfg
+    S = max(2, args.batch)
+    n = S * max(1, args.steps)
+    shots = [(s.srcpos, s.recpos) for s in w.shots[:n]]
+    eng = ifos_amd.Engine(cfg, slots=S)
+    eng.set_model(**w.true)
+    obs = []
+    for b in range(0, n, S):
+        eng.set_shots(shots[b:b + S])
+        eng.forward()
+        vx, vy = eng.seismograms()
+        vx[:, 0] = 0.0
+        vy[:, 0] = 0.0
+        obs.append((ifos_amd.DeviceBuffer.from_array(vx), ifos_amd.DeviceBuffer.from_array(vy)))
+    eng.close()
+    log(f"[batched] {w.name}: {n} shots, observed data done")
+    cells = cfg.NX * cfg.NY
+    nsnap = cfg.NT // cfg.DTINV
+
+    def algorithmic_bytes(e, snap16):
+        step = e.kernel_bytes_per_cell(0) + e.kernel_bytes_per_cell(1)
+        store = 10.0 if snap16 else 20.0
+        # correlation step: the snapshots, vxp1/vyp1 written and read, the
+        # three sums read and written, the three moduli read
+        corr = (10.0 if snap16 else 20.0) + 8 + 8 + 24 + 12
+        return cells * (2.0 * cfg.NT * step + nsnap * (store + corr))
+
+    # plain: one shot per launch, FP32 snapshots, plain sums
+    one = ifos_amd.Engine(cfg)
+    one.set_model(**w.start)
+    one.set_shot(*shots[0])
+    one.autotune(reps=5)
+    ntr = [len(r) for _, r in shots]
+    hosts = [(o[0].to_array((sum(ntr[b:b + S]), cfg.NT)), o[1].to_array((sum(ntr[b:b + S]), cfg.NT)))
+             for b, o in zip(range(0, n, S), obs)]
+    one.reset_gradient()
+    t0 = time.perf_counter()
+    for b in range(0, n, S):
+        t = 0
+        ovx, ovy = hosts[b // S]
+        for k in range(b, min(b + S, n)):
+            one.set_shot(*shots[k])
+            one.shot_gradient(ovx[t:t + ntr[k]], ovy[t:t + ntr[k]])
+            t += ntr[k]
+        log(f"[batched] plain path: {min(b + S, n)}/{n} shots")
+    t_plain = time.perf_counter() - t0
+    g_plain = one.gradient(total=True)
+    bytes_plain = algorithmic_bytes(one, False)
+    one.close()
+    # batched
+    bat = ifos_amd.Engine(cfg, slots=S)
+    bat.set_model(**w.start)
+    bat.set_snapshot_precision(args.snap)
+    bat.set_shots(shots[:S])
+    spb, grid, ms = bat.autotune(reps=5)
+    bat.reset_gradient()
+    t0 = time.perf_counter()
+    for b in range(0, n, S):
+        bat.set_shots(shots[b:b + S])
+        bat.shot_gradient(obs[b // S][0], obs[b // S][1])
+        log(f"[batched] batched path: {min(b + S, n)}/{n} shots")
+    t_bat = time.perf_counter() - t0
+    g_bat = bat.gradient(total=True)
+    bytes_bat = algorithmic_bytes(bat, args.snap != "fp32")
+    shape = dict(spb=spb, grids=list(bat.get_launch_grids()), slot_major=bat.get_work_order(),
+                 velocity_rows=bat.get_velocity_rows(), one_kernel_step=bat.get_fused())
+    bat.close()
+    rate_p, rate_b = n / t_plain, n / t_bat
+    rel = [float(np.linalg.norm((g_bat[k].astype(np.float64) - g_plain[k]).ravel()) /
+                 np.linalg.norm(g_plain[k].ravel().astype(np.float64))) for k in range(3)]
+    out = {"metric": "shot-gradients/s, batched vs one shot at a time", "value": round(rate_b, 4),
+           "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
+           "dtype": f"fp32 stencils, {args.snap} snapshots", "data": "synthetic (seeded layered model with faults)",
+           "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "NT": cfg.NT, "DTINV": cfg.DTINV, "shots": n,
+                      "batch": S, "snapshots": args.snap, "launch": shape},
+           "plain": {"shot_gradients_per_s": round(rate_p, 4), "algorithmic_bytes_per_shot_gradient": bytes_plain,
+                     "GB_s": round(bytes_plain * rate_p / 1e9, 1)},
+           "batched": {"shot_gradients_per_s": round(rate_b, 4), "algorithmic_bytes_per_shot_gradient": bytes_bat,
+                       "GB_s": round(bytes_bat * rate_b / 1e9, 1)},
+           "speedup": round(rate_b / rate_p, 3), "batched_vs_plain_rel_l2": {"vp": rel[0], "vs": rel[1], "rho": rel[2]}}
+    print(json.dumps(out))
+    for o in obs:
+        o[0].free()
+        o[1].free()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -437,6 +580,8 @@ def main():
                     help="time whole FWI iterations of the workload (gradient, PCG, step search, update)")
     ap.add_argument("--dropin", action="store_true",
                     help="one shot gradient through the drop-in layer (the reference's call sequence)")
+    ap.add_argument("--batched-vs-plain", action="store_true",
+                    help="the same shots on the plain FP32 path and on the batched path (--batch, --snap)")
     ap.add_argument("--precision-errors", action="store_true",
                     help="BF16 and FP16 snapshot errors against the FP64 build at the workload's full size")
     ap.add_argument("--shots", type=int, default=0, help="precision errors: first N shots (0: all)")
@@ -474,6 +619,8 @@ def main():
         return post_kernels(args)
     if args.precision_errors:
         return precision_errors(args)
+    if args.batched_vs_plain:
+        return batched_vs_plain(args)
     import ifos_amd
     w = build_workload(args.workload)
     cfg = w.cfg

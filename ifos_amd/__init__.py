@@ -208,6 +208,11 @@ class Engine:
         _check(lib().ifos_engine_set_shots(self.h, len(shots), _fp(ns, C.c_int), _fp(srcpos), _fp(nr, C.c_int),
                                            _fp(recpos, C.c_int)))
 
+    def set_wavelets(self, sig):
+        """Source signals [nsrc][NT] of the current shots' sources instead of the generated wavelets."""
+        sig = _f32(np.asarray(sig, np.float32).reshape(self.nsrc, self.cfg.NT))
+        _check(lib().ifos_engine_set_wavelets(self.h, _fp(sig)))
+
     def shot_misfit(self):
         """(L2, energy) per shot of the last residual, FP64."""
         out = np.zeros((self.nshots, 2), np.float64)
@@ -328,6 +333,10 @@ class Engine:
     def set_eprecond(self, mode, epsilon_we=0.005, per_shot=False):
         """Energy preconditioning of the gradient (EPRECOND 1 or 3; 0 off)."""
         _check(lib().ifos_engine_set_eprecond(self.h, mode, epsilon_we, int(per_shot)))
+
+    def set_eprecond_fp64(self, on):
+        """Energy sums of the preconditioner in FP64 (True) or FP32 as the reference (False)."""
+        _check(lib().ifos_engine_set_eprecond_fp64(self.h, int(bool(on))))
 
     def apply_eprecond(self):
         _check(lib().ifos_engine_apply_eprecond(self.h))

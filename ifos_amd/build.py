@@ -32,7 +32,7 @@ C_FLAGS = ["-O2", "-g", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-Wall",
            "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
 HIP_SOURCES = ["ifos_kernels.hip", "ifos_step.hip", "ifos_engine.hip", "ifos_dropin.hip", "ifos_opt.hip", "ifos_fp64.hip"]
-C_SOURCES = ["host/ifos_setup.c", "host/ifos_formats.c", "host/ifos_taper.c", "host/ifos_filter.c"]
+C_SOURCES = ["host/ifos_setup.c", "host/ifos_formats.c", "host/ifos_taper.c", "host/ifos_filter.c", "host/ifos_stf.c"]
 CXX_SOURCES = ["ifos_serve.cpp"]
 CXX_FLAGS = ["-O2", "-g", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 

@@ -162,6 +162,8 @@ struct ifos_engine {
     int ep_mode = 0, ep_per_shot = 0;
     float ep_eps = 0.0f;
     float *wbuf = nullptr;      // [slots][2][plane]: Ws then Wr (Ws becomes We after the adjoint)
+    int ep_fp64 = 0;            // Ws and Wr summed in FP64 (wbuf64) instead of FP32
+    double *wbuf64 = nullptr;   // [slots][2][plane] when ep_fp64
     float *we_sum = nullptr;    // [plane]
     double *ep_geom = nullptr;  // [plane], mode 3
     float *we_max = nullptr;    // [slots]
@@ -180,6 +182,7 @@ struct ifos_engine {
     int filt_nlow = 0, filt_nhigh = 0;
     double filt_c[2 * kMaxFiltSections][5];
     std::vector<float> sig_host;    // unfiltered wavelets of the current shots
+    bool sig_given = false;         // the wavelets were set by the caller (ifos_engine_set_wavelets)
     StepArgs *dargs = nullptr;  // device copies of the argument blocks: forward, forward with
                                 // snapshots, adjoint, eager/timing
 
@@ -388,7 +391,7 @@ extern "C" void ifos_engine_destroy(ifos_engine *E)
     drop_graphs(E);
     dfree(E->slab); dfree(E->mslab); dfree(E->interior); dfree(E->wcbuf); dfree(E->pml);
     dfree(E->psi); dfree(E->ctl); dfree(E->explosive); dfree(E->dargs);
-    dfree(E->wbuf); dfree(E->we_sum); dfree(E->ep_geom); dfree(E->we_max); dfree(E->red);
+    dfree(E->wbuf); dfree(E->wbuf64); dfree(E->we_sum); dfree(E->ep_geom); dfree(E->we_max); dfree(E->red);
     dfree(E->tap_m); dfree(E->tap_ij); dfree(E->tap_lohi); dfree(E->tfbuf);
     dfree(E->src_i); dfree(E->src_d); dfree(E->sig); dfree(E->rec_i); dfree(E->seis);
     dfree(E->trace_sums); dfree(E->snap); dfree(E->tiles);
@@ -554,6 +557,7 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
         filter_traces(E, E->sig, nsrc, c.SOURCE_SHAPE != 6);
         HIP_TRY(hipStreamSynchronize(E->stream));
         E->sig_host.swap(w);
+        E->sig_given = false;
     }
     if (ntr) {
         std::vector<int> ri(3 * (size_t)E->cap_tr, 0);
@@ -646,6 +650,20 @@ extern "C" int ifos_engine_set_shots(ifos_engine *E, int nshots, const int *nsrc
     }
     HIP_TRY(hipMemcpyAsync(E->tiles, blob.data(), sizeof(int) * blob.size(), hipMemcpyHostToDevice, E->stream));
     HIP_TRY(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+extern "C" int ifos_engine_set_wavelets(ifos_engine *E, const float *sig)
+{
+    HIP_TRY(hipSetDevice(E->device));
+    if (!E->nsrc) return ifos_set_error("set_wavelets needs shots with sources"), -2;
+    const size_t n = (size_t)E->nsrc * E->c.NT;
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    E->sig_host.assign(sig, sig + n);
+    E->sig_given = true;
+    HIP_TRY(hipMemcpyAsync(E->sig, E->sig_host.data(), sizeof(float) * n, hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    E->snap_valid = false;
     return 0;
 }
 
@@ -753,7 +771,8 @@ static StepArgs step_args(ifos_engine *E, int sw, bool snapshots)
     // energies for the preconditioner in gradient runs: Ws in the forward
     // run that stores snapshots, Wr in the adjoint (EPRECOND 1 only)
     if (E->ep_mode && ((sw == 0 && snapshots) || (sw == 1 && E->ep_mode == 1))) {
-        a.wacc = E->wbuf + (sw == 0 ? 0 : E->plane);
+        if (E->ep_fp64) a.wacc64 = E->wbuf64 + (sw == 0 ? 0 : E->plane);
+        else a.wacc = E->wbuf + (sw == 0 ? 0 : E->plane);
         a.wacs = 2 * E->plane;
     }
     a.base = E->ctl;
@@ -897,6 +916,8 @@ extern "C" int ifos_engine_forward(ifos_engine *E, int store_snapshots)
     // energies of this shot's wavefields start at zero (src/IFOS2D.c:1623-1631)
     if (E->ep_mode && store_snapshots)
         HIP_TRY(hipMemsetAsync(E->wbuf, 0, sizeof(float) * 2 * E->plane * E->nslots, E->stream));
+    if (E->ep_mode && store_snapshots && E->ep_fp64)
+        HIP_TRY(hipMemsetAsync(E->wbuf64, 0, sizeof(double) * 2 * E->plane * E->nslots, E->stream));
     ENG_TRY(run_loop(E, 0, store_snapshots != 0));
     ENG_TRY(check_nan(E));
     E->snap_valid = store_snapshots != 0;
@@ -1017,6 +1038,20 @@ __global__ void k_we_form(long n, int nslots, int mode, float *wbuf_, const doub
     }
 }
 
+// We of every slot from FP64 sums: mode 1 sqrt(Ws*Wr), mode 3 sqrt(Ws) times
+// the geometric receiver term, formed in FP64 and rounded to FP32 into wbuf
+__global__ void k_we_form64(long n, int nslots, int mode, const double *w64_, float *wbuf_, const double *geom_)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    for (int s = 0; s < nslots; s++) {
+        const auto W = gmem(w64_) + (long)s * 2 * n;
+        auto out = gmem(wbuf_) + (long)s * 2 * n;
+        if (mode == 1) out[e] = (float)sqrt(W[e] * W[n + e]);
+        else out[e] = (float)(sqrt(W[e]) * gmem(geom_)[e]);
+    }
+}
+
 // per-block maxima of |W| of one slot's We plane
 __global__ __launch_bounds__(256) void k_absmax_part(const float *w_, long n, float *part_)
 {
@@ -1073,7 +1108,12 @@ static int energy_weights(ifos_engine *E, GradArgs &ga)
 {
     const long n = E->plane;
     const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_we_form, dim3(nb), dim3(256), 0, E->stream, n, E->nslots, E->ep_mode, E->wbuf, E->ep_geom);
+    if (E->ep_fp64)
+        hipLaunchKernelGGL(k_we_form64, dim3(nb), dim3(256), 0, E->stream, n, E->nslots, E->ep_mode, E->wbuf64,
+                           E->wbuf, E->ep_geom);
+    else
+        hipLaunchKernelGGL(k_we_form, dim3(nb), dim3(256), 0, E->stream, n, E->nslots, E->ep_mode, E->wbuf,
+                           E->ep_geom);
     HIP_TRY(hipGetLastError());
     std::vector<float> eps(E->nslots), wmax(E->nslots);
     for (int s = 0; s < E->nslots; s++) {
@@ -1130,6 +1170,21 @@ extern "C" int ifos_engine_set_eprecond(ifos_engine *E, int mode, float epsilon_
         }
     HIP_TRY(hipMemcpyAsync(E->ep_geom, g.data(), sizeof(double) * n, hipMemcpyHostToDevice, E->stream));
     HIP_TRY(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+extern "C" int ifos_engine_set_eprecond_fp64(ifos_engine *E, int on)
+{
+    HIP_TRY(hipSetDevice(E->device));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    if ((on != 0) != (E->ep_fp64 != 0)) drop_graphs(E);
+    E->ep_fp64 = on != 0;
+    if (E->ep_fp64 && !E->wbuf64) {
+        if (dmalloc(&E->wbuf64, sizeof(double) * 2 * (size_t)E->plane * E->slots) != hipSuccess) {
+            E->ep_fp64 = 0;
+            return ifos_set_error("device allocation of the FP64 energy sums failed"), -3;
+        }
+    }
     return 0;
 }
 
@@ -1257,7 +1312,7 @@ extern "C" int ifos_engine_set_time_filter(ifos_engine *E, float F_LOW_PASS, int
         E->filt_nhigh = nh;
     }
     // wavelets of shots already set: filter afresh from the unfiltered ones
-    if (E->nsrc && !E->sig_host.empty()) {
+    if (E->nsrc && !E->sig_host.empty() && !E->sig_given) {
         HIP_TRY(hipMemcpyAsync(E->sig, E->sig_host.data(), sizeof(float) * E->sig_host.size(), hipMemcpyHostToDevice,
                                E->stream));
         filter_traces(E, E->sig, E->nsrc, E->c.SOURCE_SHAPE != 6);

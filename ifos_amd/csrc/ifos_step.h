@@ -68,6 +68,7 @@ struct StepArgs {
     long wcs;
     float *seis_vx, *seis_vy;   // [trace][NT], traces of all slots
     float *wacc;                // energy accumulator of the preconditioner (per slot, stride wacs), or null
+    double *wacc64;             // the same in FP64 (products formed in FP64), used instead when set
     long wacs;
     const int *base;            // steps completed before the current graph unit
     int *nan_flag;

@@ -49,7 +49,7 @@ static int step_flags(const StepArgs &a, int k)
     if (a.sw == 1 && (a.base ? (k + 1) % D == 0 : (a.NT - k) % D == 0)) f |= kCorrStep;
     if (a.sw == 0) f |= kForward;
     if (a.sw == 1 && a.VELOCITY == 0) f |= kAdjInt;
-    if (a.wacc) f |= kEnergy;
+    if (a.wacc || a.wacc64) f |= kEnergy;
     return f;
 }
 
@@ -329,8 +329,13 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     vys[c] = vy;
     // energy of the new particle velocities (eprecond, src/eprecond.c:8-21)
     if (flags & kEnergy) {
-        const auto W = gmem(a.wacc) + (long)s * a.wacs;
-        W[e] += ((vx * vx) + (vy * vy));
+        if (a.wacc64) {
+            const auto W = gmem(a.wacc64) + (long)s * a.wacs;
+            W[e] += ((double)vx * (double)vx) + ((double)vy * (double)vy);
+        } else {
+            const auto W = gmem(a.wacc) + (long)s * a.wacs;
+            W[e] += ((vx * vx) + (vy * vy));
+        }
     }
 }
 
@@ -434,10 +439,18 @@ __device__ __forceinline__ void vel_rows_with(const StepArgs &a, int s, int blk,
             vys[c + r * P] = vy[r];
         }
     if (flags & kEnergy) {
-        const auto W = gmem(a.wacc) + (long)s * a.wacs;
+        if (a.wacc64) {
+            const auto W = gmem(a.wacc64) + (long)s * a.wacs;
 #pragma unroll
-        for (int r = 0; r < R; r++)
-            if (live[r]) W[e + r * NX] += ((vx[r] * vx[r]) + (vy[r] * vy[r]));
+            for (int r = 0; r < R; r++)
+                if (live[r])
+                    W[e + r * NX] += ((double)vx[r] * (double)vx[r]) + ((double)vy[r] * (double)vy[r]);
+        } else {
+            const auto W = gmem(a.wacc) + (long)s * a.wacs;
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if (live[r]) W[e + r * NX] += ((vx[r] * vx[r]) + (vy[r] * vy[r]));
+        }
     }
 }
 
@@ -730,8 +743,9 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
         const long ks = (a.NT - nt) / a.DTINV, wco = (long)s * a.wcs;
         float wr = gmem(a.wc_rho)[wco + e], w = gmem(a.wc)[wco + e], wu = gmem(a.wc_u)[wco + e];
         correlate_cell(gmem(a.vxp1)[wo + c], gmem(a.vyp1)[wo + c], sxx, syy, sxy, snap_load(a, s, 0, ks, e, blk),
-                       snap_load(a, s, 1, ks, e, blk), snap_load(a, s, 2, ks, e, blk), snap_load(a, s, 3, ks, e, blk),
-                       snap_load(a, s, 4, ks, e, blk), gmem(a.muss)[e], gmem(a.lamss)[e], gmem(a.vs_i)[e] > 0.0f, wr, w, wu);
+                       snap_load(a, s, 1, ks, e, blk), snap_load(a, s, 2, ks, e, blk),
+                       snap_load(a, s, 3, ks, e, blk), snap_load(a, s, 4, ks, e, blk), gmem(a.muss)[e],
+                       gmem(a.lamss)[e], gmem(a.vs_i)[e] > 0.0f, wr, w, wu);
         gmem(a.wc_rho)[wco + e] = wr;
         gmem(a.wc)[wco + e] = w;
         gmem(a.wc_u)[wco + e] = wu;
@@ -999,8 +1013,13 @@ __global__ __launch_bounds__(512) void k_fused_step(const StepArgs *__restrict__
             snap_store<2>(a, s, 0, (nt - 1) / a.DTINV, e, blk, live, val, s_red);
         }
         if ((flags & kEnergy) && live) {
-            const auto W = gmem(a.wacc) + (long)s * a.wacs;
-            W[e] += ((nvx * nvx) + (nvy * nvy));
+            if (a.wacc64) {
+                const auto W = gmem(a.wacc64) + (long)s * a.wacs;
+                W[e] += ((double)nvx * (double)nvx) + ((double)nvy * (double)nvy);
+            } else {
+                const auto W = gmem(a.wacc) + (long)s * a.wacs;
+                W[e] += ((nvx * nvx) + (nvy * nvy));
+            }
         }
         int expl = 0, ex0 = 0, ex1 = 0, rc0 = 0, rc1 = 0;
         if (fwd) {
@@ -1129,10 +1148,11 @@ __global__ __launch_bounds__(512) void k_fused_step(const StepArgs *__restrict__
         if (corr_step) {
             const long ks = (a.NT - nt) / a.DTINV, wco = (long)s * a.wcs;
             float wr = gmem(a.wc_rho)[wco + e], w = gmem(a.wc)[wco + e], wu = gmem(a.wc_u)[wco + e];
-            correlate_cell(gmem(a.vxp1)[wo + c], gmem(a.vyp1)[wo + c], sxx, syy, sxy, snap_load(a, s, 0, ks, e, blk),
-                           snap_load(a, s, 1, ks, e, blk), snap_load(a, s, 2, ks, e, blk),
-                           snap_load(a, s, 3, ks, e, blk), snap_load(a, s, 4, ks, e, blk), gmem(a.muss)[e],
-                           gmem(a.lamss)[e], gmem(a.vs_i)[e] > 0.0f, wr, w, wu);
+            correlate_cell(gmem(a.vxp1)[wo + c], gmem(a.vyp1)[wo + c], sxx, syy, sxy,
+                           snap_load(a, s, 0, ks, e, blk), snap_load(a, s, 1, ks, e, blk),
+                           snap_load(a, s, 2, ks, e, blk), snap_load(a, s, 3, ks, e, blk),
+                           snap_load(a, s, 4, ks, e, blk), gmem(a.muss)[e], gmem(a.lamss)[e],
+                           gmem(a.vs_i)[e] > 0.0f, wr, w, wu);
             gmem(a.wc_rho)[wco + e] = wr;
             gmem(a.wc)[wco + e] = w;
             gmem(a.wc_u)[wco + e] = wu;

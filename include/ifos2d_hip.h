@@ -106,6 +106,14 @@ int  ifos_engine_set_shot(ifos_engine *e, int nsrc, const float *srcpos, int ntr
  * [sum ntr][NT] in the same order.  set_shot is set_shots with one shot. */
 int  ifos_engine_set_shots(ifos_engine *e, int nshots, const int *nsrc, const float *srcpos, const int *ntr,
                            const int *recpos);
+/* The source signals of the current shots' sources, [nsrc][NT] in source
+ * order, instead of the wavelets set_shots generated: the wavelet read
+ * from a file (SOURCE_SHAPE 7, inseis_source_wavelet, src/IFOS2D.c:1527),
+ * e.g. the wavelet convolved with an inverted source time function
+ * (ifos2d_stf.h).  Used as given: the time filter is not applied to them,
+ * as the reference skips it with INV_STF (src/IFOS2D.c:1556).  Kept until
+ * the next set_shots. */
+int  ifos_engine_set_wavelets(ifos_engine *e, const float *sig);
 /* Per shot of the last residual: per_shot[2k] = L2, per_shot[2k+1] = energy (FP64). */
 int  ifos_engine_shot_misfit(ifos_engine *e, double *per_shot);
 /* Launch shape of the step kernels: spb = shots per workgroup (1, 2, 4; a
@@ -198,6 +206,11 @@ int  ifos_engine_slot_gradient(ifos_engine *e, int slot, float *g_vp, float *g_v
  * the summed gradient by that sum over its maximum (src/IFOS2D.c:2737-2751).
  * mode 0 turns it off. */
 int  ifos_engine_set_eprecond(ifos_engine *e, int mode, float epsilon_we, int per_shot);
+/* on = 1: the energy sums Ws and Wr accumulate in FP64 (each step adds
+ * vx*vx + vy*vy formed in FP64) and We is formed from them in FP64, then
+ * rounded to FP32 (no reference counterpart: src/eprecond.c:8-21 sums in
+ * FP32, as on = 0, the default, does bit for bit). */
+int  ifos_engine_set_eprecond_fp64(ifos_engine *e, int on);
 int  ifos_engine_apply_eprecond(ifos_engine *e);
 /* which = 0: We of the first slot's shot of the last adjoint run; 1: the
  * sum of 1/We since the last apply.  [NY][NX]. */

@@ -922,6 +922,17 @@ static void energy_add(const shot_t *S, real *W)
             W[(size_t)(j - 1) * NX + (i - 1)] += ((S->vx[j][i] * S->vx[j][i]) + (S->vy[j][i] * S->vy[j][i]));
 }
 
+/* the same sums in FP64 (EPRECOND_FP64; no reference counterpart) */
+static void energy_add64(const shot_t *S, double *W)
+{
+    const int NX = S->P.NX, NY = S->P.NY;
+    for (int j = 1; j <= NY; j++)
+        for (int i = 1; i <= NX; i++) {
+            const double vx = S->vx[j][i], vy = S->vy[j][i];
+            W[(size_t)(j - 1) * NX + (i - 1)] += (vx * vx) + (vy * vy);
+        }
+}
+
 static void run_forward(shot_t *S, int nsrc, const float *srcpos, int ntr, const int *recpos, real *seis_vx,
                         real *seis_vy, snaps_t *SN, int *dtinv_help, real *Ws)
 {
@@ -960,7 +971,8 @@ static void run_forward(shot_t *S, int nsrc, const float *srcpos, int ntr, const
             hin1 += p->DTINV;
             if (dtinv_help) dtinv_help[nt] = 1;
         }
-        if (Ws) energy_add(S, Ws);
+        if (Ws && S->P.EPRECOND_FP64) energy_add64(S, (double *)Ws);
+        else if (Ws) energy_add(S, Ws);
     }
     free(sig);
     free(rsig);
@@ -1145,8 +1157,10 @@ int orc_gradient_we(const orc_params *p, const float *vp, const float *vs, const
     /* energies of the forward and adjoint wavefields, reset per shot
      * (src/IFOS2D.c:1623-1631) */
     const int ep = p->EPRECOND == 1 || p->EPRECOND == 3;
-    real *Ws = ep ? (real *)calloc(plane, sizeof(real)) : NULL;
-    real *Wr = ep ? (real *)calloc(plane, sizeof(real)) : NULL;
+    /* with EPRECOND_FP64 the two arrays hold doubles */
+    const size_t wsz = p->EPRECOND_FP64 ? sizeof(double) : sizeof(real);
+    real *Ws = ep ? (real *)calloc(plane, wsz) : NULL;
+    real *Wr = ep ? (real *)calloc(plane, wsz) : NULL;
     run_forward(&S, nsrc, srcpos, ntr, recpos, seis_vx, seis_vy, &SN, help, Ws);
 
     /* residuals (src/IFOS2D.c:1965-2060): vx then vy, L2 carried over */
@@ -1196,7 +1210,10 @@ int orc_gradient_we(const orc_params *p, const float *vp, const float *vs, const
                 }
             hin++;
         }
-        if (p->EPRECOND == 1) energy_add(&S, Wr);
+        if (p->EPRECOND == 1) {
+            if (p->EPRECOND_FP64) energy_add64(&S, (double *)Wr);
+            else energy_add(&S, Wr);
+        }
     }
 
     /* chain rule to Vp, Vs, rho (src/IFOS2D.c:2471-2611, PARAMETERIZATION 1) */
@@ -1225,7 +1242,15 @@ int orc_gradient_we(const orc_params *p, const float *vp, const float *vs, const
         for (int j = 1; j <= NY; j++)
             for (int i = 1; i <= NX; i++) {
                 const size_t c = (size_t)(j - 1) * NX + (i - 1);
-                if (p->EPRECOND == 1) {
+                if (p->EPRECOND_FP64) {
+                    const double *W64s = (const double *)Ws, *W64r = (const double *)Wr;
+                    if (p->EPRECOND == 1) {
+                        We[c] = (real)sqrt(W64s[c] * W64r[c]);
+                    } else {
+                        const float x = i * p->DH, y = j * p->DH;
+                        We[c] = (real)(sqrt(W64s[c]) * (asinh((xmax - x) / y) - asinh((xmin - x) / y)));
+                    }
+                } else if (p->EPRECOND == 1) {
                     We[c] = sqrt(Ws[c] * Wr[c]);
                 } else {
                     const float x = i * p->DH, y = j * p->DH;

@@ -62,6 +62,9 @@ typedef struct orc_params {
                               (src/eprecond1.c)                              */
     int EPRECOND_PER_SHOT; /* 1: divide each shot's gradient by We/max(We)    */
     float EPSILON_WE;      /* regularisation of We, relative to max(We)      */
+    int EPRECOND_FP64;     /* 1: Ws and Wr summed in FP64 from products of the
+                              velocities formed in FP64, We from them in FP64
+                              (no reference counterpart)                     */
 } orc_params;
 
 /* Sources: srcpos[l*8 + k] holds the reference's srcpos_loc[k+1][l+1]

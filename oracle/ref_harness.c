@@ -941,3 +941,20 @@ void ref_inseis(const char *prefix, int shot, int ntr, int ns, float *out)
         for (int k = 1; k <= ns; k++) out[(size_t)(t - 1) * ns + k - 1] = sec[t][k];
     free_matrix(sec, 1, ntr, 1, ns + 1);
 }
+
+/* taper (src/taper.c:24, TAPER_STF of stf()) on one trace of ns samples,
+ * 0-based; the reference writes its window up to sample 2*ceil(2/fc/DT),
+ * so the trace is copied into a vector that long when that is beyond ns */
+void ref_stf_taper(float *trace, int ns, float fc, float dt)
+{
+    harness_setup_opt(1, 1);
+    DT = dt;
+    MYID = 0;
+    const int tl = (int)(ceil(2.0 / fc / DT));
+    const int n = ns > 2 * tl ? ns : 2 * tl;
+    float *v = vector(1, n);
+    for (int j = 1; j <= n; j++) v[j] = j <= ns ? trace[j - 1] : 0.0f;
+    taper(v, ns, fc);
+    for (int j = 1; j <= ns; j++) trace[j - 1] = v[j];
+    free_vector(v, 1, n);
+}

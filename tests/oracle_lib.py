@@ -22,7 +22,7 @@ _FLT_FIELDS = ["DH", "DT", "VPPML", "FPML", "npower", "k_max_PML", "F_REF", "TAU
 class OrcParams(C.Structure):
     _fields_ = ([(n, C.c_int) for n in _INT_FIELDS] + [(n, C.c_float) for n in _FLT_FIELDS]
                 + [("FL", C.c_float * 8), ("SNAP_PREC", C.c_int), ("EPRECOND", C.c_int),
-                   ("EPRECOND_PER_SHOT", C.c_int), ("EPSILON_WE", C.c_float)])
+                   ("EPRECOND_PER_SHOT", C.c_int), ("EPSILON_WE", C.c_float), ("EPRECOND_FP64", C.c_int)])
 
 
 def make_params(**kw):
@@ -284,24 +284,27 @@ def lbfgs_core(iteration, s, y, rho, q, alpha, which="oracle"):
     return r, rho, alpha, q
 
 
-def gradient_we(p, model, srcpos, recpos, obs_vx, obs_vy):
+def gradient_we(p, model, srcpos, recpos, obs_vx, obs_vy, fp64=False):
     """Oracle gradient with the energy preconditioner set in p; returns
-    (gradient dict, We [NY][NX])."""
+    (gradient dict, We [NY][NX]); fp64: the FP64 build of the scheme."""
     vp, vs, rho = (_f32(model[k]) for k in ("vp", "vs", "rho"))
     taus = _f32(model["taus"]) if p.L else None
     taup = _f32(model["taup"]) if p.L else None
     srcpos = _f32(srcpos).reshape(-1, 8)
     recpos = np.ascontiguousarray(recpos, dtype=np.int32).reshape(-1, 2)
     ntr = recpos.shape[0]
-    svx = np.zeros((ntr, p.NT), np.float32)
-    svy = np.zeros((ntr, p.NT), np.float32)
-    g = np.zeros((3, p.NY, p.NX), np.float32)
-    we = np.zeros((p.NY, p.NX), np.float32)
+    dt = np.float64 if fp64 else np.float32
+    ty = C.c_double if fp64 else C.c_float
+    svx = np.zeros((ntr, p.NT), dt)
+    svy = np.zeros((ntr, p.NT), dt)
+    g = np.zeros((3, p.NY, p.NX), dt)
+    we = np.zeros((p.NY, p.NX), dt)
     mis = np.zeros(5, np.float64)
-    rc = oracle().orc_gradient_we(C.byref(p), _ptr(vp), _ptr(vs), _ptr(rho), _ptr(taus), _ptr(taup),
-                                  C.c_int(srcpos.shape[0]), _ptr(srcpos), C.c_int(ntr), _ptr(recpos, C.c_int),
-                                  _ptr(_f32(obs_vx)), _ptr(_f32(obs_vy)), None, _ptr(svx), _ptr(svy), _ptr(g[0]),
-                                  _ptr(g[1]), _ptr(g[2]), _ptr(mis, C.c_double), _ptr(we))
+    rc = oracle(fp64).orc_gradient_we(C.byref(p), _ptr(vp), _ptr(vs), _ptr(rho), _ptr(taus), _ptr(taup),
+                                      C.c_int(srcpos.shape[0]), _ptr(srcpos), C.c_int(ntr), _ptr(recpos, C.c_int),
+                                      _ptr(_f32(obs_vx)), _ptr(_f32(obs_vy)), None, _ptr(svx, ty), _ptr(svy, ty),
+                                      _ptr(g[0], ty), _ptr(g[1], ty), _ptr(g[2], ty), _ptr(mis, C.c_double),
+                                      _ptr(we, ty))
     assert rc == 0, rc
     return dict(seis_vx=svx, seis_vy=svy, grad_vp=g[0], grad_vs=g[1], grad_rho=g[2], misfit=mis), we
 
@@ -518,3 +521,11 @@ def ref_inseis(prefix, shot, ntr, ns):
     out = np.zeros((ntr, ns), np.float32)
     ref().ref_inseis(prefix.encode(), shot, ntr, ns, _ptr(out))
     return out
+
+
+def ref_stf_taper(trace, fc, dt):
+    """taper() of src/taper.c (TAPER_STF) applied in place to a float32 trace."""
+    L = ref()
+    L.ref_stf_taper.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float]
+    L.ref_stf_taper.restype = None
+    L.ref_stf_taper(trace.ctypes.data_as(C.c_void_p), trace.shape[0], fc, dt)

@@ -1,6 +1,8 @@
 """Energy preconditioning on the GPU (EPRECOND 1 and 3, per shot or summed
 over shots) against the oracle, itself bit-identical to the reference's
-eprecond / eprecond1 (tests/test_oracle.py)."""
+eprecond / eprecond1 (tests/test_oracle.py); with the energy sums in FP32 as
+the reference forms them, and in FP64 (set_eprecond_fp64, the oracle's
+EPRECOND_FP64)."""
 import numpy as np
 import pytest
 
@@ -12,17 +14,19 @@ from test_precision import params_of
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("fp64", [0, 1], ids=["fp32_sums", "fp64_sums"])
 @pytest.mark.parametrize("mode,per_shot,L", [(1, 1, 0), (1, 0, 2), (3, 1, 2), (3, 0, 0)])
-def test_eprecond_matches_oracle(gpu_lib, mode, per_shot, L):
+def test_eprecond_matches_oracle(gpu_lib, mode, per_shot, L, fp64):
     w = workloads.small_case(seed=23 + mode + L, NX=90, NY=50, NT=200, FDORDER=4, L=L, VELOCITY=1, DTINV=2)
     shots = [w.shots[0]]
     sp2 = np.array(w.shots[0].srcpos, np.float32).reshape(-1, 8).copy()
     sp2[:, 0] += 20
     shots.append(type(w.shots[0])(srcpos=sp2, recpos=w.shots[0].recpos))
-    p = params_of(w, EPRECOND=mode, EPRECOND_PER_SHOT=per_shot, EPSILON_WE=0.005)
+    p = params_of(w, EPRECOND=mode, EPRECOND_PER_SHOT=per_shot, EPSILON_WE=0.005, EPRECOND_FP64=fp64)
     eng = ifos_amd.Engine(w.cfg, slots=2)
     eng.set_model(**w.start)
     eng.set_eprecond(mode, 0.005, per_shot)
+    eng.set_eprecond_fp64(fp64)
     eng.reset_gradient()
     obs, want, wes = [], [], []
     for s in shots:

@@ -261,6 +261,28 @@ def test_energy_preconditioner_matches_reference(mode, per_shot, L):
         np.testing.assert_allclose(g["grad_vp"], g0["grad_vp"] / scale, rtol=2e-6, atol=0)
 
 
+@pytest.mark.parametrize("mode", [1, 3])
+def test_energy_sums_in_fp64(mode):
+    """EPRECOND_FP64 (no reference counterpart: Ws, Wr summed in FP64): the
+    weights are not the FP32 ones bit for bit, but agree with them to the
+    FP32 rounding of a sum of positive terms (NT * 2^-24 relative), and with
+    the FP64 build of the whole scheme to the same order."""
+    w = workloads.small_case(seed=71 + mode, NX=52, NY=38, NT=110, FDORDER=4, L=0, VELOCITY=1, DTINV=2, FW=8)
+    s = w.shots[0]
+    p32 = params_of(w, EPRECOND=mode, EPRECOND_PER_SHOT=1, EPSILON_WE=0.005)
+    p64 = params_of(w, EPRECOND=mode, EPRECOND_PER_SHOT=1, EPSILON_WE=0.005, EPRECOND_FP64=1)
+    ovx, ovy, _ = O.forward(p32, w.true, s.srcpos, s.recpos)
+    ovx[:, 0] = 0
+    ovy[:, 0] = 0
+    _, we32 = O.gradient_we(p32, w.start, s.srcpos, s.recpos, ovx, ovy)
+    _, we64 = O.gradient_we(p64, w.start, s.srcpos, s.recpos, ovx, ovy)
+    assert not np.array_equal(we32, we64)
+    bound = 4 * w.cfg.NT * 2.0 ** -24
+    np.testing.assert_allclose(we64, we32, rtol=bound, atol=0)
+    _, we_all64 = O.gradient_we(p32, w.start, s.srcpos, s.recpos, ovx, ovy, fp64=True)
+    np.testing.assert_allclose(we64, we_all64, rtol=1e-4, atol=0)
+
+
 # ------------------------------------------------------------ golden data --
 @pytest.mark.parametrize("name", ["el4_fs_v0", "visc8_fs_v1", "el2_nofs_rot", "visc6_fs_fx_v1"])
 def test_oracle_reproduces_golden(name):
