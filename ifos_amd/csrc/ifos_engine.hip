@@ -1001,7 +1001,6 @@ extern "C" int ifos_engine_residual(ifos_engine *E, const float *obs_vx, const f
         ra.ntr = ntr; ra.NT = c.NT; ra.VELOCITY = c.VELOCITY; ra.DT = c.DT;
         ra.syn = seis_buf(E, comp); ra.obs = comp == 0 ? ovx : ovy; ra.kill = dkill; ra.diff = diff;
         ra.l2_trace = E->trace_sums; ra.en_trace = E->trace_sums + E->cap_tr;
-        ra.l2f_trace = ra.enf_trace = nullptr;
         launch_residual(ra, E->stream);
         HIP_TRY(hipMemcpyAsync(h.data(), E->trace_sums, sizeof(double) * ntr, hipMemcpyDeviceToHost, E->stream));
         HIP_TRY(hipMemcpyAsync(h.data() + ntr, E->trace_sums + E->cap_tr, sizeof(double) * ntr,

@@ -213,7 +213,6 @@ struct ResArgs {
     const int *kill;          // may be null
     float *diff;              // [ntr][NT], time reversed
     double *l2_trace, *en_trace;   // per-trace FP64 sums
-    float *l2f_trace, *enf_trace;  // per-trace FP32 running sums
 };
 
 // time-domain Butterworth filter of traces [ntr][ns] (stride between

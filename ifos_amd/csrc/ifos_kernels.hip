@@ -499,40 +499,69 @@ IFOS_KERNEL void k_gradient(GradArgs a)
 // ---------------------------------------------------------------------------
 // residual / adjoint source (src/calc_res.c:136-282, LNORM 2) with the
 // misfit and data energy (src/calc_misfit.c:193-194, src/calc_energy.c:157)
-// summed per trace in FP64.  One thread per trace: the time integration is
-// a sequential FP32 running sum exactly as in the reference.
+// summed per trace in FP64.
+//
+// One wavefront per trace, four per workgroup.  Samples are read and the
+// time-reversed adjoint source written 64 at a time (one contiguous 256-byte
+// row per load or store); each lane sums its own samples' squares in FP64
+// and a shuffle tree combines the 64 partial sums.  With VELOCITY 0 the
+// traces are integrated first by a sequential FP32 running sum as in the
+// reference: the sum is carried through the lanes of each 64-sample chunk
+// in sample order (a readlane per sample, the sum uniform across the wave),
+// so every partial sum has the reference's bits.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ float lane_value(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 IFOS_KERNEL void k_residual(ResArgs a)
 {
-    const int itr = blockIdx.x * blockDim.x + threadIdx.x;
-    if (itr >= a.ntr) return;
+    const int lane = threadIdx.x & 63;
+    const int itr = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (itr >= a.ntr) return;  // the whole wavefront leaves together
     const float *syn = a.syn + (long)itr * a.NT;
     const float *obs = a.obs + (long)itr * a.NT;
     float *diff = a.diff + (long)itr * a.NT;
+    const bool killed = a.kill && a.kill[itr] == 1;
     double l2 = 0.0, en = 0.0;
-    if (a.kill && a.kill[itr] == 1) {
-        for (int k = 0; k < a.NT; k++) diff[k] = 0.0f;
-    } else {
-        float is = 0.0f, id = 0.0f;
-        for (int k = 0; k < a.NT; k++) {
-            float ys, yd;
-            if (a.VELOCITY == 0) {
-                is += syn[k];
-                id += obs[k];
-                ys = is * a.DT;
-                yd = id * a.DT;
-            } else {
-                ys = syn[k];
-                yd = obs[k];
+    float is = 0.0f, id = 0.0f;  // running integrals (VELOCITY 0), equal in every lane
+    for (int k0 = 0; k0 < a.NT; k0 += 64) {
+        const int k = k0 + lane;
+        const bool in = k < a.NT;
+        if (killed) {
+            if (in) diff[a.NT - 1 - k] = 0.0f;
+            continue;
+        }
+        const float s = in ? syn[k] : 0.0f;
+        const float o = in ? obs[k] : 0.0f;
+        float ys = s, yd = o;
+        if (a.VELOCITY == 0) {
+            const int m = min(64, a.NT - k0);
+            for (int l = 0; l < m; l++) {
+                is += lane_value(s, l);
+                id += lane_value(o, l);
+                if (lane == l) {
+                    ys = is * a.DT;
+                    yd = id * a.DT;
+                }
             }
+        }
+        if (in) {
             const float d = ys - yd;
             diff[a.NT - 1 - k] = d;
             l2 += (double)d * (double)d;
             en += (double)yd * (double)yd;
         }
     }
-    a.l2_trace[itr] = l2;
-    a.en_trace[itr] = en;
+    for (int off = 32; off > 0; off >>= 1) {
+        l2 += __shfl_xor(l2, off, 64);
+        en += __shfl_xor(en, off, 64);
+    }
+    if (lane == 0) {
+        a.l2_trace[itr] = l2;
+        a.en_trace[itr] = en;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -696,7 +725,8 @@ void launch_time_filter(const FiltArgs &a, hipStream_t s)
 
 void launch_residual(const ResArgs &a, hipStream_t s)
 {
-    dim3 b(64), gr((a.ntr + 63) / 64);
+    if (a.ntr <= 0) return;
+    dim3 b(256), gr((a.ntr + 3) / 4);
     hipLaunchKernelGGL(k_residual, gr, b, 0, s, a);
 }
 

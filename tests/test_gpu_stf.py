@@ -5,8 +5,11 @@ of src/stf.c:24 around the engine.
 
   * the generated wavelet handed back in gives the same seismograms bit
     for bit (the given signal replaces the generated one and nothing else);
-  * twice the wavelet gives exactly twice the seismograms (the scheme is
-    linear, and doubling commutes with every rounding);
+  * twice the wavelet gives the same seismograms as the generated wavelet
+    of a source with twice the amplitude, bit for bit; they are twice the
+    first seismograms only to about 1e-5 of their peak, because values
+    ahead of the wavefront pass through the subnormal range, where rounding
+    is absolute (the oracle shows the same differences);
   * observed traces made with the wavelet convolved with a known short
     filter: the inverted correction filter, convolved with the wavelet,
     gives that convolved wavelet (times 1/dt, the library's scaling of the
@@ -57,8 +60,16 @@ def test_given_wavelets_and_stf_loop(gpu_lib, case):
         eng.set_wavelets(2 * sig)
         eng.forward()
         vx2, vy2 = eng.seismograms()
-        np.testing.assert_array_equal(vx2, 2 * vx0)
-        np.testing.assert_array_equal(vy2, 2 * vy0)
+        sp2 = sp.copy()
+        sp2[:, 5] *= 2  # amplitude column
+        eng.set_shot(sp2, rp)
+        eng.forward()
+        vxa, vya = eng.seismograms()
+        np.testing.assert_array_equal(vx2, vxa)
+        np.testing.assert_array_equal(vy2, vya)
+        assert np.abs(vx2 - 2 * vx0).max() <= 1e-5 * np.abs(vx0).max()
+        assert np.abs(vy2 - 2 * vy0).max() <= 1e-5 * np.abs(vy0).max()
+        eng.set_shot(sp, rp)
         print(f"[stf] {case['seed']} inversion loop", flush=True)
         h = np.zeros(cfg.NT)
         h[0], h[4], h[9] = 0.8, 0.3, -0.2
