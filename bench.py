@@ -697,6 +697,7 @@ def main():
         eng.set_velocity_lds(True)
     vel_rows = eng.get_velocity_rows()
     vel_lds = eng.get_velocity_lds()
+    graphs = eng.get_graphs()
 
     def step(k):
         batch = batches[k]
@@ -756,6 +757,7 @@ def main():
         "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "dh": cfg.DH, "FDORDER": cfg.FDORDER, "L": cfg.L,
                    "NT": cfg.NT, "DTINV": cfg.DTINV, "snapshots": args.snap, "shots_per_step": S, "receivers": int(len(w.shots[0].recpos)),
                    "velocity_rows_per_thread": vel_rows, "velocity_lds": vel_lds,
+                   "time_loop": "captured HIP graphs" if graphs else "eager launches",
                    "parallelism": "one GPU, one shot at a time" if S == 1 else
                    f"one GPU, {S} shots per launch" + (f" ({shape[0]} per workgroup, {'slot' if shape[3] else 'tile'}-major)"
                                                        if shape else "")},

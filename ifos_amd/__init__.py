@@ -219,6 +219,15 @@ class Engine:
         _check(lib().ifos_engine_shot_misfit(self.h, _fp(out, C.c_double)))
         return out
 
+    def set_graphs(self, mode):
+        """Time loop: 1 captured graphs, 0 eager launches, -1 by grid size."""
+        _check(lib().ifos_engine_set_graphs(self.h, int(mode)))
+
+    def get_graphs(self):
+        on = C.c_int()
+        _check(lib().ifos_engine_get_graphs(self.h, C.byref(on)))
+        return bool(on.value)
+
     def set_launch(self, spb=1, grid=0):
         _check(lib().ifos_engine_set_launch(self.h, spb, grid))
 

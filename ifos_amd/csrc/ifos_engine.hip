@@ -198,7 +198,7 @@ struct ifos_engine {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};  // forward(with snapshots), adjoint
     int gsteps[2] = {0, 0};
     int gkey[2][6] = {{-1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1}};
-    bool use_graphs = true;
+    int graph_mode = -1;            // time loop: 1 captured graphs, 0 eager launches, -1 by size (use_graphs)
     int swizzle = 1;
 };
 
@@ -241,7 +241,8 @@ static int engine_create(const ifos_config *cfg, int device, int slots, ifos_eng
     E->device = device;
     E->slots = slots;
     const char *ng = getenv("IFOS_NO_GRAPH");
-    E->use_graphs = !(ng && ng[0] == '1');
+    if (ng && ng[0] == '1') E->graph_mode = 0;
+    if (ng && ng[0] == '0') E->graph_mode = 1;
     const char *ns = getenv("IFOS_NO_SWIZZLE");
     E->swizzle = (ns && ns[0] == '1') ? 0 : 1;
     auto fail = [&](int rc) {
@@ -795,6 +796,19 @@ static int upload_args(ifos_engine *E, const StepArgs &a, int idx, const StepArg
     return 0;
 }
 
+// Captured graphs pay where launching is the bottleneck: below about 2^19
+// cell-slots a step's two kernels take less time than the host needs to
+// issue them.  Above it eager launches keep up, and they ran 2 % faster on
+// the 2000x800 grid (430 against 440 ms per shot gradient, each kernel a
+// little shorter than inside a graph).
+constexpr long kGraphCells = 1L << 19;
+
+static bool use_graphs(const ifos_engine *E)
+{
+    if (E->graph_mode >= 0) return E->graph_mode == 1;
+    return (long)E->c.NX * E->c.NY * E->nslots < kGraphCells;
+}
+
 // NT steps as graph units of `unit` steps: kernels take their position k in
 // the unit and read the number of steps already done from a device counter,
 // which a one-thread kernel advances at the end of each unit.
@@ -815,7 +829,7 @@ static int run_loop(ifos_engine *E, int sw, bool snapshots)
                         ((ls.order * 8 + ls.spb) * 4096 + ls.grid_v) * 4096 + ls.grid, unit};
     StepArgs a = shaped_args(step_args(E, sw, snapshots), ls);
     const StepArgs *da = nullptr;
-    if (E->use_graphs) {
+    if (use_graphs(E)) {
         ENG_TRY(upload_args(E, a, sw == 1 ? 2 : (snapshots ? 1 : 0), &da));
         if (!E->gexec[kind] || E->gsteps[kind] != unit || memcmp(E->gkey[kind], key, sizeof key)) {
             if (E->gexec[kind]) hipGraphExecDestroy(E->gexec[kind]);
@@ -1506,6 +1520,19 @@ extern "C" int ifos_engine_get_launch_grids(ifos_engine *E, int *grid_vel, int *
 {
     *grid_vel = E->shape.grid_v;
     *grid_str = E->shape.grid;
+    return 0;
+}
+
+extern "C" int ifos_engine_set_graphs(ifos_engine *E, int mode)
+{
+    if (mode < -1 || mode > 1) return ifos_set_error("graph mode must be -1, 0 or 1"), -2;
+    E->graph_mode = mode;
+    return 0;
+}
+
+extern "C" int ifos_engine_get_graphs(ifos_engine *E, int *on)
+{
+    *on = use_graphs(E) ? 1 : 0;
     return 0;
 }
 

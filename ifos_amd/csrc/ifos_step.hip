@@ -53,6 +53,11 @@ static int step_flags(const StepArgs &a, int k)
     return f;
 }
 
+// a step whose kernels need no more than the update, the sources and the
+// receivers: most steps of a run (every step but each DTINV-th of the
+// forward run with snapshots and of the adjoint run with VELOCITY 1)
+static bool lean_step(int flags) { return (flags & ~kForward) == 0; }
+
 // Workgroups are dealt round-robin to the 8 XCDs (workgroup h runs on XCD
 // h % 8), and each XCD has its own L2.  With swizzle on, XCD x works through
 // one contiguous run [xlo[x], xlo[x+1]) of the work items in row-major tile
@@ -473,9 +478,12 @@ __device__ __forceinline__ void vel_cell_rows(const StepArgs &a, int s, int blk,
 // RY = 1: a 64x8 block, one cell per thread.  RY = 2 or 4: a 64x(8/RY)
 // block, each thread owns rows RY*ty .. RY*ty+RY-1 of the 64x8 tile
 // (vel_cell_rows).
-template <int N, int SPB, int RY>
-__global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ pa, int k, int persistent, int flags)
+template <int N, int SPB, int RY, bool LEAN>
+__global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ pa, int k, int persistent, int flags_)
 {
+    // LEAN: a step with no snapshot, correlation, adjoint integration or
+    // energy sum (lean_step), compiled without that code and its registers
+    const int flags = LEAN ? (flags_ & kForward) : flags_;
     const StepArgs &a = *pa;
     __shared__ float s_red[2][kTileY];
     const StepGeom &g = a.g;
@@ -764,9 +772,10 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
     }
 }
 
-template <int N, int L, int SPB>
-__global__ __launch_bounds__(512) void k_str_step(const StepArgs *__restrict__ pa, int k, int persistent, int flags)
+template <int N, int L, int SPB, bool LEAN>
+__global__ __launch_bounds__(512) void k_str_step(const StepArgs *__restrict__ pa, int k, int persistent, int flags_)
 {
+    const int flags = LEAN ? (flags_ & kForward) : flags_;  // as in k_vel_step
     const StepArgs &a = *pa;
     __shared__ float s_syy[kTileY][kTileX];
     __shared__ float s_sxy[kTileY][kTileX];
@@ -1233,22 +1242,33 @@ static int grid_of(const StepArgs &a, const LaunchShape &ls, int grid)
     return 8 * widest;
 }
 
+template <int N, int SPB, bool LEAN>
+static void vel_launch_rows(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, int flags, int gr,
+                            hipStream_t s)
+{
+    const int pers = ls.grid_v > 0 ? 1 : 0;
+    if (ls.rows_v == 4)
+        hipLaunchKernelGGL((k_vel_step<N, SPB, 4, LEAN>), dim3(gr), dim3(kTileX, kTileY / 4), 0, s, da, k, pers,
+                           flags);
+    else if (ls.rows_v == 2)
+        hipLaunchKernelGGL((k_vel_step<N, SPB, 2, LEAN>), dim3(gr), dim3(kTileX, kTileY / 2), 0, s, da, k, pers,
+                           flags);
+    else
+        hipLaunchKernelGGL((k_vel_step<N, SPB, 1, LEAN>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, pers, flags);
+}
+
 template <int N, int SPB>
 static void vel_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
 {
     const int gr = grid_of(a, ls, ls.grid_v);
+    const int flags = step_flags(a, k);
     if (ls.lds_v)
         hipLaunchKernelGGL((k_vel_step_lds<N, SPB>), dim3(gr), dim3(kTileX, kTileY / 2), 0, s, da, k,
-                           ls.grid_v > 0 ? 1 : 0, step_flags(a, k));
-    else if (ls.rows_v == 4)
-        hipLaunchKernelGGL((k_vel_step<N, SPB, 4>), dim3(gr), dim3(kTileX, kTileY / 4), 0, s, da, k,
-                           ls.grid_v > 0 ? 1 : 0, step_flags(a, k));
-    else if (ls.rows_v == 2)
-        hipLaunchKernelGGL((k_vel_step<N, SPB, 2>), dim3(gr), dim3(kTileX, kTileY / 2), 0, s, da, k,
-                           ls.grid_v > 0 ? 1 : 0, step_flags(a, k));
+                           ls.grid_v > 0 ? 1 : 0, flags);
+    else if (lean_step(flags))
+        vel_launch_rows<N, SPB, true>(a, da, ls, k, flags, gr, s);
     else
-        hipLaunchKernelGGL((k_vel_step<N, SPB, 1>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k,
-                           ls.grid_v > 0 ? 1 : 0, step_flags(a, k));
+        vel_launch_rows<N, SPB, false>(a, da, ls, k, flags, gr, s);
 }
 
 template <int N>
@@ -1279,8 +1299,13 @@ template <int N, int L, int SPB>
 static void str_launch(const StepArgs &a, const StepArgs *da, const LaunchShape &ls, int k, hipStream_t s)
 {
     const int gr = grid_of(a, ls, ls.grid);
-    hipLaunchKernelGGL((k_str_step<N, L, SPB>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k, ls.grid > 0 ? 1 : 0,
-                       step_flags(a, k));
+    const int flags = step_flags(a, k);
+    if (lean_step(flags))
+        hipLaunchKernelGGL((k_str_step<N, L, SPB, true>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k,
+                           ls.grid > 0 ? 1 : 0, flags);
+    else
+        hipLaunchKernelGGL((k_str_step<N, L, SPB, false>), dim3(gr), dim3(kTileX, kTileY), 0, s, da, k,
+                           ls.grid > 0 ? 1 : 0, flags);
 }
 
 template <int N, int L>

@@ -120,6 +120,14 @@ int  ifos_engine_shot_misfit(ifos_engine *e, double *per_shot);
  * cell's materials are loaded once for all of them), grid = 0 for one
  * workgroup per (tile, shot group) or a multiple of 8 for a persistent grid
  * of that many workgroups. */
+/* Time loop: mode 1 replays captured HIP graphs of a block of steps, 0
+ * launches every kernel from the host, -1 (the default) chooses by size:
+ * graphs below 2^19 cells x slots, where launching would be the bottleneck.
+ * Results are the same bits either way.  The environment variable
+ * IFOS_NO_GRAPH=1 (or 0) sets 0 (or 1) at creation.  get_graphs reports
+ * what the next run will use. */
+int  ifos_engine_set_graphs(ifos_engine *e, int mode);
+int  ifos_engine_get_graphs(ifos_engine *e, int *on);
 int  ifos_engine_set_launch(ifos_engine *e, int spb, int grid);
 int  ifos_engine_get_launch(ifos_engine *e, int *spb, int *grid);
 /* The persistent grids of the velocity and the stress step (set_launch sets

@@ -102,6 +102,36 @@ def test_graph_replay_equals_eager(gpu_lib, monkeypatch):
     assert_bitexact(out[0][1], out[1][1], "graph vs eager gradient")
 
 
+def test_graph_mode_by_size(gpu_lib, monkeypatch):
+    """Captured graphs below 2^19 cells x slots, eager launches above; the
+    setter overrides the choice, and the environment sets it at creation."""
+    monkeypatch.delenv("IFOS_NO_GRAPH", raising=False)
+    small = ifos_amd.make_config(NX=400, NY=200, NT=20, FDORDER=4, DH=10.0, DT=1e-3, FW=10)
+    large = ifos_amd.make_config(NX=1024, NY=600, NT=20, FDORDER=4, DH=10.0, DT=1e-3, FW=10)
+    sp = np.array([[50, 20, 0, 0, 10, 1e10, 0, 1]], np.float32)
+    rp = np.array([[60, 30]], np.int32)
+    want = {}
+    for name, cfg in (("small", small), ("large", large)):
+        eng = ifos_amd.Engine(cfg)
+        m = np.full((cfg.NY, cfg.NX), 3000.0, np.float32)
+        eng.set_model(m, m / 1.7, m * 0.7)
+        eng.set_shot(sp, rp)
+        want[name] = eng.get_graphs()
+        eng.set_graphs(1)
+        assert eng.get_graphs()
+        eng.set_graphs(0)
+        assert not eng.get_graphs()
+        eng.close()
+    assert want == {"small": True, "large": False}
+    monkeypatch.setenv("IFOS_NO_GRAPH", "0")
+    eng = ifos_amd.Engine(large)
+    m = np.full((large.NY, large.NX), 3000.0, np.float32)
+    eng.set_model(m, m / 1.7, m * 0.7)
+    eng.set_shot(sp, rp)
+    assert eng.get_graphs()
+    eng.close()
+
+
 @pytest.mark.parametrize("FDORDER,L,VEL,FS,stype", [(10, 0, 0, 1, 1), (12, 2, 1, 1, 1), (8, 3, 1, 0, 3),
                                                     (4, 2, 0, 1, 2)])
 def test_forward_matches_oracle(gpu_lib, FDORDER, L, VEL, FS, stype):
