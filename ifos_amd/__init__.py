@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libifos2d_hip.so")
+# IFOS_LIB names another build of the library (for comparing builds)
+LIB_PATH = os.environ.get("IFOS_LIB") or os.path.join(HERE, "lib", "libifos2d_hip.so")
 
 _INT_FIELDS = ["NX", "NY", "FDORDER", "MAXRELERROR", "FW", "FREE_SURF", "BOUNDARY", "L",
                "VELOCITY", "ADJOINT_TYPE", "PARAMETERIZATION", "DTINV", "NT",
