@@ -46,9 +46,10 @@ def test_header_included_twice_and_together(tmp_path):
     _ok(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"), str(src)])
 
 
-def test_c_client_compiles():
-    _ok(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"),
-         "-I" + os.path.join(ROOT, "ifos_amd", "csrc"), os.path.join(ROOT, "tests", "c", "dropin_forward.c")])
+@pytest.mark.parametrize("client", ["dropin_forward.c", "engine_gradient.c"])
+def test_c_client_compiles(client):
+    _ok(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"),
+         "-I" + os.path.join(ROOT, "ifos_amd", "csrc"), os.path.join(ROOT, "tests", "c", client)])
 
 
 def test_clean_build_rebuilds_every_object(tmp_path):
