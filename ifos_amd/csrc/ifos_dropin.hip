@@ -126,6 +126,75 @@ float *strip_of(float **m, int *pitch, const char *name)
     return o + p + 1;
 }
 
+// A time loop's source series on the device.  update_v_PML and psource read
+// one to three samples per source and step (signals[l][nt], or nt-1..nt+1 for
+// the explosive amplitude); staging them cost a copy per step.  The whole
+// block signals[1..nsrc][1..NT] is uploaded when first seen and again
+// whenever a sample a step reads differs, bit for bit, from the copy last
+// uploaded (compared on the host every step), so each step reads the host's
+// values of that moment, as the reference does.
+struct SeriesCache {
+    float **key = nullptr;
+    int nsrc = 0, ns = 0;
+    std::vector<float> mirror;  // sample t of source l at l * ns + t - 1
+    float *dev = nullptr;
+    size_t cap = 0;
+};
+
+// device block [nsrc][NT] (source-major) of host signals, checked at samples t0..t1
+const float *series_dev(SeriesCache &c, float **signals, int nsrc, int t0, int t1)
+{
+    const int T = NT;
+    t0 = std::max(t0, 1);
+    t1 = std::min(t1, T);
+    bool fresh = c.key != signals || c.nsrc != nsrc || c.ns != T;
+    for (int l = 1; l <= nsrc && !fresh; l++)
+        fresh = memcmp(&signals[l][t0], &c.mirror[(size_t)(l - 1) * T + t0 - 1], sizeof(float) * (t1 - t0 + 1)) != 0;
+    if (fresh) {
+        // kernels queued earlier may still read the old block
+        hip_check(hipStreamSynchronize(stream()), "series sync");
+        const size_t n = (size_t)nsrc * T;
+        c.mirror.resize(n);
+        for (int l = 1; l <= nsrc; l++) memcpy(&c.mirror[(size_t)(l - 1) * T], &signals[l][1], sizeof(float) * T);
+        if (n > c.cap) {
+            if (c.dev) hip_check(hipFree(c.dev), "hipFree");
+            hip_check(hipMalloc(&c.dev, sizeof(float) * n), "hipMalloc(series)");
+            c.cap = n;
+        }
+        hip_check(hipMemcpyAsync(c.dev, c.mirror.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream()),
+                  "series upload");
+        c.key = signals;
+        c.nsrc = nsrc;
+        c.ns = T;
+    }
+    return c.dev;
+}
+
+// small per-loop tables (positions, types, azimuths) on the device,
+// uploaded again only when their bytes change
+struct BlobCache {
+    std::vector<char> host;
+    char *dev = nullptr;
+    size_t cap = 0;
+};
+
+const char *blob_dev(BlobCache &c, const void *data, size_t n)
+{
+    if (c.host.size() == n && !memcmp(c.host.data(), data, n)) return c.dev;
+    hip_check(hipStreamSynchronize(stream()), "table sync");
+    c.host.assign((const char *)data, (const char *)data + n);
+    if (n > c.cap) {
+        if (c.dev) hip_check(hipFree(c.dev), "hipFree");
+        hip_check(hipMalloc(&c.dev, n), "hipMalloc(table)");
+        c.cap = n;
+    }
+    hip_check(hipMemcpyAsync(c.dev, c.host.data(), n, hipMemcpyHostToDevice, stream()), "table upload");
+    return c.dev;
+}
+
+SeriesCache g_vsig[2], g_psig;
+BlobCache g_vsrc, g_psrc;
+
 // device copy of a 1-based host or device vector of n entries, element 1 first
 struct VecCache {
     std::vector<float> host;
@@ -276,6 +345,24 @@ __global__ void k_point_add(int n, const int *ix, const int *iy, const double *a
         const long c = (long)iy[l] * pitch + ix[l];
         f1[c] = (float)((double)f1[c] + 0.5 * amp[l]);
         if (f2) f2[c] = (float)((double)f2[c] + 0.5 * amp[l]);
+    }
+}
+
+// psource (src/psource.c:26-56) for every source in order, from the device
+// copy of the series (sig: sample t of source l at l * ns + t - 1): the
+// amplitude is the reference's, formed in double from the float samples
+// (explosive_amp for the forward run, the sample itself for the adjoint),
+// then added half to sxx and half to syy
+__global__ void k_psource_series(int n, const int *ix, const int *iy, const float *sig, int ns, int nt, float DH,
+                                 int sw, int pitch, float *f1, float *f2)
+{
+    if (threadIdx.x != 0) return;
+    for (int l = 0; l < n; l++) {
+        const float *s = sig + (long)l * ns - 1;  // 1-based
+        const float a = sw == 0 ? explosive_amp(s, nt, ns, DH) : s[nt];
+        const long c = (long)iy[l] * pitch + ix[l];
+        f1[c] = (float)((double)f1[c] + 0.5 * (double)a);
+        if (f2) f2[c] = (float)((double)f2[c] + 0.5 * (double)a);
     }
 }
 
@@ -454,10 +541,10 @@ extern "C" void update_v_PML(int nx1, int nx2, int ny1, int ny2, int nt, float *
 
     if (nsrc <= 0) return;
     // point sources after the loop (src/update_v_PML.c:553-578): positions,
-    // types and this step's samples are staged from the host arrays
+    // types and azimuths from the host arrays, this step's samples from the
+    // device copies of the source series
     std::vector<int> pos(3 * nsrc);
     std::vector<double> sc(2 * nsrc);
-    std::vector<float> s(2 * nsrc);
     int any = 0;
     for (int l = 1; l <= nsrc; l++) {
         pos[l - 1] = (int)srcpos_loc[1][l];
@@ -472,22 +559,17 @@ extern "C" void update_v_PML(int nx1, int nx2, int ny1, int ny2, int nt, float *
         } else {
             any = 1;
         }
-        const DevArr *d0 = lookup(signals), *d1 = lookup(signals1);
-        if (d0 || d1) fatal("update_v_PML: signals are host arrays in this interface");
-        s[l - 1] = signals[l][nt];
-        s[nsrc + l - 1] = signals1[l][nt];
     }
     if (!any) return;
-    const size_t o1 = sizeof(int) * pos.size(), o2 = o1 + sizeof(double) * sc.size();
-    const size_t nb = o2 + sizeof(float) * s.size();
-    char *st = (char *)g_stage[0].get(nb);
-    int slot;
-    char *blob = g_ring.get(nb, slot);
-    memcpy(blob, pos.data(), o1);
-    memcpy(blob + o1, sc.data(), sizeof(double) * sc.size());
-    memcpy(blob + o2, s.data(), sizeof(float) * s.size());
-    hip_check(hipMemcpyAsync(st, blob, nb, hipMemcpyHostToDevice, stream()), "stage sources");
-    g_ring.queued(slot);
+    if (lookup(signals) || lookup(signals1)) fatal("update_v_PML: signals are host arrays in this interface");
+    if (nt < 1 || nt > NT) fatal("update_v_PML: time step outside 1..NT");
+    const float *sig0 = series_dev(g_vsig[0], signals, nsrc, nt, nt);
+    const float *sig1 = signals1 == signals ? sig0 : series_dev(g_vsig[1], signals1, nsrc, nt, nt);
+    const size_t o1 = sizeof(int) * pos.size(), nb = o1 + sizeof(double) * sc.size();
+    std::vector<char> tab(nb);
+    memcpy(tab.data(), pos.data(), o1);
+    memcpy(tab.data() + o1, sc.data(), sizeof(double) * sc.size());
+    const char *st = blob_dev(g_vsrc, tab.data(), nb);
     PointArgs pa;
     pa.n = nsrc;
     pa.ix = (const int *)st;
@@ -495,9 +577,9 @@ extern "C" void update_v_PML(int nx1, int nx2, int ny1, int ny2, int nt, float *
     pa.type = (const int *)st + 2 * nsrc;
     pa.sin_azi = (const double *)(st + o1);
     pa.cos_azi = (const double *)(st + o1) + nsrc;
-    pa.sig = (const float *)(st + o2);
-    pa.sig1 = (const float *)(st + o2) + nsrc;
-    pa.ld = 1;  // one sample per source: column nt-1 with nt = 1
+    pa.sig = sig0;
+    pa.sig1 = sig1;
+    pa.ld = NT;  // sample nt of source l at l * NT + nt - 1
     // in source order, as the reference loop: serially where positions may
     // repeat; adjoint sources at distinct cells are added in parallel (each
     // cell then receives one addition, so the order does not matter)
@@ -507,9 +589,9 @@ extern "C" void update_v_PML(int nx1, int nx2, int ny1, int ny2, int nt, float *
         for (int l = 0; l < nsrc && distinct; l++) distinct = seen.insert((long)pos[nsrc + l] * P + pos[l]).second;
     }
     if (distinct)
-        launch_inject(pa, a.g, a.vx, a.vy, a.rip, a.rjp, sw, ADJOINT_TYPE, nullptr, 1, nullptr, stream());
+        launch_inject(pa, a.g, a.vx, a.vy, a.rip, a.rjp, sw, ADJOINT_TYPE, nullptr, nt, nullptr, stream());
     else
-        launch_inject_serial(pa, a.g, a.vx, a.vy, a.rip, a.rjp, sw, ADJOINT_TYPE, nullptr, 1, nullptr, stream());
+        launch_inject_serial(pa, a.g, a.vx, a.vy, a.rip, a.rjp, sw, ADJOINT_TYPE, nullptr, nt, nullptr, stream());
 }
 
 // ---------------------------------------------------------------------------
@@ -642,33 +724,20 @@ extern "C" void psource(int nt, float **sxx, float **syy, float **sp, float **sr
 {
     (void)sp;
     if (nsrc <= 0) return;
+    if (nt < 1 || nt > NT) fatal("psource: time step outside 1..NT");
+    if (lookup(signals)) fatal("psource: signals are host arrays in this interface");
     int P, p2;
     float *oxx = origin_of(sxx, &P, "sxx"), *oyy = origin_of(syy, &p2, "syy");
     std::vector<int> pos(2 * nsrc);
-    std::vector<double> amp(nsrc);
     for (int l = 1; l <= nsrc; l++) {
         pos[l - 1] = (int)srcpos_loc[1][l];
         pos[nsrc + l - 1] = (int)srcpos_loc[2][l];
-        float a = 0.0f;
-        if (sw == 0) {
-            if (nt == 1) a = signals[l][nt + 1] / (2.0 * DH * DH);
-            if ((nt > 1) && (nt < NT)) a = (signals[l][nt + 1] - signals[l][nt - 1]) / (2.0 * DH * DH);
-            if (nt == NT) a = -signals[l][nt - 1] / (2.0 * DH * DH);
-        } else {
-            a = signals[l][nt];
-        }
-        amp[l - 1] = a;
     }
-    const size_t nb = sizeof(int) * pos.size() + sizeof(double) * amp.size();
-    char *st = (char *)g_stage[1].get(nb);
-    int slot;
-    char *blob = g_ring.get(nb, slot);
-    memcpy(blob, pos.data(), sizeof(int) * pos.size());
-    memcpy(blob + sizeof(int) * pos.size(), amp.data(), sizeof(double) * amp.size());
-    hip_check(hipMemcpyAsync(st, blob, nb, hipMemcpyHostToDevice, stream()), "stage psource");
-    g_ring.queued(slot);
-    hipLaunchKernelGGL(k_point_add, dim3(1), dim3(64), 0, stream(), nsrc, (const int *)st, (const int *)st + nsrc,
-                       (const double *)(st + sizeof(int) * pos.size()), P, oxx, oyy);
+    // the amplitude reads samples nt-1..nt+1 (sw 0) or nt (sw 1)
+    const float *sig = series_dev(g_psig, signals, nsrc, sw == 0 ? nt - 1 : nt, sw == 0 ? nt + 1 : nt);
+    const int *dpos = (const int *)blob_dev(g_psrc, pos.data(), sizeof(int) * pos.size());
+    hipLaunchKernelGGL(k_psource_series, dim3(1), dim3(64), 0, stream(), nsrc, dpos, dpos + nsrc, sig, NT, nt, DH, sw,
+                       P, oxx, oyy);
 }
 
 // ---------------------------------------------------------------------------

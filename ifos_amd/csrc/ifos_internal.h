@@ -68,6 +68,10 @@ struct StepGeom {
     Cpml pml;
     int px, py;       // CPML strip pitches
     int K_one;        // every CPML K value is 1 (the divisions by K are identities)
+    // tile order of the reference-signature update kernels (tile_of_block):
+    // tnbx tiles of 64x4 cells per tile row; XCD x runs tiles [xlo[x], xlo[x+1])
+    int tnbx;
+    int xlo[9];
 };
 
 // update_v_PML (src/update_v_PML.c:29-596)

@@ -11,6 +11,10 @@
 // device int (nt_dev) so that a captured graph of a block of time steps can
 // be replayed.  The single-block inject kernel advances the counter; kernels
 // that run before it within a step (update_v) see the previous value.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include "ifos_cellmath.h"
 #include "ifos_internal.h"
 
@@ -20,6 +24,22 @@ namespace ifos {
 
 __device__ __forceinline__ int step_of(const int *nt_dev, int nt) { return nt_dev ? *nt_dev : nt; }
 
+// Tile of this workgroup for the update kernels: workgroups are dealt
+// round-robin to the 8 XCDs (workgroup h runs on XCD h % 8) and each XCD has
+// its own L2, so XCD x works through one contiguous run [xlo[x], xlo[x+1])
+// of the row-major 64x4 tiles and the halo rows neighbouring tiles share are
+// fetched into one L2.  False for the workgroups past the end of a run.
+__device__ __forceinline__ bool tile_of_block(const StepGeom &g, int &i, int &j)
+{
+    const int h = blockIdx.x, x = h & 7;
+    const int t = g.xlo[x] + (h >> 3);
+    if (t >= g.xlo[x + 1]) return false;
+    const int by = t / g.tnbx, bx = t - by * g.tnbx;
+    i = bx * blockDim.x + threadIdx.x + 1;
+    j = by * blockDim.y + threadIdx.y + 1;
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // update_v_PML  (src/update_v_PML.c:29-596; orders 2-8 of the reference and
 // the same form for orders 10/12)
@@ -28,9 +48,8 @@ template <int N>
 IFOS_KERNEL void k_update_v(VArgs a, const int *nt_dev, int DTINV, int NT)
 {
     const StepGeom &g = a.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (i > g.NX || j > g.NY) return;
+    int i, j;
+    if (!tile_of_block(g, i, j) || i > g.NX || j > g.NY) return;
     const int P = g.pitch;
     const long c = (long)j * P + i;
     float sxx_x, sxy_x, sxy_y, syy_y;
@@ -120,9 +139,8 @@ template <int N>
 IFOS_KERNEL void k_update_s_elastic(SElArgs a, const int *nt_dev, int DTINV)
 {
     const StepGeom &g = a.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (i > g.NX || j > g.NY) return;
+    int i, j;
+    if (!tile_of_block(g, i, j) || i > g.NX || j > g.NY) return;
     const long c = (long)j * g.pitch + i;
     const float dhi = g.DT / g.DH;
     float vxx, vyx, vxy, vyy;
@@ -159,9 +177,8 @@ template <int N, int L>
 IFOS_KERNEL void k_update_s_visc(SViscArgs a, const int *nt_dev, int DTINV)
 {
     const StepGeom &g = a.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (i > g.NX || j > g.NY) return;
+    int i, j;
+    if (!tile_of_block(g, i, j) || i > g.NX || j > g.NY) return;
     const long c = (long)j * g.pitch + i;
     const float dhi = 1.0 / (double)g.DH;
     const float dthalbe = g.DT / 2.0;
@@ -569,6 +586,52 @@ IFOS_KERNEL void k_residual(ResArgs a)
 // ---------------------------------------------------------------------------
 static inline dim3 grid2d(int NX, int NY, dim3 b) { return dim3((NX + b.x - 1) / b.x, (NY + b.y - 1) / b.y); }
 
+// XCD runs of the 64x4 tiles of a grid (tile_of_block) with equal cost, a
+// tile that reaches into an absorbing strip counting double; returns the
+// number of workgroups to launch.  The last few geometries are kept, since
+// the update kernels run with the same one every step.
+static int tile_runs(StepGeom &g)
+{
+    constexpr int TW = 64, TH = 4;
+    struct Entry {
+        int NX, NY, FW, FREE_SURF, BOUNDARY, grid;
+        int xlo[9];
+    };
+    static thread_local Entry cache[4];
+    static thread_local int next = 0;
+    g.tnbx = (g.NX + TW - 1) / TW;
+    for (const Entry &e : cache)
+        if (e.grid && e.NX == g.NX && e.NY == g.NY && e.FW == g.FW && e.FREE_SURF == g.FREE_SURF &&
+            e.BOUNDARY == g.BOUNDARY) {
+            memcpy(g.xlo, e.xlo, sizeof g.xlo);
+            return e.grid;
+        }
+    const int nby = (g.NY + TH - 1) / TH, nt = g.tnbx * nby;
+    std::vector<long> cum(nt + 1, 0);
+    for (int t = 0; t < nt; t++) {
+        const int by = t / g.tnbx, bx = t - by * g.tnbx;
+        const int i0 = bx * TW + 1, i1 = std::min(i0 + TW - 1, g.NX);
+        const int j0 = by * TH + 1, j1 = std::min(j0 + TH - 1, g.NY);
+        const bool strip = (!g.BOUNDARY && (i0 <= g.FW || i1 >= g.NX - g.FW + 1)) ||
+                           (!g.FREE_SURF && j0 <= g.FW) || j1 >= g.NY - g.FW + 1;
+        cum[t + 1] = cum[t] + (strip ? 2 : 1);
+    }
+    int widest = 0;
+    g.xlo[0] = 0;
+    for (int x = 1; x <= 8; x++) {
+        const long want = cum[nt] * x / 8;
+        int t = g.xlo[x - 1];
+        while (t < nt && cum[t] < want) t++;
+        g.xlo[x] = x == 8 ? nt : t;
+        widest = std::max(widest, g.xlo[x] - g.xlo[x - 1]);
+    }
+    Entry &e = cache[next];
+    next = (next + 1) % 4;
+    e = Entry{g.NX, g.NY, g.FW, g.FREE_SURF, g.BOUNDARY, 8 * widest, {}};
+    memcpy(e.xlo, g.xlo, sizeof g.xlo);
+    return e.grid;
+}
+
 #define IFOS_DISPATCH_N(N, CALL) \
     switch (N) {                 \
     case 1: CALL(1); break;      \
@@ -593,9 +656,10 @@ void set_step_control(const int *nt_dev, int DTINV, int NT)
     g_ctl.NT = NT;
 }
 
-void launch_update_v(const VArgs &a, hipStream_t s)
+void launch_update_v(const VArgs &a0, hipStream_t s)
 {
-    dim3 b(64, 4), gr = grid2d(a.g.NX, a.g.NY, b);
+    VArgs a = a0;
+    dim3 b(64, 4), gr(tile_runs(a.g));
 #define L_(n) hipLaunchKernelGGL(k_update_v<n>, gr, b, 0, s, a, g_ctl.nt_dev, g_ctl.DTINV, g_ctl.NT)
     IFOS_DISPATCH_N(a.g.N, L_)
 #undef L_
@@ -617,18 +681,20 @@ void launch_inject_serial(const PointArgs &a, const StepGeom &g, float *vx, floa
                        nt_tick, 1);
 }
 
-void launch_update_s_elastic(const SElArgs &a, hipStream_t s)
+void launch_update_s_elastic(const SElArgs &a0, hipStream_t s)
 {
-    dim3 b(64, 4), gr = grid2d(a.g.NX, a.g.NY, b);
+    SElArgs a = a0;
+    dim3 b(64, 4), gr(tile_runs(a.g));
 #define L_(n) hipLaunchKernelGGL(k_update_s_elastic<n>, gr, b, 0, s, a, g_ctl.nt_dev, g_ctl.DTINV)
     IFOS_DISPATCH_N(a.g.N, L_)
 #undef L_
 }
 
 template <int N>
-static void launch_visc_n(const SViscArgs &a, hipStream_t s)
+static void launch_visc_n(const SViscArgs &a0, hipStream_t s)
 {
-    dim3 b(64, 4), gr = grid2d(a.g.NX, a.g.NY, b);
+    SViscArgs a = a0;
+    dim3 b(64, 4), gr(tile_runs(a.g));
     switch (a.L) {
     case 1: hipLaunchKernelGGL((k_update_s_visc<N, 1>), gr, b, 0, s, a, g_ctl.nt_dev, g_ctl.DTINV); break;
     case 2: hipLaunchKernelGGL((k_update_s_visc<N, 2>), gr, b, 0, s, a, g_ctl.nt_dev, g_ctl.DTINV); break;
