@@ -255,33 +255,51 @@ __device__ __forceinline__ void surface_column(const PostArgs &a, int i, float &
     const StepGeom &g = a.g;
     const int P = g.pitch;
     const long c1 = (long)1 * P + i;
+    // Every value read here is loaded before the first store: none of them
+    // lies in the rows written (the mirrored rows 1-N..0, row 1 of syy and
+    // uy), and no other thread of the launch writes them, so the loads need
+    // not wait on the stores one round trip at a time.
+    float syy_s[N], sxy_s[N], uy_s[N], uxy_s[N], vxp[N], vxm[N], vyp[N], vym[N];
+#pragma unroll
+    for (int m = 1; m <= N; m++) {
+        syy_s[m - 1] = a.syy[c1 + (long)m * P];
+        sxy_s[m - 1] = a.sxy[c1 + (long)(m - 1) * P];
+        uy_s[m - 1] = a.uy ? a.uy[c1 + (long)m * P] : 0.0f;
+        uxy_s[m - 1] = a.uxy ? a.uxy[c1 + (long)(m - 1) * P] : 0.0f;
+        vxp[m - 1] = a.vx[c1 + m - 1];
+        vxm[m - 1] = a.vx[c1 - m];
+        vyp[m - 1] = a.vy[c1 + (long)(m - 1) * P];
+        vym[m - 1] = a.vy[c1 - (long)m * P];
+    }
+    const Cpml &pm = g.pml;
+    const bool left = !g.BOUNDARY && i <= g.FW, right = !g.BOUNDARY && i >= g.NX - g.FW + 1;
+    const int hl = i - 1, hr = i - g.NX + 2 * g.FW - 1;
+    const float psl = left ? a.psi_vxxs[hl] : 0.0f, psr = right ? a.psi_vxxs[hr] : 0.0f;
     a.syy[c1] = 0.0f;
     if (a.uy) a.uy[c1] = 0.0f;
     float vxx = 0.0f, vyy = 0.0f;
 #pragma unroll
     for (int m = 1; m <= N; m++) {
-        a.syy[c1 - (long)m * P] = -a.syy[c1 + (long)m * P];
-        a.sxy[c1 - (long)m * P] = -a.sxy[c1 + (long)(m - 1) * P];
-        if (a.uy) a.uy[c1 - (long)m * P] = -a.uy[c1 + (long)m * P];
-        if (a.uxy) a.uxy[c1 - (long)m * P] = -a.uxy[c1 + (long)(m - 1) * P];
-        vxx += g.hc[m] * (a.vx[c1 + m - 1] - a.vx[c1 - m]);
-        vyy += g.hc[m] * (a.vy[c1 + (long)(m - 1) * P] - a.vy[c1 - (long)m * P]);
+        a.syy[c1 - (long)m * P] = -syy_s[m - 1];
+        a.sxy[c1 - (long)m * P] = -sxy_s[m - 1];
+        if (a.uy) a.uy[c1 - (long)m * P] = -uy_s[m - 1];
+        if (a.uxy) a.uxy[c1 - (long)m * P] = -uxy_s[m - 1];
+        vxx += g.hc[m] * (vxp[m - 1] - vxm[m - 1]);
+        vyy += g.hc[m] * (vyp[m - 1] - vym[m - 1]);
     }
     const float dh24 = 1.0 / (double)g.DH;
     vxx *= dh24;
     vyy *= dh24;
-    const Cpml &pm = g.pml;
-    if (!g.BOUNDARY && i <= g.FW) {
-        const int h = i - 1;
-        float *ps = a.psi_vxxs + h;
-        *ps = pm.b_x[h] * *ps + pm.a_x[h] * vxx;
-        vxx = vxx / pm.K_x[h] + *ps;
+    if (left) {
+        const float ps = pm.b_x[hl] * psl + pm.a_x[hl] * vxx;
+        a.psi_vxxs[hl] = ps;
+        vxx = vxx / pm.K_x[hl] + ps;
     }
-    if (!g.BOUNDARY && i >= g.NX - g.FW + 1) {
-        const int h = i - g.NX + 2 * g.FW - 1;
-        float *ps = a.psi_vxxs + h;
-        *ps = pm.b_x[h] * *ps + pm.a_x[h] * vxx;
-        vxx = vxx / pm.K_x[h] + *ps;
+    if (right) {
+        // the left strip's update above wrote another entry (hl != hr)
+        const float ps = pm.b_x[hr] * psr + pm.a_x[hr] * vxx;
+        a.psi_vxxs[hr] = ps;
+        vxx = vxx / pm.K_x[hr] + ps;
     }
     vxx_out = vxx;
     vyy_out = vyy;
@@ -311,56 +329,84 @@ IFOS_KERNEL void k_post(PostArgs a)
             }
         }
         if (g.FREE_SURF) {
-            float vxx, vyy;
-            surface_column<N>(a, i, vxx, vyy);
             const long c1 = (long)P + i;
             const float DT = g.DT;
+            // the column's own values are loaded with the surface stencil's,
+            // before surface_column's stores (which write none of them)
+            const float sxx1 = a.sxx[c1];
+            const float ux1 = a.ux ? a.ux[c1] : 0.0f;
+            const long mc = c1 * a.cs;
+            float m0, m1, m2 = 0.0f, m3 = 0.0f, m4 = 0.0f;
+            if (a.L == 0) {
+                if (a.f) {
+                    m0 = a.f[c1];
+                    m1 = a.gl[c1];
+                } else {
+                    m0 = a.rho[c1];
+                    m1 = a.vs[c1];
+                    m2 = a.vp[c1];
+                }
+            } else {
+                m0 = a.vs[c1];
+                m1 = a.vp[c1];
+                m2 = a.rho[c1];
+                m3 = a.taus[c1];
+                m4 = a.taup[c1];
+            }
+            float pv[kMaxL];
+#pragma unroll
+            for (int l = 0; l < kMaxL; l++) pv[l] = l < a.L ? a.p[mc + (long)l * a.ls] : 0.0f;
+            float vxx, vyy;
+            surface_column<N>(a, i, vxx, vyy);
             if (a.L == 0) {
                 float fjm, gl;
                 if (a.f) {
-                    fjm = (float)((double)a.f[c1] * 2.0);
-                    gl = a.gl[c1];
+                    fjm = (float)((double)m0 * 2.0);
+                    gl = m1;
                 } else {
-                    const float rho = a.rho[c1], u = a.vs[c1], pi = a.vp[c1];
+                    const float rho = m0, u = m1, pi = m2;
                     fjm = (float)((double)(rho * u * u) * 2.0);
                     gl = rho * ((pi * pi) - 2 * u * u);
                 }
-                a.sxx[c1] = a.sxx[c1] + (-DT * ((gl * gl) / (gl + fjm) * vxx + gl * vyy));
+                a.sxx[c1] = sxx1 + (-DT * ((gl * gl) / (gl + fjm) * vxx + gl * vyy));
                 if (a.ux)
                     a.ux[c1] = (float)(((2.0 * (double)gl * (double)fjm + (double)(fjm * fjm)) / (double)(gl + fjm)) *
                                        (double)vxx);
             } else {
                 const int L = a.L;
-                const long mc = c1 * a.cs;
                 for (int l = 0; l < L; l++) a.q[mc + (long)l * a.ls] = 0.0f;
-                const float u = a.vs[c1], pi = a.vp[c1], rho = a.rho[c1];
-                const float taus = a.taus[c1], taup = a.taup[c1];
+                const float u = m0, pi = m1, rho = m2;
+                const float taus = m3, taup = m4;
                 const float mu = (float)((double)(u * u * rho) / (1.0 + (double)(a.sumu * taus)));
                 const float pim = (float)((double)(pi * pi * rho) / (1.0 + (double)(a.sumpi * taup)));
                 float sump = 0.0f;
-                for (int l = 0; l < L; l++) sump += a.p[mc + (long)l * a.ls];
+#pragma unroll
+                for (int l = 0; l < kMaxL; l++)
+                    if (l < L) sump += pv[l];
                 const float fjm = (float)((double)mu * 2.0 * (1.0 + (double)(L * taus)));
                 const float gv = (float)((double)pim * (1.0 + (double)(L * taup)));
                 const float dthalbe = DT / 2.0;
-                a.sxx[c1] = a.sxx[c1] + ((-(DT * (gv - fjm) * (gv - fjm) * vxx / gv) - (DT * (gv - fjm) * vyy)) -
-                                         (dthalbe * sump));
-                if (a.ux)
-                    a.ux[c1] = (float)((double)a.ux[c1] +
-                                       ((double)(-((gv - fjm) * (gv - fjm) * vxx / gv) - ((gv - fjm) * vyy)) -
-                                        (0.5 * (double)sump)));
+                const float sxx2 = sxx1 + ((-(DT * (gv - fjm) * (gv - fjm) * vxx / gv) - (DT * (gv - fjm) * vyy)) -
+                                           (dthalbe * sump));
+                const float ux2 = (float)((double)ux1 +
+                                          ((double)(-((gv - fjm) * (gv - fjm) * vxx / gv) - ((gv - fjm) * vyy)) -
+                                           (0.5 * (double)sump)));
                 sump = 0.0f;
-                for (int l = 1; l <= L; l++) {
+#pragma unroll
+                for (int l = 1; l <= kMaxL; l++) {
+                    if (l > L) continue;
                     const float bjm = (float)((double)a.eta[l] / (1.0 + ((double)a.eta[l] * 0.5)));
                     const float djm = (float)(2.0 * (double)mu * (double)taus);
                     const float e = pim * taup;
-                    float *pp = a.p + mc + (long)(l - 1) * a.ls;
-                    *pp = (float)((double)*pp + (double)bjm * ((((double)(djm - e) * ((double)(fjm / gv) - 1.0)) *
-                                                                 (double)vxx) -
-                                                                (double)((djm - e) * vyy)));
-                    sump += *pp;
+                    const float pn = (float)((double)pv[l - 1] +
+                                             (double)bjm * ((((double)(djm - e) * ((double)(fjm / gv) - 1.0)) *
+                                                             (double)vxx) -
+                                                            (double)((djm - e) * vyy)));
+                    a.p[mc + (long)(l - 1) * a.ls] = pn;
+                    sump += pn;
                 }
-                a.sxx[c1] = a.sxx[c1] + (dthalbe * sump);
-                if (a.ux) a.ux[c1] = (float)((double)a.ux[c1] + (0.5 * (double)sump));
+                a.sxx[c1] = sxx2 + (dthalbe * sump);
+                if (a.ux) a.ux[c1] = (float)((double)ux2 + (0.5 * (double)sump));
             }
         }
     } else if (t < g.NX + a.ntr) {
