@@ -16,6 +16,12 @@
  * (third index slowest), so every kernel reads and writes whole planes; the
  * float*** it returns is a handle for these calls, not for host indexing,
  * and ifos_dev_copy_array() converts from and to util.c's [j][i][l] block.
+ * update_v_PML and psource keep a device copy of the whole signals block
+ * and of the source positions between calls; each call compares the
+ * samples and positions it reads with that copy and uploads them again when
+ * they differ, so changed host values take effect at the next step as in
+ * the reference.  The layer serves one host thread, like the reference's
+ * driver (one process per MPI rank).
  *
  * Globals read by these functions, by the reference names (src/globvar.h):
  * DT, DH, NX, NY, NT, FDORDER, FW, FREE_SURF, BOUNDARY, L, FL, F_REF,
