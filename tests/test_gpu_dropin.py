@@ -101,3 +101,45 @@ def test_c_client_forward_matches_oracle(gpu_lib, tmp_path):
     np.testing.assert_array_equal(vx, ovx)
     np.testing.assert_array_equal(vy, ovy)
     np.testing.assert_array_equal(fin, ofin)
+
+
+@pytest.mark.parametrize("name", ["el4_fs_v0", "el2_nofs_rot", "visc6_fs_fx_v1"])
+def test_source_series_changed_between_steps(gpu_lib, name):
+    """update_v_PML and psource keep a device copy of the host signals and
+    check at every step the samples that step reads (psource reads columns
+    nt-1..nt+1, update_v_PML column nt).  Signals overwritten in place after
+    step k with a series equal to the first up to column k+2 must give the
+    same bits as running with the second series from the start: the first
+    step that reads a changed sample (k+2) has to see it."""
+    from ifos_amd.dropin import HostNR
+    cfg, z, true, _ = load_case(name)
+    lib = ifos_amd.lib()
+    src = np.ascontiguousarray(z["srcpos"], np.float32).reshape(-1, 8)
+    nsrc, NT = src.shape[0], cfg.NT
+    k = NT // 2
+    s1 = np.zeros((nsrc, NT), np.float32)
+    lib.ifos_wavelet(C.byref(cfg), nsrc, src.ctypes.data_as(C.POINTER(C.c_float)),
+                     s1.ctypes.data_as(C.POINTER(C.c_float)))
+    s2 = s1.copy()
+    s2[:, k + 2:] *= np.float32(-0.7)  # 1-based columns k+3.. differ
+    srcpos_loc = HostNR(1, 8, 1, nsrc)
+    for l in range(nsrc):
+        for c in range(8):
+            srcpos_loc[c + 1, l + 1] = src[l, c]
+    out = []
+    for in_place in (True, False):
+        print(f"[drop-in series] {name} in place {in_place}", flush=True)
+        run = DropinShot(lib, cfg, true)
+        signals = HostNR(1, nsrc, 1, NT)
+        signals.a[:] = s1 if in_place else s2
+        run.zero()
+        for nt in range(1, NT + 1):
+            if in_place and nt == k + 1:
+                signals.a[:] = s2
+            run.step(nt, srcpos_loc.ptr, signals.ptr, signals.ptr, nsrc, 0)
+        assert lib.ifos_dev_sync() == 0
+        out.append([run.download(f) for f in ("vx", "vy", "sxx", "syy", "sxy")])
+        run.close()
+    for a, b, f in zip(out[0], out[1], ("vx", "vy", "sxx", "syy", "sxy")):
+        np.testing.assert_array_equal(a, b, err_msg=f)
+    assert np.abs(out[1][1]).max() > 0
