@@ -463,7 +463,8 @@ def batched_vs_plain(args):
     import ifos_amd
     w = build_workload(args.workload)
     cfg = w.cfg
-    S = max(2, args.batch)
+    # shots per launch: as given, or the MALL-sized count the library suggests
+    S = args.batch if args.batch >= 2 else max(2, ifos_amd.mall_slots(cfg))
     n = S * max(1, args.steps)
     shots = [(s.srcpos, s.recpos) for s in w.shots[:n]]
     eng = ifos_amd.Engine(cfg, slots=S)
@@ -536,7 +537,8 @@ def batched_vs_plain(args):
            "unit": "shot-gradients/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
            "dtype": f"fp32 stencils, {args.snap} snapshots", "data": "synthetic (seeded layered model with faults)",
            "config": {"workload": w.name, "grid": [cfg.NX, cfg.NY], "NT": cfg.NT, "DTINV": cfg.DTINV, "shots": n,
-                      "batch": S, "snapshots": args.snap, "launch": shape},
+                      "batch": S, "batch_rule": "given" if args.batch >= 2 else "MALL-sized (ifos_engine_mall_slots)",
+                      "snapshots": args.snap, "launch": shape},
            "plain": {"shot_gradients_per_s": round(rate_p, 4), "algorithmic_bytes_per_shot_gradient": bytes_plain,
                      "GB_s": round(bytes_plain * rate_p / 1e9, 1)},
            "batched": {"shot_gradients_per_s": round(rate_b, 4), "algorithmic_bytes_per_shot_gradient": bytes_bat,
@@ -558,7 +560,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--batch", type=int, default=1,
-                    help="shots advanced together per kernel launch (1: the one-shot-at-a-time path)")
+                    help="shots advanced together per kernel launch (1: the one-shot-at-a-time path; with "
+                         "--batched-vs-plain, 1 picks the MALL-sized count of ifos_engine_mall_slots)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape (no timing sweep)")
     ap.add_argument("--order", default="auto", choices=["auto", "tile", "slot"],
                     help="work order of several-shot launches (auto: the autotuner's pick)")

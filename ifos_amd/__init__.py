@@ -385,6 +385,16 @@ class Engine:
 SNAP_PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
 
 
+def mall_slots(config):
+    """Shots per launch whose per-step working set fits the MALL (ifos_engine_mall_slots)."""
+    L = lib()
+    L.ifos_engine_mall_slots.argtypes = [C.POINTER(Config)]
+    n = L.ifos_engine_mall_slots(C.byref(config))
+    if n < 1:
+        raise IfosError("invalid config")
+    return n
+
+
 def snapshot_bytes(config, prec=0):
     """Device bytes of one shot's snapshot storage at that precision."""
     return int(lib().ifos_snapshot_bytes(C.byref(config), SNAP_PRECISIONS.get(prec, prec)))

@@ -1491,6 +1491,25 @@ extern "C" size_t ifos_engine_bytes(const ifos_config *c, int slots, int snap_pr
     return b;
 }
 
+// Shots per launch whose per-step working set stays in the MALL (the 256 MB
+// Infinity Cache between the L2s and HBM): each slot touches its velocities,
+// stresses and memory variables every step, all slots share the buoyancies
+// and moduli.  A budget of 160 MB leaves room for the snapshot and trace
+// streams.  On the 1700x350, L=2 grid this gives 5; launches of 4-5 shots ran
+// at 10.6 shot-gradients/s there, 8 at 8.9 and 16 at 8.7 (DRAM-bound), 1 at
+// 6.8 (the chip underfilled).
+extern "C" int ifos_engine_mall_slots(const ifos_config *c)
+{
+    if (validate(c)) return -2;
+    constexpr double kMallBudget = 160e6;
+    const int N = c->FDORDER / 2, H = N + 1, A = 16;
+    const double pitch = (double)(((A + c->NX + H) + 63) / 64 * 64);
+    const double field = 4.0 * pitch * (double)(c->NY + 2 * H + 1);
+    const double per_slot = field * (5 + 3 * c->L), shared = field * (c->L ? 8 : 5);
+    const double n = (kMallBudget - shared) / per_slot;
+    return n < 1.0 ? 1 : (n > 4096.0 ? 4096 : (int)n);
+}
+
 extern "C" size_t ifos_snapshot_bytes(const ifos_config *c, int prec)
 {
     if (validate(c) || prec < 0 || prec > 2) return 0;

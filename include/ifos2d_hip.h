@@ -270,6 +270,12 @@ size_t ifos_snapshot_bytes(const ifos_config *c, int prec);
  * receivers and nsrc_total sources in all, with snapshot storage at that
  * precision when snapshots != 0 (0 for an invalid configuration); the
  * admission budget of the request server (ifos2d_serve.h) counts this. */
+/* Shots per launch for several-shot engines: the largest slot count whose
+ * per-step working set (each slot's wavefields and memory variables, the
+ * shared materials) fits in 160 MB of the 256 MB MALL; at least 1.  Larger
+ * launches turn DRAM-bound: on the 1700x350, L=2 grid 4-5 shots per launch
+ * gave 10.6 shot-gradients/s, 16 gave 8.7.  -2 for an invalid config. */
+int ifos_engine_mall_slots(const ifos_config *c);
 size_t ifos_engine_bytes(const ifos_config *c, int slots, int snap_prec, int snapshots, int ntr_total,
                          int nsrc_total);
 /* Device memory helpers for callers without a GPU runtime of their own. */

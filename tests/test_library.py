@@ -90,3 +90,28 @@ def test_engine_rejects_bad_config():
     cfg = ifos_amd.make_config(NX=50, NY=40, DH=1.0, DT=1e-4, NT=100, FDORDER=4, DTINV=2, BOUNDARY=1)
     with pytest.raises(ifos_amd.IfosError, match="BOUNDARY"):
         ifos_amd.Engine(cfg)
+
+
+def test_mall_slots_rule():
+    """ifos_engine_mall_slots: the largest slot count whose per-step working
+    set (per slot 4 B x padded field x (5 + 3L), shared 4 B x padded field x
+    (8 if L else 5)) fits 160 MB; host arithmetic only."""
+    import ifos_amd
+    from ifos_amd import workloads
+
+    def rule(cfg):
+        N = cfg.FDORDER // 2
+        H = N + 1
+        pitch = ((16 + cfg.NX + H) + 63) // 64 * 64
+        field = 4.0 * pitch * (cfg.NY + 2 * H + 1)
+        n = (160e6 - field * (8 if cfg.L else 5)) / (field * (5 + 3 * cfg.L))
+        return max(1, min(4096, int(n)))
+
+    for name, want in (("batched_aperture", 5), ("layered_visc", 1), ("precision_large", 1)):
+        cfg = getattr(workloads, name)().cfg
+        assert ifos_amd.mall_slots(cfg) == want == rule(cfg), name
+    cfg = workloads.toy_elastic().cfg
+    assert ifos_amd.mall_slots(cfg) == rule(cfg) > 16
+    bad = ifos_amd.make_config(NX=0, NY=10, NT=10, FDORDER=4, DH=1.0, DT=1e-3)
+    with pytest.raises(ifos_amd.IfosError):
+        ifos_amd.mall_slots(bad)
