@@ -42,6 +42,7 @@ struct Req {
     std::vector<int> src0, tr0;  // first source row and trace of each shot
     std::vector<float> srcpos, vp, vs, rho, taus, taup, obs_vx, obs_vy;
     long long model_id = 0;
+    int cap = 1;                 // shots per launch: max_slots, at most max(2, MALL-sized count)
     int slots = 1;               // shots per launch with which it fits the budget alone
     int overtaken = 0;
     int next_shot = 0, done_shots = 0, folded = 0;
@@ -196,7 +197,7 @@ struct ifos_server {
     bool plan(size_t q, Launch &L, Resident **where)
     {
         const auto &r = queue[q];
-        for (int cap = max_slots;; cap = (cap + 1) / 2) {
+        for (int cap = r->cap;; cap = (cap + 1) / 2) {
             L = Launch();
             // the lead request's next shots, then later requests' of the same kind
             for (size_t k = q; k < queue.size() && (int)L.shots.size() < cap; k++) {
@@ -511,7 +512,11 @@ extern "C" int ifos_server_submit(ifos_server *s, const ifos_request *q, long lo
         r->seis_vx.assign((size_t)nt * q->cfg.NT, 0.0f);
         r->seis_vy.assign((size_t)nt * q->cfg.NT, 0.0f);
     }
-    r->slots = std::min(s->max_slots, q->nshots);
+    // launches wider than the MALL holds turn DRAM-bound and slower per shot
+    // (on a 2000x800 grid: 2 shots per launch 2.57, 1 shot 2.38, 4 shots 2.13
+    // shot-gradients/s); two are allowed where one already exceeds it
+    r->cap = std::min(s->max_slots, std::max(2, ifos_engine_mall_slots(&q->cfg)));
+    r->slots = std::min(r->cap, q->nshots);
     // fewer shots per launch when the request's engine would not fit in the budget
     size_t bytes = 0;
     for (;;) {

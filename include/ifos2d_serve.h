@@ -11,11 +11,12 @@
  *    (and keep their model when the model id matches); idle engines are
  *    evicted, least recently used first, when a new request needs the room.
  *  - Continuous batching: work is admitted as launches of up to max_slots
- *    shots on one engine.  A launch takes the oldest waiting request's next
- *    shots and fills its free slots with the next shots of later requests of
- *    the same configuration, kind, snapshot precision and model id (!= 0),
- *    so requests that arrive while others run join the next launch of their
- *    kind.  Worker threads each drive their own engine and stream, so
+ *    shots on one engine, and of at most max(2, ifos_engine_mall_slots(cfg))
+ *    shots for a grid whose wider launches would not stay in the MALL.  A
+ *    launch takes the oldest waiting request's next shots and fills its free
+ *    slots with the next shots of later requests of the same configuration,
+ *    kind, snapshot precision and model id (!= 0), so requests that arrive
+ *    while others run join the next launch of their kind.  Worker threads each drive their own engine and stream, so
  *    launches of different requests overlap on the device.
  *  - Order: first come first served, except that requests behind one that
  *    does not fit may start while it waits; after 8 such overtakes the
