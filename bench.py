@@ -11,8 +11,10 @@ dominant kernel from HIP-event timing, and a CPU baseline measured with the
 reference's own kernels, oracle/_ref, on a bounded sample of the same grid).
 """
 import argparse
+import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -563,6 +565,8 @@ def main():
                     help="shots advanced together per kernel launch (1: the one-shot-at-a-time path; with "
                          "--batched-vs-plain, 1 picks the MALL-sized count of ifos_engine_mall_slots)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape (no timing sweep)")
+    ap.add_argument("--no-several", action="store_true",
+                    help="skip the companion run with several shots per launch (max(2, MALL-sized), at most 8)")
     ap.add_argument("--order", default="auto", choices=["auto", "tile", "slot"],
                     help="work order of several-shot launches (auto: the autotuner's pick)")
     ap.add_argument("--vel-rows", type=int, default=0, choices=[0, 1, 2, 4],
@@ -737,6 +741,30 @@ def main():
                 velocity_kernel_GBs=round(eng.kernel_bytes_per_cell(0) * cfg.NX * cfg.NY * S / (ms_v * 1e-3) / 1e9, 1))
     eng.close()
 
+    # the same job with several shots per launch (bit-identical gradients),
+    # measured in a child process of its own so that its engine starts clean
+    several = None
+    B = max(2, min(8, ifos_amd.mall_slots(cfg)))
+    prec = {"fp32": 0, "bf16": 1, "fp16": 2}[args.snap]
+    ntr, nsrc = len(w.shots[0].recpos), len(np.asarray(w.shots[0].srcpos).reshape(-1, 8))
+    fits = ifos_amd.lib().ifos_engine_bytes(ctypes.byref(cfg), B, prec, 1, ntr * B, nsrc * B) < 250e9
+    if S == 1 and not args.no_several and fits:
+        cmd = [sys.executable, os.path.abspath(__file__), "--workload", args.workload, "--batch", str(B),
+               "--steps", str(args.steps), "--warmup", str(args.warmup), "--snap", args.snap,
+               "--no-cpu-baseline", "--no-several"]
+        log(f"[bench] {B} shots per launch, child process")
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
+        lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+        if r.returncode == 0 and lines:
+            d = json.loads(lines[-1])
+            several = dict(shots_per_launch=B, value=d["value"], unit=d["unit"], ms_per_step=d["ms_per_step"],
+                           note="the same shots, gradients bit-identical to one shot per launch",
+                           roofline=dict(frac=d["roofline"]["frac"], achieved=d["roofline"]["achieved"],
+                                         avg_ms=d["roofline"]["avg_ms"], bytes_per_launch=d["roofline"]["bytes_per_cell"]
+                                         * cfg.NX * cfg.NY * B))
+        else:
+            several = dict(shots_per_launch=B, value=None, error=f"child exited with {r.returncode}")
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -771,6 +799,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if several is not None:
+        out["several_shots_per_launch"] = several
     print(json.dumps(out))
     return 0
 
