@@ -20,7 +20,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import DeviceBuffer, Engine
+from . import DeviceBuffer, Engine, mall_slots
 from .opt import Optimizer, update_params
 
 
@@ -42,10 +42,15 @@ class FWISettings:
 
 
 class FWI:
-    def __init__(self, workload, settings=None, slots=1, snap="fp32", device=0):
+    def __init__(self, workload, settings=None, slots=0, snap="fp32", device=0):
+        """slots: shots advanced per launch; 0 sizes it to the MALL
+        (ifos_engine_mall_slots, at least 2, at most the number of shots).
+        Gradients are the same bits for every count."""
         self.w = workload
         self.s = settings or FWISettings()
         cfg = workload.cfg
+        if not slots:
+            slots = max(1, min(len(workload.shots), max(2, mall_slots(cfg))))
         self.cfg = cfg
         self.n = cfg.NX * cfg.NY
         self.eng = Engine(cfg, device=device, slots=slots)

@@ -34,7 +34,7 @@ def anomaly_case():
     return w
 
 
-@pytest.mark.parametrize("method,slots", [(1, 1), (1, 3), (2, 2)])
+@pytest.mark.parametrize("method,slots", [(1, 1), (1, 3), (2, 2), (1, 0)])
 def test_fwi_iterations_reduce_misfit(gpu_lib, method, slots):
     w = anomaly_case()
     s = FWISettings(GRAD_METHOD=method, TESTSHOT_START=1, TESTSHOT_END=3, TESTSHOT_INCR=1, EPS_SCALE=0.01)
