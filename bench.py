@@ -576,6 +576,8 @@ def main():
                     help="velocity step with two rows per thread and the tile's stresses staged in LDS")
     ap.add_argument("--snap", default="fp32", choices=["fp32", "bf16", "fp16"],
                     help="snapshot storage (fp32 keeps the gradient bit-identical to the reference)")
+    ap.add_argument("--copy-floor", action="store_true",
+                    help="with --kernels-only: also time a device copy moving each kernel's algorithmic bytes")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the velocity and stress kernels on the live model (for counter runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
@@ -668,8 +670,15 @@ def main():
             f"velocity rows per thread {eng.get_velocity_rows()}, LDS {eng.get_velocity_lds()}")
         for kern in (0, 1):
             ms = eng.time_kernel(kern, reps=20)
-            gbs = eng.kernel_bytes_per_cell(kern) * cfg.NX * cfg.NY * S / (ms * 1e-3) / 1e9
+            nbytes = eng.kernel_bytes_per_cell(kern) * cfg.NX * cfg.NY * S
+            gbs = nbytes / (ms * 1e-3) / 1e9
             log(f"[bench] kernel {kern}: {ms:.5f} ms/launch, {gbs:.1f} GB/s algorithmic")
+            if args.copy_floor:
+                mc = eng.time_copy(nbytes)
+                log(f"[bench] copy of the same {nbytes / 1e6:.1f} MB (half read, half written), back to back: "
+                    f"{mc:.5f} ms/launch, {nbytes / (mc * 1e-3) / 1e9:.1f} GB/s; kernel / copy {ms / mc:.3f}")
+        if args.copy_floor:
+            log(f"[bench] copy of 4 KB, back to back: {eng.time_copy(4096):.5f} ms/launch")
         eng.close()
         return 0
     # observed data: forward runs on the true model (untimed), kept in HBM

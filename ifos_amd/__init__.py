@@ -84,6 +84,8 @@ def lib():
                   "set_eprecond", "apply_eprecond", "energy_weight"]:
             getattr(L, "ifos_engine_" + n).restype = C.c_int
         L.ifos_snapshot_bytes.restype = C.c_size_t
+        L.ifos_engine_time_copy.restype = C.c_int
+        L.ifos_engine_time_copy.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.ifos_engine_set_eprecond.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_int]
         L.ifos_engine_bytes.restype = C.c_size_t
         L.ifos_engine_bytes.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -335,6 +337,13 @@ class Engine:
     def time_kernel(self, kernel, reps=20):
         ms = C.c_double()
         _check(lib().ifos_engine_time_kernel(self.h, kernel, reps, C.byref(ms)))
+        return ms.value
+
+    def time_copy(self, nbytes, reps=20):
+        """Average ms of a device copy reading and writing nbytes/2 each,
+        back to back on the engine's stream."""
+        ms = C.c_double()
+        _check(lib().ifos_engine_time_copy(self.h, int(nbytes), reps, C.byref(ms)))
         return ms.value
 
     def kernel_bytes_per_cell(self, kernel):

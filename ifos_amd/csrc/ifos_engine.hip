@@ -1653,6 +1653,41 @@ extern "C" int ifos_memcpy_d2d(void *dst, const void *src, size_t bytes)
 // ---------------------------------------------------------------------------
 // kernel timing for the roofline figures
 // ---------------------------------------------------------------------------
+
+// device-to-device copies reading and writing bytes/2 each, reps of them back
+// to back on the engine's stream between two events: the streaming rate a
+// kernel moving those bytes could reach at that size, launch and drain
+// included
+extern "C" int ifos_engine_time_copy(ifos_engine *E, size_t bytes, int reps, double *ms_avg)
+{
+    HIP_TRY(hipSetDevice(E->device));
+    if (reps < 1) reps = 1;
+    const size_t half = std::max<size_t>(256, bytes / 2 / 256 * 256);
+    void *src = nullptr, *dst = nullptr;
+    if (dmalloc(&src, half) != hipSuccess || dmalloc(&dst, half) != hipSuccess) {
+        dfree(src);
+        return ifos_set_error("cannot allocate 2 x %zu bytes for the copy timing", half), -3;
+    }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    float ms = 0.0f;
+    hipError_t e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    if (e == hipSuccess) e = hipMemsetAsync(src, 0, half, E->stream);
+    for (int k = 0; e == hipSuccess && k < 2; k++) e = hipMemcpyAsync(dst, src, half, hipMemcpyDeviceToDevice, E->stream);
+    if (e == hipSuccess) e = hipEventRecord(t0, E->stream);
+    for (int k = 0; e == hipSuccess && k < reps; k++)
+        e = hipMemcpyAsync(dst, src, half, hipMemcpyDeviceToDevice, E->stream);
+    if (e == hipSuccess) e = hipEventRecord(t1, E->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(t1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+    if (t0) hipEventDestroy(t0);
+    if (t1) hipEventDestroy(t1);
+    dfree(src);
+    dfree(dst);
+    if (e != hipSuccess) return ifos_set_error("copy timing: %s", hipGetErrorString(e)), -1;
+    *ms_avg = ms / reps;
+    return 0;
+}
 extern "C" double ifos_engine_kernel_bytes_per_cell(ifos_engine *E, int kernel)
 {
     const int L = E->c.L;
@@ -1712,10 +1747,15 @@ extern "C" int ifos_engine_time_kernel(ifos_engine *E, int kernel, int reps, dou
     return 0;
 }
 
+// persistent-grid candidates of the tuner, workgroups per compute unit (0:
+// one workgroup per work item).  Grids of 8-12 per CU beat one workgroup per
+// tile by 4-10 % on 1-5 shots of 1.6 M and 0.6 M cells per launch.
+static constexpr int kGridsPerCU[] = {0, 2, 3, 4, 6, 8, 10, 12};
+
 // Launch-shape tuner: times one forward step (velocity + stress kernels, no
 // snapshot) of the current model and shots for every candidate shape --
 // 1, 2 or 4 shots per workgroup (up to the slots in use) times one
-// workgroup per work item or a persistent grid of 2, 3 or 4 workgroups per
+// workgroup per work item or a persistent grid of 2 to 12 workgroups per
 // compute unit -- keeps the fastest, and zeroes the wavefields it stepped.
 extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int *grid_out, double *ms_out)
 {
@@ -1754,7 +1794,7 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
             ENG_TRY(upload_args(E, a, 3, &da));
             float best_v = 1e30f, best_s = 1e30f;
             int grid_v = 0, grid_s = 0, rows_v = 1, lds_v = 0;
-            for (int per_cu : {0, 2, 3, 4})
+            for (int per_cu : kGridsPerCU)
             // rows per thread; the LDS-staged variant (lds_v) ran at half the
             // speed of two rows per thread on every workload measured, so it is
             // not a candidate here
@@ -1799,7 +1839,7 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
     bool fused = false;
     for (float cw : {0.0f, 1.0f, 2.0f})
         for (int order = 0; order < (E->nslots > 1 ? 2 : 1); order++)
-            for (int per_cu : {0, 2, 3, 4}) {
+            for (int per_cu : kGridsPerCU) {
                 LaunchShape ls = pick;
                 ls.spb = 1;
                 ls.order = order;

@@ -289,6 +289,11 @@ int  ifos_memcpy_d2d(void *dst, const void *src, size_t bytes);
  * without snapshot).  Runs reps launches on the live state, then restores
  * nothing: call between shots.  Returns the average in ms. */
 int  ifos_engine_time_kernel(ifos_engine *e, int kernel, int reps, double *ms_avg);
+/* Time reps device-to-device copies that read and write bytes/2 each, back
+ * to back on the engine's stream (HIP events; no counterpart in the
+ * reference): the streaming rate a kernel moving that many bytes per launch
+ * can reach at that size.  Returns the average in ms. */
+int  ifos_engine_time_copy(ifos_engine *e, size_t bytes, int reps, double *ms_avg);
 /* Algorithmic HBM bytes per cell of one launch of that kernel. */
 double ifos_engine_kernel_bytes_per_cell(ifos_engine *e, int kernel);
 /* Synchronise and report the first NaN step seen by the stability probe (0: none). */
