@@ -224,10 +224,11 @@ static int validate(const ifos_config *c)
     if (c->FW < 0 || 2 * c->FW > c->NX || 2 * c->FW > c->NY)
         return ifos_set_error("FW (%d) must fit twice into NX and NY", c->FW), -2;
     if (!(c->DT > 0) || !(c->DH > 0)) return ifos_set_error("DT and DH must be positive"), -2;
-    // BOUNDARY 1 makes the sides periodic through the wavefield exchanges
-    // of the domain decomposition (src/exchange_v.c, src/exchange_s.c), which
-    // the one-grid engine does not have
-    if (c->BOUNDARY != 0) return ifos_set_error("BOUNDARY must be 0 (periodic sides are not supported)"), -2;
+    // BOUNDARY 1: periodic sides, no CPML on the left and right edges; the
+    // step kernels copy the edge columns into the opposite halo as the
+    // reference's exchanges do on one rank (src/exchange_v.c, src/exchange_s.c)
+    if (c->BOUNDARY != 0 && c->BOUNDARY != 1) return ifos_set_error("BOUNDARY must be 0 or 1"), -2;
+    if (c->BOUNDARY && c->NX < c->FDORDER) return ifos_set_error("BOUNDARY 1 needs NX >= FDORDER"), -2;
     return 0;
 }
 
@@ -1434,6 +1435,9 @@ extern "C" int ifos_engine_slot_gradient(ifos_engine *E, int slot, float *g_vp, 
 extern "C" int ifos_engine_set_fused(ifos_engine *E, int on)
 {
     HIP_TRY(hipSetDevice(E->device));
+    if (on && E->c.BOUNDARY)
+        return ifos_set_error("the one-kernel step has no periodic sides (BOUNDARY 1): its tiles form their "
+                              "neighbours' velocities from stresses beyond the halo copies"), -2;
     HIP_TRY(hipStreamSynchronize(E->stream));
     if ((on != 0) != E->fused) drop_graphs(E);
     E->fused = on != 0;
@@ -1834,10 +1838,11 @@ extern "C" int ifos_engine_autotune(ifos_engine *E, int reps, int *spb_out, int 
                 pick.grid = grid_s;
             }
         }
-    // the one-kernel step (one shot per workgroup): every CPML cost weight,
-    // work order and grid, timed back to back
+    // the one-kernel step (one shot per workgroup; not with periodic sides):
+    // every CPML cost weight, work order and grid, timed back to back
     bool fused = false;
     for (float cw : {0.0f, 1.0f, 2.0f})
+        if (!E->c.BOUNDARY)
         for (int order = 0; order < (E->nslots > 1 ? 2 : 1); order++)
             for (int per_cu : kGridsPerCU) {
                 LaunchShape ls = pick;

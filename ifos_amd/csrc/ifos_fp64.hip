@@ -602,6 +602,7 @@ extern "C" int ifos_fp64_create(const ifos_config *cfg, int device, int segment_
         return ifos_set_error("fp64: NT must be a positive multiple of DTINV"), -2;
     if (cfg->L < 0 || cfg->L > 7) return ifos_set_error("fp64: L must be 0..7"), -2;
     if (cfg->PARAMETERIZATION != 1) return ifos_set_error("fp64: PARAMETERIZATION 1 only"), -2;
+    if (cfg->BOUNDARY != 0) return ifos_set_error("fp64: BOUNDARY 0 only (no periodic halo copies)"), -2;
     if (cfg->FW < 0 || 2 * cfg->FW > cfg->NX || (!cfg->FREE_SURF && 2 * cfg->FW > cfg->NY) || cfg->FW > cfg->NY)
         return ifos_set_error("fp64: CPML wider than the grid"), -2;
     const int C = segment_steps > 0 ? segment_steps : default_segment(cfg);

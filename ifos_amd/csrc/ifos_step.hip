@@ -284,6 +284,31 @@ __device__ __forceinline__ float snap_load(const StepArgs &a, int s, int f, long
 }
 
 // ---------------------------------------------------------------------------
+// Periodic sides (BOUNDARY 1).  On one rank the reference's exchanges after
+// each half step copy the opposite edge into the halo columns
+// (src/exchange_v.c:143-228, src/exchange_s.c:145-229): after the velocity
+// step vy to columns NX+1..NX+N and 2-N..0, vx to NX+1..NX+N-1 and 1-N..0;
+// after the stress step (sources and free surface included) sxx to
+// NX+1..NX+N and 2-N..0, sxy to NX+1..NX+N-1 and 1-N..0.  The cell that owns
+// the final value also stores it into its halo copy.  No kernel of the same
+// launch reads those halo columns: the velocity step reads stress halos, the
+// stress step velocity halos.  The padded rows hold at least 16 columns left
+// of column 1 and N+1 right of column NX, so the copies stay in their row.
+template <int N, typename Ptr>
+__device__ __forceinline__ void wrap_pair(const StepGeom &g, Ptr a, Ptr b, long c, int i, float av, float bv)
+{
+    // a: the field with N copies on the right and N-1 on the left (vy, sxx),
+    // b: N-1 on the right and N on the left (vx, sxy)
+    if (i <= N) {
+        a[c + g.NX] = av;
+        if (i <= N - 1) b[c + g.NX] = bv;
+    }
+    if (i >= g.NX - N + 1) {
+        b[c - g.NX] = bv;
+        if (i >= g.NX - N + 2) a[c - g.NX] = av;
+    }
+}
+
 // velocity step of one cell of slot s
 template <int N>
 __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int i, int j, bool live, long c, long e,
@@ -347,6 +372,7 @@ __device__ __forceinline__ void vel_cell(const StepArgs &a, int s, int blk, int 
     }
     vxs[c] = vx;
     vys[c] = vy;
+    if (g.BOUNDARY) wrap_pair<N>(g, vys, vxs, c, i, vy, vx);
     // energy of the new particle velocities (eprecond, src/eprecond.c:8-21)
     if (flags & kEnergy) {
         if (a.wacc64) {
@@ -457,6 +483,7 @@ __device__ __forceinline__ void vel_rows_with(const StepArgs &a, int s, int blk,
         if (live[r]) {
             vxs[c + r * P] = vx[r];
             vys[c + r * P] = vy[r];
+            if (g.BOUNDARY) wrap_pair<N>(g, vys, vxs, c + r * P, i, vy[r], vx[r]);
         }
     if (flags & kEnergy) {
         if (a.wacc64) {
@@ -754,6 +781,7 @@ __device__ __forceinline__ void str_cell(const StepArgs &a, int s, int blk, int 
     gmem(a.sxx)[wo + c] = sxx;
     gmem(a.syy)[wo + c] = syy;
     gmem(a.sxy)[wo + c] = sxy;
+    if (g.BOUNDARY) wrap_pair<N>(g, gmem(a.sxx) + wo, gmem(a.sxy) + wo, c, i, sxx, sxy);
     if (L) {
 #pragma unroll
         for (int l = 0; l < L; l++) {

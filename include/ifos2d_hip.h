@@ -42,7 +42,7 @@ typedef struct ifos_config {
     int MAXRELERROR;       /* row of the Holberg table, 0..4                   */
     int FW;                /* CPML width in cells                              */
     int FREE_SURF;         /* free surface at j = 1                            */
-    int BOUNDARY;          /* 1: no CPML on the left/right edges               */
+    int BOUNDARY;          /* 1: periodic left/right edges, no CPML there      */
     int L;                 /* relaxation mechanisms (0: elastic)               */
     int VELOCITY;          /* 0: displacement misfit / snapshots of stress     */
     int ADJOINT_TYPE;      /* 1: vx+vy residuals, 2: vy, 3: vx                 */

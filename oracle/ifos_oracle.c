@@ -815,11 +815,31 @@ static void surface_visc(shot_t *S)
     }
 }
 
+/* ---- exchange_v / exchange_s on one rank --------------------------------
+ * src/exchange_v.c:143-228 and src/exchange_s.c:145-229 with NPROCX = 1: the
+ * rank is its own left and right neighbour, so with BOUNDARY 1 the halo
+ * columns take the opposite edge's values; with BOUNDARY 0 nothing is
+ * unpacked.  a gets N columns on the right (NX+1..NX+N from 1..N) and N-1
+ * on the left (2-N..0 from NX-N+2..NX), b the other way round: (vy, vx)
+ * after the velocity update, (sxx, sxy) after the stress update. */
+static void exchange_x(shot_t *S, real **a, real **b)
+{
+    if (!S->P.BOUNDARY) return;
+    const int N = S->fdoh, NX = S->P.NX, NY = S->P.NY;
+    for (int j = 1; j <= NY; j++) {
+        for (int l = 1; l <= N; l++) a[j][NX + l] = a[j][l];
+        for (int l = 1; l <= N - 1; l++) b[j][NX + l] = b[j][l];
+        for (int l = 1; l <= N - 1; l++) a[j][1 - l] = a[j][NX - l + 1];
+        for (int l = 1; l <= N; l++) b[j][1 - l] = b[j][NX - l + 1];
+    }
+}
+
 /* ---- one full time step in the driver's order (src/IFOS2D.c:1651-1803) -- */
 static void time_step(shot_t *S, int nt, int sw, int nsrc, const float *srcpos, const real *sig, const real *sig1,
                       int sig_stride, const float *fsig, int explosive)
 {
     update_v(S, nt, sw, nsrc, srcpos, sig, sig1, sig_stride);
+    exchange_x(S, S->vy, S->vx);
     if (S->P.L) update_s_visc(S);
     else update_s_elastic(S);
     if (sw == 0 && explosive) psource_fwd(S, nt, nsrc, srcpos, fsig);
@@ -827,6 +847,7 @@ static void time_step(shot_t *S, int nt, int sw, int nsrc, const float *srcpos, 
         if (S->P.L) surface_visc(S);
         else surface_elastic(S);
     }
+    exchange_x(S, S->sxx, S->sxy);
 }
 
 /* ---- 16-bit snapshot storage (no reference counterpart) ----------------- */

@@ -8,7 +8,7 @@
  * The reference kernels (update_v_PML, update_s_elastic_PML,
  * update_s_visc_PML, psource, surface_elastic_PML, surface_PML, seismo_ssg,
  * holbergcoeff, PML_pro, matcopy*, av_*, prepare_update_s, wavelet,
- * zero_fdveps*) are called unmodified, with the globals they read defined by
+ * zero_fdveps*, exchange_v, exchange_s) are called unmodified, with the globals they read defined by
  * the reference's own globvar.h.  The per-step call order restates
  * src/IFOS2D.c:1651-1803 (forward) and :2224-2419 (adjoint); the snapshot
  * store, zero-lag correlation and chain rule are inline code in the
@@ -125,6 +125,8 @@ typedef struct {
     float ***pt, ***po;
     float **psi_sxx_x, **psi_sxy_x, **psi_vxx, **psi_vyx, **psi_vxxs, **psi_sxz_x, **psi_vzx;
     float **psi_syy_y, **psi_sxy_y, **psi_vyy, **psi_vxy, **psi_syz_y, **psi_vzy;
+    /* halo buffers of exchange_v / exchange_s (src/IFOS2D.c:649-652) */
+    float **buf_lr, **buf_rl, **buf_tb, **buf_bt;
 } run_t;
 
 static void run_init(run_t *R, const float *vp, const float *vs, const float *rho, const float *taus,
@@ -185,6 +187,9 @@ static void run_init(run_t *R, const float *vp, const float *vs, const float *rh
     R->psi_syy_y = matrix(1, fw2, 1, NX); R->psi_sxy_y = matrix(1, fw2, 1, NX);
     R->psi_vyy = matrix(1, fw2, 1, NX); R->psi_vxy = matrix(1, fw2, 1, NX);
     R->psi_syz_y = matrix(1, fw2, 1, NX); R->psi_vzy = matrix(1, fw2, 1, NX);
+    const int fdo3 = 2 * nd; /* WAVETYPE 1 (src/IFOS2D.c:311-325) */
+    R->buf_lr = matrix(1, NY, 1, fdo3); R->buf_rl = matrix(1, NY, 1, fdo3);
+    R->buf_tb = matrix(1, NX, 1, fdo3); R->buf_bt = matrix(1, NX, 1, fdo3);
 #undef GRID
 #undef TENS
 }
@@ -213,6 +218,10 @@ static void run_step(run_t *R, int nt, int sw, float **srcpos_loc, float **sig, 
                  R->psxx, R->psyy, R->psxy, R->prip, R->prjp, srcpos_loc, sig, sig1, nsrc, NULL, R->hc, 0, sw,
                  P->K_x, P->a_x, P->b_x, P->K_xh, P->a_xh, P->b_xh, P->K_y, P->a_y, P->b_y, P->K_yh, P->a_yh,
                  P->b_yh, R->psi_sxx_x, R->psi_syy_y, R->psi_sxy_y, R->psi_sxy_x);
+    /* on one rank the exchanges fill only the periodic halo columns of
+     * BOUNDARY 1; with BOUNDARY 0 they send to the rank itself and unpack
+     * nothing */
+    exchange_v(R->pvx, R->pvy, R->pvz, R->buf_lr, R->buf_rl, R->buf_tb, R->buf_bt, NULL, NULL, 1);
     if (L)
         update_s_visc_PML(1, NX, 1, NY, R->pvx, R->pvy, R->ux, R->uy, R->uxy, R->uyx, R->psxx, R->psyy, R->psxy,
                           R->ppi, R->pu, R->puipjp, R->prho, R->hc, 0, R->pr, R->pp, R->pq, R->fipjp, R->f, R->g,
@@ -235,6 +244,8 @@ static void run_step(run_t *R, int nt, int sw, float **srcpos_loc, float **sig, 
                                 R->hc, P->K_x, P->a_x, P->b_x, R->psi_vxxs, R->ux, R->uy, R->uxy, R->uyz, R->psxz,
                                 R->uxz);
     }
+    exchange_s(R->psxx, R->psyy, R->psxy, R->psxz, R->psyz, R->buf_lr, R->buf_rl, R->buf_tb, R->buf_bt, NULL, NULL,
+               1);
 }
 
 static void copy_interior(float **a, float *out)

@@ -87,8 +87,12 @@ def test_engine_rejects_bad_config():
     cfg = ifos_amd.make_config(NX=50, NY=40, DH=1.0, DT=1e-4, NT=100, FDORDER=5, DTINV=2)
     with pytest.raises(ifos_amd.IfosError, match="FDORDER"):
         ifos_amd.Engine(cfg)
-    cfg = ifos_amd.make_config(NX=50, NY=40, DH=1.0, DT=1e-4, NT=100, FDORDER=4, DTINV=2, BOUNDARY=1)
+    cfg = ifos_amd.make_config(NX=50, NY=40, DH=1.0, DT=1e-4, NT=100, FDORDER=4, DTINV=2, BOUNDARY=2)
     with pytest.raises(ifos_amd.IfosError, match="BOUNDARY"):
+        ifos_amd.Engine(cfg)
+    # periodic sides copy N = FDORDER/2 edge columns into each halo
+    cfg = ifos_amd.make_config(NX=6, NY=40, DH=1.0, DT=1e-4, NT=100, FDORDER=8, FW=2, DTINV=2, BOUNDARY=1)
+    with pytest.raises(ifos_amd.IfosError, match="NX >= FDORDER"):
         ifos_amd.Engine(cfg)
 
 
