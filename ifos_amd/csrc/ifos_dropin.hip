@@ -319,7 +319,7 @@ StepGeom make_geom(int nx1, int nx2, int ny1, int ny2, int pitch, const float *h
 {
     if (nx1 != 1 || ny1 != 1) fatal("the device kernels update the whole grid: nx1 and ny1 must be 1");
     if (NPROCX != 1 || NPROCY != 1) fatal("one GPU holds the whole domain: NPROCX = NPROCY = 1 required");
-    if (BOUNDARY != 0) fatal("BOUNDARY must be 0: periodic sides need the wavefield exchanges of src/exchange_v.c");
+    if (BOUNDARY != 0 && BOUNDARY != 1) fatal("BOUNDARY must be 0 or 1");
     StepGeom g;
     g.NX = nx2; g.NY = ny2; g.pitch = pitch;
     g.FW = FW; g.FREE_SURF = FREE_SURF; g.BOUNDARY = BOUNDARY;
@@ -364,6 +364,38 @@ __global__ void k_psource_series(int n, const int *ix, const int *iy, const floa
         f1[c] = (float)((double)f1[c] + 0.5 * (double)a);
         if (f2) f2[c] = (float)((double)f2[c] + 0.5 * (double)a);
     }
+}
+
+// one-rank exchanges with BOUNDARY 1 (src/exchange_v.c:143-228,
+// src/exchange_s.c:145-229): each row's halo columns take the opposite
+// edge; a gets N columns on the right (NX+1..NX+N from 1..N) and N-1 on the
+// left (2-N..0 from NX-N+2..NX), b N-1 on the right and N on the left.  One
+// thread per row; the columns read and the columns written are disjoint.
+__global__ void k_exchange_x(float *a, float *b, int pitch, int NX, int NY, int N)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (j > NY) return;
+    float *ra = a + (long)j * pitch, *rb = b + (long)j * pitch;
+    for (int l = 1; l <= N; l++) ra[NX + l] = ra[l];
+    for (int l = 1; l <= N - 1; l++) rb[NX + l] = rb[l];
+    for (int l = 1; l <= N - 1; l++) ra[1 - l] = ra[NX - l + 1];
+    for (int l = 1; l <= N; l++) rb[1 - l] = rb[NX - l + 1];
+}
+
+void exchange_x(float **a, float **b, const char *na, const char *nb)
+{
+    const int N = FDORDER / 2;
+    if (NX < FDORDER) fatal("BOUNDARY 1 needs NX >= FDORDER");
+    int Pa, Pb;
+    float *oa = origin_of(a, &Pa, na), *ob = origin_of(b, &Pb, nb);
+    for (float **m : {a, b}) {
+        const DevArr *d = lookup(m);
+        if (d->nrl > 1 || d->nrh < NY || d->ncl > 1 - N || d->nch < NX + N)
+            fatal("exchange: the wavefield matrices need rows 1..NY and columns 1-FDORDER/2..NX+FDORDER/2");
+    }
+    if (Pa != Pb) fatal("exchange: the two wavefield matrices must have the same column range");
+    hipLaunchKernelGGL(k_exchange_x, dim3((NY + 255) / 256), dim3(256), 0, stream(), oa, ob, Pa, NX, NY, N);
+    hip_check(hipGetLastError(), "exchange launch");
 }
 
 __global__ void k_zero_rect(float *origin, int pitch, int j0, int j1, int i0, int i1)
@@ -892,6 +924,35 @@ extern "C" void seismo_ssg(int lsamp, int ntr, int **recpos, float **sectionvx, 
                        a, b, SEISMO == 1 ? 1 : 2, sb.dev, NT, lsamp - 1);
     sb.lo = sb.hi < sb.lo ? lsamp : std::min(sb.lo, lsamp);
     sb.hi = std::max(sb.hi, lsamp);
+}
+
+// ---------------------------------------------------------------------------
+// exchange_v / exchange_s (src/exchange_v.c:28, src/exchange_s.c:28,
+// src/fd.h:116-124) on the one rank that holds the whole grid: with
+// BOUNDARY 1 the rank is its own left and right neighbour and the halo
+// columns take the opposite edge (vy and sxx: N right, N-1 left; vx and sxy:
+// N-1 right, N left); with BOUNDARY 0 nothing changes, as in the reference,
+// where a single rank sends to itself and unpacks nothing.  The buffers and
+// requests are the driver's MPI ones and are not used; WAVETYPE 1 only.
+// ---------------------------------------------------------------------------
+extern "C" void exchange_v(float **vx, float **vy, float **vz, float **bufferlef_to_rig, float **bufferrig_to_lef,
+                           float **buffertop_to_bot, float **bufferbot_to_top, void *req_send, void *req_rec,
+                           int wavetyp_start)
+{
+    (void)vz; (void)bufferlef_to_rig; (void)bufferrig_to_lef; (void)buffertop_to_bot; (void)bufferbot_to_top;
+    (void)req_send; (void)req_rec;
+    if (wavetyp_start != 1) fatal("exchange_v: WAVETYPE 1 (P-SV) only");
+    if (BOUNDARY) exchange_x(vy, vx, "vy", "vx");
+}
+
+extern "C" void exchange_s(float **sxx, float **syy, float **sxy, float **sxz, float **syz, float **bufferlef_to_rig,
+                           float **bufferrig_to_lef, float **buffertop_to_bot, float **bufferbot_to_top,
+                           void *req_send, void *req_rec, int wavetyp_start)
+{
+    (void)syy; (void)sxz; (void)syz; (void)bufferlef_to_rig; (void)bufferrig_to_lef; (void)buffertop_to_bot;
+    (void)bufferbot_to_top; (void)req_send; (void)req_rec;
+    if (wavetyp_start != 1) fatal("exchange_s: WAVETYPE 1 (P-SV) only");
+    if (BOUNDARY) exchange_x(sxx, sxy, "sxx", "sxy");
 }
 
 // ---------------------------------------------------------------------------

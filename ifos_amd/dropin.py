@@ -168,14 +168,15 @@ class DropinShot:
             lib.zero_fdveps(*args)
 
     def step(self, nt, srcpos, signals, signals1, nsig, sw):
-        """update_v, update_s, psource, surface: src/IFOS2D.c:1682-1770 (sw 0)
-        and :2253-2360 (sw 1)"""
+        """update_v, exchange_v, update_s, psource, surface, exchange_s:
+        src/IFOS2D.c:1682-1785 (sw 0) and :2253-2366 (sw 1)"""
         lib, D, cfg, K = self.lib, self.D, self.cfg, self.K
         NX, NY = cfg.NX, cfg.NY
         hc = self.hc.ctypes.data_as(FP)
         lib.update_v_PML(1, NX, 1, NY, nt, D["vx"], D["vxp1"], D["vxm1"], D["vy"], D["vyp1"], D["vym1"], None, None,
                          D["sxx"], D["syy"], D["sxy"], D["rip"], D["rjp"], srcpos, signals, signals1, nsig, None, hc,
                          0, sw, *K, D["psi_sxx_x"], D["psi_syy_y"], D["psi_sxy_y"], D["psi_sxy_x"])
+        lib.exchange_v(D["vx"], D["vy"], None, None, None, None, None, None, None, 1)
         if cfg.L:
             v = self.vec
             lib.update_s_visc_PML(1, NX, 1, NY, D["vx"], D["vy"], D["ux"], D["uy"], D["uxy"], D["uyx"], D["sxx"],
@@ -198,6 +199,7 @@ class DropinShot:
                 lib.surface_elastic_PML(1, D["vx"], D["vy"], D["sxx"], D["syy"], D["sxy"], None, D["pi"], D["u"],
                                         D["rho"], hc, K[0], K[1], K[2], D["psi_vxxs"], D["ux"], D["uy"], D["uxy"],
                                         None, None, None)
+        lib.exchange_s(D["sxx"], D["syy"], D["sxy"], None, None, None, None, None, None, None, None, 1)
 
     def snaps(self):
         D = self.D
