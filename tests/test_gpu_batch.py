@@ -137,3 +137,20 @@ def test_batch_rejects_too_many_shots(gpu_lib):
     with pytest.raises(ifos_amd.IfosError):
         eng.set_launch(1, 12)
     eng.close()
+
+
+def test_kernel_and_copy_timing(gpu_lib):
+    """ifos_engine_time_kernel and ifos_engine_time_copy (the figures of
+    bench.py --kernels-only --copy-floor): positive times, a copy of 256 MB
+    takes longer than one of 4 KB, and the engine still runs afterwards."""
+    w = workloads.small_case(seed=2, NX=200, NY=120, NT=60)
+    eng = ifos_amd.Engine(w.cfg)
+    eng.set_model(**w.true)
+    eng.set_shot(w.shots[0].srcpos, w.shots[0].recpos)
+    for k in (0, 1):
+        assert eng.time_kernel(k, reps=3) > 0
+    small, large = eng.time_copy(4096, reps=5), eng.time_copy(256 << 20, reps=5)
+    assert 0 < small < large
+    eng.forward()  # the engine still runs after the timings
+    assert np.isfinite(eng.seismograms()[1]).all()
+    eng.close()
