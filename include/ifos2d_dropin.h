@@ -99,14 +99,22 @@ void seismo_ssg(int lsamp, int ntr, int **recpos, float **sectionvx, float **sec
 /* replace exchange_v() / exchange_s() src/exchange_v.c:28, src/exchange_s.c:28, src/fd.h:116-124,
  * on the one rank that holds the whole grid: with BOUNDARY 1 the halo
  * columns take the opposite edge (periodic sides), with BOUNDARY 0 nothing
- * changes.  The buffers and the MPI_Request pointers (void * here) are not
- * used; wavetyp_start must be 1. */
+ * changes.  The buffers and the request pointers are not used;
+ * wavetyp_start must be 1.  After mpi.h the prototypes are src/fd.h's own
+ * (MPI_Request *), so a driver may include both headers; without MPI the
+ * request pointers are plain pointers (the same calling convention). */
+#ifdef MPI_VERSION
+#define IFOS_MPI_REQUEST MPI_Request
+#else
+#define IFOS_MPI_REQUEST void
+#endif
 void exchange_v(float **vx, float **vy, float **vz, float **bufferlef_to_rig, float **bufferrig_to_lef,
-                float **buffertop_to_bot, float **bufferbot_to_top, void *req_send, void *req_rec,
-                int wavetyp_start);
+                float **buffertop_to_bot, float **bufferbot_to_top, IFOS_MPI_REQUEST *req_send,
+                IFOS_MPI_REQUEST *req_rec, int wavetyp_start);
 void exchange_s(float **sxx, float **syy, float **sxy, float **sxz, float **syz, float **bufferlef_to_rig,
-                float **bufferrig_to_lef, float **buffertop_to_bot, float **bufferbot_to_top, void *req_send,
-                void *req_rec, int wavetyp_start);
+                float **bufferrig_to_lef, float **buffertop_to_bot, float **bufferbot_to_top,
+                IFOS_MPI_REQUEST *req_send, IFOS_MPI_REQUEST *req_rec, int wavetyp_start);
+#undef IFOS_MPI_REQUEST
 
 /* replaces zero_fdveps()             src/zero_fdveps.c:28, src/fd.h:428-429 */
 void zero_fdveps(int ny1, int ny2, int nx1, int nx2, float **vx, float **vy, float **vz, float **sxx, float **syy,

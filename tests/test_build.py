@@ -46,6 +46,29 @@ def test_header_included_twice_and_together(tmp_path):
     _ok(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"), str(src)])
 
 
+MPI_INCLUDE = "/opt/conda/include"
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(MPI_INCLUDE, "mpi.h")), reason="no mpi.h in the image")
+def test_dropin_header_beside_the_reference_prototypes(tmp_path):
+    """A driver that includes mpi.h, the reference's prototypes of
+    exchange_v / exchange_s (src/fd.h:116-124, restated here) and the drop-in
+    header compiles: the declarations agree."""
+    src = tmp_path / "driver.c"
+    src.write_text(
+        "#include <mpi.h>\n"
+        "void exchange_v(float ** vx, float ** vy, float ** vz, float ** bufferlef_to_rig,\n"
+        "    float ** bufferrig_to_lef, float ** buffertop_to_bot, float ** bufferbot_to_top,\n"
+        "    MPI_Request * req_send, MPI_Request * req_rec, int wavetyp_start);\n"
+        "void exchange_s(float ** sxx, float ** syy, float ** sxy, float ** sxz, float ** syz,\n"
+        "    float ** bufferlef_to_rig, float ** bufferrig_to_lef, float ** buffertop_to_bot,\n"
+        "    float ** bufferbot_to_top, MPI_Request * req_send, MPI_Request * req_rec, int wavetyp_start);\n"
+        '#include "ifos2d_dropin.h"\n'
+        "int main(void){return 0;}\n")
+    _ok(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I" + MPI_INCLUDE,
+         "-I" + os.path.join(ROOT, "include"), str(src)])
+
+
 @pytest.mark.parametrize("client", ["dropin_forward.c", "engine_gradient.c"])
 def test_c_client_compiles(client):
     _ok(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"),
