@@ -556,8 +556,13 @@ __global__ __launch_bounds__(512) void k_vel_step(const StepArgs *__restrict__ p
 #pragma unroll
             for (int r = 0; r < RY; r++) {
                 live[r] = i <= g.NX && j + r <= g.NY;
-                rip[r] = live[r] ? gmem(a.rip)[c + r * g.pitch] : 0.0f;
-                rjp[r] = live[r] ? gmem(a.rjp)[c + r * g.pitch] : 0.0f;
+                // unconditional loads, then a select: live lanes read their
+                // own cell, the others cell (1, 1), which every grid has (no
+                // per-lane branch, so the pointers are loaded once)
+                const long ci = live[r] ? c + r * g.pitch : (long)g.pitch + 1;
+                const float vr = gmem(a.rip)[ci], vj = gmem(a.rjp)[ci];
+                rip[r] = live[r] ? vr : 0.0f;
+                rjp[r] = live[r] ? vj : 0.0f;
             }
 #pragma unroll
             for (int u = 0; u < SPB; u++)
